@@ -1,0 +1,163 @@
+"""Throughput benchmark of the MI355X playout path (one JSON line on stdout).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c0] [--no-cpu-baseline]
+
+A step is one launch of the persistent playout kernel over the staged batch of the workload:
+  c1 (default): 1024 seeded positions (k in [0, 30] random plies from the start) x 1024 random
+                playouts = 1,048,576 bit-exact playouts
+  c2          : 1024 seeded midgame positions (k in [20, 60], finished games redrawn) x 1024
+                heuristic playouts
+  c0          : 64 positions x 4096 random playouts
+Positions are generated on the device (mi_gen_start_positions) and staged in HBM once; each step
+uses its own epoch (a fresh Philox stream), so no step repeats another.
+
+roofline: the path is integer- and branch-bound.  Its work unit is one ply (generate, choose,
+apply, terminal test) priced at a frozen W = 192 int32 lane-ops; achieved = plies x W / average
+kernel time (HIP events on the library's stream), against the chip's int32 VALU issue rate
+CUs x 64 lanes x clock.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+recorded in profiles/ (null when absent).
+cpu_baseline: the reference's own host code (oracle/_ref/libref.so: State::genMoves twice per ply
+as src/playout.cpp:17-32 does, OpenMP over playouts, Philox draws) on a bounded sample of the same
+workload, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "playouts/s on 1 MI355X; win rate of net-leaf vs playout MCTS at equal time"
+W_PLY = 192                      # int32 lane-ops per ply (frozen cost model)
+SPEC_CUS, SPEC_CLOCK_HZ = 256, 2.4e9
+CONFIGS = {
+    "c1": dict(n=1024, per_pos=1024, kmin=0, kmax=30, reject=False, policy=0,
+               desc="config 1: 1024 seeded positions (k in [0,30] random plies) x 1024 bit-exact "
+                    "random playouts"),
+    "c2": dict(n=1024, per_pos=1024, kmin=20, kmax=60, reject=True, policy=1,
+               desc="config 2: 1024 seeded midgame positions (k in [20,60], finished games "
+                    "redrawn) x 1024 heuristic playouts"),
+    "c0": dict(n=64, per_pos=4096, kmin=0, kmax=30, reject=False, policy=0,
+               desc="config 0: 64 seeded positions x 4096 random playouts"),
+}
+POS_SEED, PLAY_SEED = 2016, 12345
+
+
+def read_traffic(config: str):
+    """HBM bytes per launch recorded from rocprofv3 --pmc runs (profiles/*traffic*.md)."""
+    path = os.path.join(ROOT, "profiles", f"playout_{config}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(cfg: dict, pos, threads: int) -> dict:
+    """Times the reference build (or the C restatement when that is absent) on a sample."""
+    from oracle import pyoracle
+    kind, lib = "reference", None
+    if os.path.exists(pyoracle.REF_SO):
+        lib = pyoracle.Reference(pyoracle.REF_SO)
+    else:
+        kind, lib = "port", pyoracle.Oracle()
+    npos = min(len(pos), 256 if cfg["policy"] == 0 else 128)
+    per = min(cfg["per_pos"], 1024)
+    t0 = time.perf_counter()
+    lib.run_playouts(pos[:npos], per_pos=per, seed=PLAY_SEED, epoch=0, policy=cfg["policy"],
+                     threads=threads, traces=False)
+    dt = time.perf_counter() - t0
+    return {"value": round(npos * per / dt, 1), "unit": "playouts/s", "cores": threads,
+            "kind": kind,
+            "sample": f"first {npos} positions x {per} playouts of the workload "
+                      f"({npos * per} playouts, {dt:.2f} s wall on {threads} threads)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the playout path runs on a single MI355X here "
+              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        return 2
+
+    from gpu_ai_amd.engine import Engine
+
+    cfg = CONFIGS[args.config]
+    eng = Engine(0)
+    info = eng.device_info()
+    pos = eng.start_positions(POS_SEED, cfg["n"], cfg["kmin"], cfg["kmax"], cfg["reject"])
+    total = eng.stage(pos, per_pos=cfg["per_pos"])
+    for i in range(args.warmup):
+        eng.launch_staged(seed=PLAY_SEED, epoch=1000 + i, policy=cfg["policy"])
+
+    # every launch returns after its HIP stream has synchronised, so the device is idle at both
+    # ends of the timed region
+    kernel_ms, plies = [], 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = eng.launch_staged(seed=PLAY_SEED, epoch=i, policy=cfg["policy"])
+        kernel_ms.append(st["kernel_ms"])
+        plies += st["plies"]
+    elapsed = time.perf_counter() - t0
+    tuning = eng.tuning
+
+    value = total * args.steps / elapsed
+    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+    plies_per_launch = plies / args.steps
+    clock_hz = info["clock_khz"] * 1e3 if info["clock_khz"] else SPEC_CLOCK_HZ
+    peak_dev = info["compute_units"] * 64 * clock_hz
+    peak_spec = SPEC_CUS * 64 * SPEC_CLOCK_HZ
+    achieved = plies_per_launch * W_PLY / avg_kernel_s
+    traffic = read_traffic(args.config)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "playouts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded positions generated on the device)",
+        "config": {"workload": cfg["desc"], "positions": cfg["n"],
+                   "playouts_per_position": cfg["per_pos"], "playouts_per_step": total,
+                   "policy": "random" if cfg["policy"] == 0 else "heuristic",
+                   "position_seed": POS_SEED, "playout_seed": PLAY_SEED,
+                   "block_size": tuning["block_size"], "blocks_per_cu": tuning["blocks_per_cu"],
+                   "refill": "queue" if tuning["refill"] == 1 else "static"},
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 4),
+                     "peak": round(peak_spec / 1e12, 3), "unit": "Tops/s (int32 lane-ops)",
+                     "frac": round(achieved / peak_spec, 4), "traffic": traffic,
+                     "kernel": "k_playouts", "avg_kernel_ms": round(avg_kernel_s * 1e3, 4),
+                     "plies_per_launch": int(plies_per_launch), "ops_per_ply": W_PLY,
+                     "peak_device_clock": round(peak_dev / 1e12, 3),
+                     "device": {"compute_units": info["compute_units"],
+                                "clock_mhz": info["clock_khz"] / 1e3}},
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            line["cpu_baseline"] = cpu_baseline(cfg, pos, threads)
+        except Exception as e:  # the baseline is reported, never required for the GPU figure
+            line["cpu_baseline"] = {"value": None, "error": str(e)}
+    eng.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

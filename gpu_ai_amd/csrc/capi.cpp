@@ -1,0 +1,496 @@
+// capi.cpp -- the C ABI of include/gpuai/mi_playout.h.
+//
+// A context owns one HIP stream, two timing events and device buffers that grow on demand and
+// are reused across calls (the reference allocates, copies 776 bytes per state and frees on
+// every call, src/singlePlayout.cu:76-120).  Calls on one context are serialised by a mutex;
+// separate contexts are independent, so each MCTS worker thread can own one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gpuai/mi_playout.h"
+#include "kernels.h"
+
+namespace {
+
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct mi_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  mi_tuning tuning = {256u, 8u, MI_REFILL_QUEUE, 0u};
+  uint32_t cus = 0, wave = 64, clock_khz = 0;
+  DevBuf pos, offs, winners, plies, hashes, work, tallies, moves, counts;
+  // staged batch
+  uint32_t staged_npos = 0, staged_per_pos = 0, staged_total = 0;
+  bool staged = false, staged_offs = false;
+  std::vector<uint32_t> host_offs;
+  std::string last_error;
+  std::mutex mu;
+};
+
+namespace {
+
+int fail(mi_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->last_error = msg;
+  return code;
+}
+
+int hip_fail(mi_ctx *ctx, hipError_t e, const char *what) {
+  std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+  int code = (e == hipErrorOutOfMemory) ? MI_ERR_OUT_OF_MEMORY : MI_ERR_DEVICE;
+  return fail(ctx, code, msg);
+}
+
+#define MI_HIP(ctx, call)                                 \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+  } while (0)
+
+int ensure(mi_ctx *ctx, DevBuf &b, size_t bytes) {
+  if (bytes <= b.bytes && b.ptr) return MI_OK;
+  if (b.ptr) {
+    hipError_t e = hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipFree");
+  }
+  size_t want = std::max<size_t>(bytes, 256);
+  want = (want + 4095) & ~(size_t)4095;
+  hipError_t e = hipMalloc(&b.ptr, want);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+  b.bytes = want;
+  return MI_OK;
+}
+
+int check_tuning(mi_ctx *ctx, const mi_tuning &t) {
+  if (t.block_size < 64 || t.block_size > 1024 || t.block_size % 64)
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 1024]");
+  if (t.blocks_per_cu < 1 || t.blocks_per_cu > 64)
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 64]");
+  if (t.refill != MI_REFILL_STATIC && t.refill != MI_REFILL_QUEUE)
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "unknown refill policy");
+  return MI_OK;
+}
+
+// Validates a batch and computes playout offsets; total must stay below 2^31.
+int plan_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+               uint32_t per_pos, uint64_t &total) {
+  if (npos > 0 && !pos) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "positions pointer is null");
+  total = 0;
+  if (counts) {
+    ctx->host_offs.resize((size_t)npos + 1);
+    ctx->host_offs[0] = 0;
+    for (uint32_t i = 0; i < npos; i++) {
+      total += counts[i];
+      if (total >= (1ull << 31)) break;
+      ctx->host_offs[i + 1] = (uint32_t)total;
+    }
+  } else {
+    total = (uint64_t)npos * per_pos;
+  }
+  if (total >= (1ull << 31))
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "more than 2^31-1 playouts in one call");
+  for (uint32_t i = 0; i < npos; i++) {
+    const mi_pos &q = pos[i];
+    if ((q.p1 & q.p2) || q.turn > 1)
+      return fail(ctx, MI_ERR_INVALID_ARGUMENT,
+                  "position " + std::to_string(i) + " is malformed (overlapping sides or turn > 1)");
+  }
+  return MI_OK;
+}
+
+int upload_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                 uint64_t total) {
+  int rc;
+  if ((rc = ensure(ctx, ctx->pos, sizeof(mi_pos) * (size_t)std::max<uint32_t>(npos, 1))))
+    return rc;
+  if ((rc = ensure(ctx, ctx->winners, (size_t)total))) return rc;
+  if ((rc = ensure(ctx, ctx->work, sizeof(gpuai::WorkWords)))) return rc;
+  MI_HIP(ctx, hipMemcpyAsync(ctx->pos.ptr, pos, sizeof(mi_pos) * npos, hipMemcpyHostToDevice,
+                             ctx->stream));
+  if (counts) {
+    if ((rc = ensure(ctx, ctx->offs, sizeof(uint32_t) * ((size_t)npos + 1)))) return rc;
+    MI_HIP(ctx, hipMemcpyAsync(ctx->offs.ptr, ctx->host_offs.data(),
+                               sizeof(uint32_t) * ((size_t)npos + 1), hipMemcpyHostToDevice,
+                               ctx->stream));
+  }
+  return MI_OK;
+}
+
+uint32_t grid_for(const mi_ctx *ctx, uint64_t total) {
+  const uint64_t block = ctx->tuning.block_size;
+  const uint64_t need = (total + block - 1) / block;
+  const uint64_t resident = (uint64_t)ctx->cus * ctx->tuning.blocks_per_cu;
+  return (uint32_t)std::max<uint64_t>(1, std::min(need, resident));
+}
+
+// Launches the playout kernel on the staged device buffers and reads back the statistics.
+int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, bool with_offs,
+               uint32_t seed, uint32_t epoch, uint32_t policy, bool trace, mi_stats *stats) {
+  gpuai::PlayoutArgs a;
+  a.pos = (const mi_pos *)ctx->pos.ptr;
+  a.offs = with_offs ? (const uint32_t *)ctx->offs.ptr : nullptr;
+  a.npos = npos;
+  a.per_pos = per_pos;
+  a.total = (uint32_t)total;
+  a.seed = seed;
+  a.epoch = epoch;
+  a.winners = (int8_t *)ctx->winners.ptr;
+  a.plies = trace ? (uint32_t *)ctx->plies.ptr : nullptr;
+  a.hashes = trace ? (uint32_t *)ctx->hashes.ptr : nullptr;
+  a.work = (gpuai::WorkWords *)ctx->work.ptr;
+  const uint32_t grid = grid_for(ctx, total);
+  MI_HIP(ctx, gpuai::launch_reset(a.work, ctx->stream));
+  MI_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+  MI_HIP(ctx, gpuai::launch_playouts(a, policy, trace, ctx->tuning.refill, grid,
+                                     ctx->tuning.block_size, ctx->stream));
+  MI_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+  gpuai::WorkWords w;
+  MI_HIP(ctx, hipMemcpyAsync(&w, a.work, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0.f;
+  MI_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+  if (stats) {
+    stats->playouts = total;
+    stats->plies = w.plies;
+    stats->kernel_ms = ms;
+    stats->max_moves = w.max_moves;
+    stats->grid_blocks = grid;
+    stats->block_size = ctx->tuning.block_size;
+    stats->refill = ctx->tuning.refill;
+  }
+  return MI_OK;
+}
+
+int bad_policy(mi_ctx *ctx, uint32_t policy) {
+  if (policy != MI_POLICY_RANDOM && policy != MI_POLICY_HEURISTIC)
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "unknown policy");
+  return MI_OK;
+}
+
+using Clock = std::chrono::steady_clock;
+
+void finish_stats(mi_stats *stats, Clock::time_point t0) {
+  if (stats)
+    stats->total_ms = std::chrono::duration<float, std::milli>(Clock::now() - t0).count();
+}
+
+int run_common(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+               uint32_t per_pos, uint32_t seed, uint32_t epoch, uint32_t policy, bool trace,
+               int8_t *winners, uint32_t *plies, uint32_t *hashes, uint32_t *tallies,
+               mi_stats *stats) {
+  if (!ctx) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const auto t0 = Clock::now();
+  ctx->last_error.clear();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  int rc;
+  if ((rc = bad_policy(ctx, policy))) return rc;
+  uint64_t total = 0;
+  if ((rc = plan_batch(ctx, pos, npos, counts, per_pos, total))) return rc;
+  if (tallies && npos) std::memset(tallies, 0, sizeof(uint32_t) * 3 * (size_t)npos);
+  if (total == 0) {  // src/singlePlayout.cu:73-75: an empty batch returns at once
+    finish_stats(stats, t0);
+    return MI_OK;
+  }
+  if (!winners && !tallies) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "winners pointer is null");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  if ((rc = upload_batch(ctx, pos, npos, counts, total))) return rc;
+  if (trace) {
+    if ((rc = ensure(ctx, ctx->plies, sizeof(uint32_t) * total))) return rc;
+    if ((rc = ensure(ctx, ctx->hashes, sizeof(uint32_t) * total))) return rc;
+  }
+  ctx->staged = false;  // the batch buffers now hold this call's data
+  if ((rc = run_kernel(ctx, npos, per_pos, total, counts != nullptr, seed, epoch, policy, trace,
+                       stats)))
+    return rc;
+  if (winners)
+    MI_HIP(ctx, hipMemcpyAsync(winners, ctx->winners.ptr, total, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  if (trace && plies)
+    MI_HIP(ctx, hipMemcpyAsync(plies, ctx->plies.ptr, sizeof(uint32_t) * total,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  if (trace && hashes)
+    MI_HIP(ctx, hipMemcpyAsync(hashes, ctx->hashes.ptr, sizeof(uint32_t) * total,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  if (tallies) {
+    if ((rc = ensure(ctx, ctx->tallies, sizeof(uint32_t) * 3 * (size_t)npos))) return rc;
+    MI_HIP(ctx, gpuai::launch_tally((const int8_t *)ctx->winners.ptr,
+                                    counts ? (const uint32_t *)ctx->offs.ptr : nullptr, npos,
+                                    per_pos, (uint32_t *)ctx->tallies.ptr, ctx->stream));
+    MI_HIP(ctx, hipMemcpyAsync(tallies, ctx->tallies.ptr, sizeof(uint32_t) * 3 * (size_t)npos,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  }
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  finish_stats(stats, t0);
+  return MI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mi_strerror(int code) {
+  switch (code) {
+    case MI_OK: return "ok";
+    case MI_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case MI_ERR_NO_DEVICE: return "no HIP device";
+    case MI_ERR_DEVICE: return "device error";
+    case MI_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case MI_ERR_CAPACITY: return "move list longer than the capacity";
+    case MI_ERR_NOT_STAGED: return "no staged batch";
+    default: return "unknown error";
+  }
+}
+
+const char *mi_last_error(const mi_ctx *ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+int mi_create(mi_ctx **out, int device, const mi_tuning *tuning) {
+  if (!out) return MI_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MI_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return MI_ERR_INVALID_ARGUMENT;
+  mi_ctx *ctx = new mi_ctx();
+  ctx->device = device;
+  if (tuning) {
+    mi_tuning t = *tuning;
+    if (!t.block_size) t.block_size = ctx->tuning.block_size;
+    if (!t.blocks_per_cu) t.blocks_per_cu = ctx->tuning.blocks_per_cu;
+    if (check_tuning(ctx, t)) {
+      delete ctx;
+      return MI_ERR_INVALID_ARGUMENT;
+    }
+    ctx->tuning = t;
+  }
+  hipDeviceProp_t prop;
+  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess) {
+    mi_destroy(ctx);
+    return MI_ERR_DEVICE;
+  }
+  ctx->cus = (uint32_t)prop.multiProcessorCount;
+  ctx->wave = (uint32_t)prop.warpSize;
+  ctx->clock_khz = (uint32_t)prop.clockRate;
+  *out = ctx;
+  return MI_OK;
+}
+
+int mi_destroy(mi_ctx *ctx) {
+  if (!ctx) return MI_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  DevBuf *bufs[] = {&ctx->pos,     &ctx->offs,    &ctx->winners, &ctx->plies, &ctx->hashes,
+                    &ctx->work,    &ctx->tallies, &ctx->moves,   &ctx->counts};
+  for (DevBuf *b : bufs)
+    if (b->ptr) (void)hipFree(b->ptr);
+  if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+  if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MI_OK;
+}
+
+int mi_set_tuning(mi_ctx *ctx, const mi_tuning *tuning) {
+  if (!ctx || !tuning) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  mi_tuning t = *tuning;
+  if (!t.block_size) t.block_size = 256;
+  if (!t.blocks_per_cu) t.blocks_per_cu = 8;
+  int rc = check_tuning(ctx, t);
+  if (rc) return rc;
+  ctx->tuning = t;
+  return MI_OK;
+}
+
+int mi_get_tuning(const mi_ctx *ctx, mi_tuning *tuning) {
+  if (!ctx || !tuning) return MI_ERR_INVALID_ARGUMENT;
+  *tuning = ctx->tuning;
+  return MI_OK;
+}
+
+int mi_device_info(const mi_ctx *ctx, uint32_t *compute_units, uint32_t *wave_size,
+                   uint32_t *clock_khz) {
+  if (!ctx) return MI_ERR_INVALID_ARGUMENT;
+  if (compute_units) *compute_units = ctx->cus;
+  if (wave_size) *wave_size = ctx->wave;
+  if (clock_khz) *clock_khz = ctx->clock_khz;
+  return MI_OK;
+}
+
+int mi_run_playouts(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                    uint32_t per_pos, uint32_t seed, uint32_t epoch, uint32_t policy,
+                    int8_t *winners, mi_stats *stats) {
+  return run_common(ctx, pos, npos, counts, per_pos, seed, epoch, policy, false, winners, nullptr,
+                    nullptr, nullptr, stats);
+}
+
+int mi_run_playouts_traced(mi_ctx *ctx, const mi_pos *pos, uint32_t npos,
+                           const uint32_t *counts, uint32_t per_pos, uint32_t seed,
+                           uint32_t epoch, uint32_t policy, int8_t *winners, uint32_t *plies,
+                           uint32_t *hashes, mi_stats *stats) {
+  return run_common(ctx, pos, npos, counts, per_pos, seed, epoch, policy, true, winners, plies,
+                    hashes, nullptr, stats);
+}
+
+int mi_run_playouts_tally(mi_ctx *ctx, const mi_pos *pos, uint32_t npos,
+                          const uint32_t *counts, uint32_t per_pos, uint32_t seed,
+                          uint32_t epoch, uint32_t policy, uint32_t *tallies, mi_stats *stats) {
+  if (!tallies && npos) return MI_ERR_INVALID_ARGUMENT;
+  return run_common(ctx, pos, npos, counts, per_pos, seed, epoch, policy, false, nullptr, nullptr,
+                    nullptr, tallies, stats);
+}
+
+int mi_gen_moves(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, mi_move *moves, uint32_t cap,
+                 uint32_t *counts) {
+  if (!ctx) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  if (npos == 0) return MI_OK;
+  if (!pos || !moves || !counts || cap == 0)
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "null buffer or zero capacity");
+  uint64_t dummy = 0;
+  int rc;
+  if ((rc = plan_batch(ctx, pos, npos, nullptr, 0, dummy))) return rc;
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  const size_t move_bytes = sizeof(mi_move) * (size_t)npos * cap;
+  if ((rc = ensure(ctx, ctx->pos, sizeof(mi_pos) * npos))) return rc;
+  if ((rc = ensure(ctx, ctx->moves, move_bytes))) return rc;
+  if ((rc = ensure(ctx, ctx->counts, sizeof(uint32_t) * npos))) return rc;
+  if ((rc = ensure(ctx, ctx->work, sizeof(gpuai::WorkWords)))) return rc;
+  ctx->staged = false;
+  MI_HIP(ctx, hipMemcpyAsync(ctx->pos.ptr, pos, sizeof(mi_pos) * npos, hipMemcpyHostToDevice,
+                             ctx->stream));
+  gpuai::WorkWords *work = (gpuai::WorkWords *)ctx->work.ptr;
+  MI_HIP(ctx, gpuai::launch_reset(work, ctx->stream));
+  MI_HIP(ctx, gpuai::launch_gen_moves((const mi_pos *)ctx->pos.ptr, npos, (mi_move *)ctx->moves.ptr,
+                                      cap, (uint32_t *)ctx->counts.ptr, work, ctx->stream));
+  gpuai::WorkWords w;
+  MI_HIP(ctx, hipMemcpyAsync(&w, work, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
+  MI_HIP(ctx, hipMemcpyAsync(counts, ctx->counts.ptr, sizeof(uint32_t) * npos,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  MI_HIP(ctx, hipMemcpyAsync(moves, ctx->moves.ptr, move_bytes, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (w.overflow) return fail(ctx, MI_ERR_CAPACITY, "a move list is longer than cap");
+  return MI_OK;
+}
+
+int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin, uint32_t kmax,
+                           uint32_t reject_terminal, mi_pos *out) {
+  if (!ctx) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  if (n == 0) return MI_OK;
+  if (!out || kmax < kmin) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "null output or kmax < kmin");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  int rc;
+  if ((rc = ensure(ctx, ctx->pos, sizeof(mi_pos) * (size_t)n))) return rc;
+  if ((rc = ensure(ctx, ctx->work, sizeof(gpuai::WorkWords)))) return rc;
+  ctx->staged = false;
+  gpuai::WorkWords *work = (gpuai::WorkWords *)ctx->work.ptr;
+  MI_HIP(ctx, gpuai::launch_reset(work, ctx->stream));
+  MI_HIP(ctx, gpuai::launch_start_positions(seed, n, kmin, kmax, reject_terminal,
+                                            (mi_pos *)ctx->pos.ptr, work, ctx->stream));
+  gpuai::WorkWords w;
+  MI_HIP(ctx, hipMemcpyAsync(&w, work, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
+  MI_HIP(ctx, hipMemcpyAsync(out, ctx->pos.ptr, sizeof(mi_pos) * (size_t)n, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (w.overflow) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "no unfinished position in 4096 attempts");
+  return MI_OK;
+}
+
+int mi_stage(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+             uint32_t per_pos) {
+  if (!ctx) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  uint64_t total = 0;
+  int rc;
+  if ((rc = plan_batch(ctx, pos, npos, counts, per_pos, total))) return rc;
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  if (total > 0 && (rc = upload_batch(ctx, pos, npos, counts, total))) return rc;
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->staged = true;
+  ctx->staged_offs = counts != nullptr;
+  ctx->staged_npos = npos;
+  ctx->staged_per_pos = per_pos;
+  ctx->staged_total = (uint32_t)total;
+  return MI_OK;
+}
+
+int mi_launch_staged(mi_ctx *ctx, uint32_t seed, uint32_t epoch, uint32_t policy,
+                     mi_stats *stats) {
+  if (!ctx) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const auto t0 = Clock::now();
+  ctx->last_error.clear();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  int rc;
+  if ((rc = bad_policy(ctx, policy))) return rc;
+  if (!ctx->staged) return fail(ctx, MI_ERR_NOT_STAGED, "mi_stage was not called");
+  if (ctx->staged_total == 0) return MI_OK;
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  rc = run_kernel(ctx, ctx->staged_npos, ctx->staged_per_pos, ctx->staged_total, ctx->staged_offs,
+                  seed, epoch, policy, false, stats);
+  finish_stats(stats, t0);
+  return rc;
+}
+
+int mi_fetch_winners(mi_ctx *ctx, int8_t *winners, uint64_t n) {
+  if (!ctx || (!winners && n)) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (!ctx->staged) return fail(ctx, MI_ERR_NOT_STAGED, "mi_stage was not called");
+  if (n > ctx->staged_total) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "n exceeds the staged batch");
+  if (n == 0) return MI_OK;
+  MI_HIP(ctx, hipMemcpyAsync(winners, ctx->winners.ptr, n, hipMemcpyDeviceToHost, ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MI_OK;
+}
+
+int mi_check_divmod(mi_ctx *ctx, const uint32_t *r, const uint32_t *n, uint32_t count,
+                    uint32_t *q, uint32_t *rem) {
+  if (!ctx) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  if (count == 0) return MI_OK;
+  if (!r || !n || !q || !rem) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "null buffer");
+  for (uint32_t i = 0; i < count; i++)
+    if (n[i] == 0) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "zero divisor");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  DevBuf buf;
+  const size_t bytes = sizeof(uint32_t) * (size_t)count;
+  MI_HIP(ctx, hipMalloc(&buf.ptr, 4 * bytes));
+  uint32_t *d = (uint32_t *)buf.ptr;
+  int rc = MI_OK;
+  hipError_t e = hipMemcpyAsync(d, r, bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d + count, n, bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = gpuai::launch_divmod(d, d + count, count, d + 2 * (size_t)count,
+                                                d + 3 * (size_t)count, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(q, d + 2 * (size_t)count, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(rem, d + 3 * (size_t)count, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) rc = hip_fail(ctx, e, "divmod check");
+  (void)hipFree(buf.ptr);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,652 @@
+// checkers.h -- 8x8 checkers on 32-bit bitboards, one position per lane.
+//
+// The game rules are those of gpu_ai's host path (src/state.cu, src/state.cpp,
+// src/heuristic.cu); the representation is this project's own.  A position is held in the
+// frame of the side to move ("mover frame"): `own` are the mover's pieces, `opp` the
+// opponent's, and the mover's men advance toward higher rows.  Player 1's absolute frame is the
+// mover frame; player 2's is the 180-degree rotation, which maps square s to 31 - s and is a
+// single bit reversal.
+//
+// Reference move order and the mover frame.  State::genMoves lists moves square by square in
+// index order (src/state.cu:171-180).  Within a square, direct moves go (+1,+1), (+1,-1),
+// (-1,+1), (-1,-1) restricted to the piece's directions (src/state.cu:255-281); capture
+// sequences are the leaves of a depth-first search that tries a man's (f,-1) jump before its
+// (f,+1) jump (src/state.cu:283-340) and a king's four jumps in the direction order above
+// (src/state.cu:388-420).  Rotating the board by 180 degrees reverses the square order and
+// reverses every per-square and per-node order, so for player 2 the list enumerated in the
+// mover frame with player 1's orders is exactly the reference list reversed: index k of the
+// reference list is index n-1-k here.
+//
+// Square geometry: s = 4*row + i, i = s & 3; col = 2i + 1 on even rows and 2i on odd rows.
+// One-step neighbours differ by +5/+4 (even/odd row) for (+1,+1), +4/+3 for (+1,-1), -3/-4 for
+// (-1,+1) and -4/-5 for (-1,-1); two-step jumps by +9, +7, -7, -9 on every row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CK_HD __host__ __device__ __forceinline__
+
+namespace ck {
+
+// ------------------------------------------------------------------------------------------
+// bit helpers
+// ------------------------------------------------------------------------------------------
+CK_HD uint32_t popc(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
+CK_HD uint32_t ctz(uint32_t x) { return (uint32_t)__builtin_ctz(x); }
+CK_HD uint32_t brev(uint32_t x) { return __builtin_bitreverse32(x); }
+CK_HD uint32_t bit(uint32_t s) { return 1u << s; }
+CK_HD uint32_t getbit(uint32_t m, uint32_t s) { return (m >> s) & 1u; }
+
+// ------------------------------------------------------------------------------------------
+// geometry
+// ------------------------------------------------------------------------------------------
+constexpr uint32_t EVEN_ROWS = 0x0F0F0F0Fu;  // rows 0, 2, 4, 6
+constexpr uint32_t ODD_ROWS = 0xF0F0F0F0u;   // rows 1, 3, 5, 7
+constexpr uint32_t I0 = 0x11111111u;         // i == 0
+constexpr uint32_t I3 = 0x88888888u;         // i == 3
+constexpr uint32_t ROW6 = 0x0F000000u;
+constexpr uint32_t ROW7 = 0xF0000000u;       // the mover's far row (promotion)
+constexpr uint32_t FAR_ROW_FIRST = 28u;
+
+// Squares reached by one step from the set b, per direction (off-board steps vanish).
+CK_HD uint32_t step0(uint32_t b) { return ((b & (EVEN_ROWS & ~I3)) << 5) | ((b & ODD_ROWS) << 4); }
+CK_HD uint32_t step1(uint32_t b) { return ((b & EVEN_ROWS) << 4) | ((b & (ODD_ROWS & ~I0)) << 3); }
+CK_HD uint32_t step2(uint32_t b) { return ((b & (EVEN_ROWS & ~I3)) >> 3) | ((b & ODD_ROWS) >> 4); }
+CK_HD uint32_t step3(uint32_t b) { return ((b & EVEN_ROWS) >> 4) | ((b & (ODD_ROWS & ~I0)) >> 5); }
+
+// Jump displacement of direction d: (+1,+1) +9, (+1,-1) +7, (-1,+1) -7, (-1,-1) -9.
+CK_HD int jump_off(uint32_t d) {
+  int j = ((d ^ (d >> 1)) & 1u) ? 7 : 9;
+  return (d & 2u) ? -j : j;
+}
+// Square one step from x in direction d (the step is half the jump, rounded by row parity).
+CK_HD uint32_t step_to(uint32_t x, uint32_t d) {
+  int j = jump_off(d);
+  int odd = (int)((x >> 2) & 1u);
+  return (uint32_t)((int)x + ((j + 1 - 2 * odd) >> 1));
+}
+// Square jumped when moving from x to land (two rows apart).
+CK_HD uint32_t mid_sq(uint32_t x, uint32_t land) { return ((x + land + 1u) >> 1) - ((x >> 2) & 1u); }
+
+// ------------------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+// ------------------------------------------------------------------------------------------
+CK_HD void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                  uint32_t out[4]) {
+#pragma unroll
+  for (int round = 0; round < 10; round++) {
+    uint64_t a = (uint64_t)0xD2511F53u * c0;
+    uint64_t b = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0;
+    uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)b;
+    c3 = (uint32_t)a;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+constexpr uint32_t TAG_PLAYOUT = 0u;
+constexpr uint32_t TAG_START = 1u;
+constexpr uint32_t TAG_SEARCH = 2u;
+constexpr uint32_t TAG_MLP = 3u;
+
+// Exact r mod n for 1 <= n.  On the device: one double-precision reciprocal estimate, a floor,
+// an exact fused remainder, and one correction each way (the estimate's relative error is far
+// below 2^-33, so the floor is off by at most one).
+CK_HD uint32_t mod_u32(uint32_t r, uint32_t n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double dn = (double)n, dr = (double)r;
+  double q = __builtin_floor(dr * __builtin_amdgcn_rcp(dn));
+  int32_t rem = (int32_t)__builtin_fma(-q, dn, dr);
+  rem += (rem < 0) ? (int32_t)n : 0;
+  rem -= (rem >= (int32_t)n) ? (int32_t)n : 0;
+  return (uint32_t)rem;
+#else
+  return r % n;
+#endif
+}
+
+// Exact quotient and remainder of r / n for 1 <= n (same estimate-and-correct scheme).
+CK_HD void divmod_u32(uint32_t r, uint32_t n, uint32_t &q, uint32_t &rem) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double dn = (double)n, dr = (double)r;
+  double qd = __builtin_floor(dr * __builtin_amdgcn_rcp(dn));
+  int32_t m = (int32_t)__builtin_fma(-qd, dn, dr);
+  uint32_t qq = (uint32_t)(int64_t)qd;
+  if (m < 0) {
+    m += (int32_t)n;
+    qq -= 1u;
+  } else if (m >= (int32_t)n) {
+    m -= (int32_t)n;
+    qq += 1u;
+  }
+  q = qq;
+  rem = (uint32_t)m;
+#else
+  q = r / n;
+  rem = r % n;
+#endif
+}
+
+// ------------------------------------------------------------------------------------------
+// positions
+// ------------------------------------------------------------------------------------------
+struct Lane {
+  uint32_t own, opp, kings;  // mover frame
+  uint32_t turn;             // absolute side to move: 0 player 1, 1 player 2
+  uint32_t counter;          // plies since the last capture
+};
+
+CK_HD Lane lane_from_abs(uint32_t p1, uint32_t p2, uint32_t kings, uint32_t turn,
+                         uint32_t counter) {
+  Lane L;
+  if (turn == 0) {
+    L.own = p1;
+    L.opp = p2;
+    L.kings = kings & (p1 | p2);
+  } else {
+    L.own = brev(p2);
+    L.opp = brev(p1);
+    L.kings = brev(kings & (p1 | p2));
+  }
+  L.turn = turn;
+  L.counter = counter;
+  return L;
+}
+
+CK_HD void lane_to_abs(const Lane &L, uint32_t &p1, uint32_t &p2, uint32_t &kings) {
+  if (L.turn == 0) {
+    p1 = L.own;
+    p2 = L.opp;
+    kings = L.kings;
+  } else {
+    p1 = brev(L.opp);
+    p2 = brev(L.own);
+    kings = brev(L.kings);
+  }
+}
+
+// Absolute square of a mover-frame square.
+CK_HD uint32_t abs_sq(const Lane &L, uint32_t s) { return L.turn ? 31u - s : s; }
+
+// ------------------------------------------------------------------------------------------
+// move-list analysis of one position
+// ------------------------------------------------------------------------------------------
+// cap[d]  : squares from which a jump in direction d is valid on the unmodified board (an
+//           opponent piece one step away, an empty square two steps away; src/state.cu:122-132)
+// slot[0..3]: per-square move sources in reference order, for every square that has no
+//           multi-jump: captures (men: (+1,-1) then (+1,+1); kings: directions 0..3) when
+//           captures exist, else direct moves (directions 0..3; men only 0 and 1)
+// multi   : capture origins with a sequence longer than one jump (need the tree walk)
+struct Gen {
+  uint32_t cap[4];
+  uint32_t slot[4];
+  uint32_t multi;
+  uint32_t capture;  // captures are mandatory this ply (src/state.cu:239-245)
+  uint32_t origins;  // capture origins (all, in square order)
+};
+
+CK_HD void analyse(const Lane &L, Gen &g) {
+  const uint32_t empty = ~(L.own | L.opp);
+  const uint32_t men = L.own & ~L.kings;
+  const uint32_t kng = L.own & L.kings;
+  g.cap[0] = step3(L.opp & step3(empty));
+  g.cap[1] = step2(L.opp & step2(empty));
+  g.cap[2] = step1(L.opp & step1(empty));
+  g.cap[3] = step0(L.opp & step0(empty));
+  const uint32_t fwd = g.cap[0] | g.cap[1];
+  const uint32_t any = fwd | g.cap[2] | g.cap[3];
+  const uint32_t csA = (men & g.cap[1]) | (kng & g.cap[0]);
+  const uint32_t csB = (men & g.cap[0]) | (kng & g.cap[1]);
+  const uint32_t csC = kng & g.cap[2];
+  const uint32_t csD = kng & g.cap[3];
+  g.origins = csA | csB | csC | csD;
+  g.capture = g.origins != 0u;
+  // An origin needs the walk if a first jump lands on a square with a further valid jump.  For
+  // a king the jump straight back would land on its own (occupied) origin, so it never counts.
+  g.multi = (men & ((g.cap[1] & (fwd >> 7)) | (g.cap[0] & (fwd >> 9)))) |
+            (kng & ((g.cap[0] & (any >> 9)) | (g.cap[1] & (any >> 7)) | (g.cap[2] & (any << 7)) |
+                    (g.cap[3] & (any << 9))));
+  if (g.capture) {
+    g.slot[0] = csA;
+    g.slot[1] = csB;
+    g.slot[2] = csC;
+    g.slot[3] = csD;
+  } else {
+    g.slot[0] = L.own & step3(empty);
+    g.slot[1] = L.own & step2(empty);
+    g.slot[2] = kng & step1(empty);
+    g.slot[3] = kng & step0(empty);
+  }
+}
+
+CK_HD uint32_t slot_count(const uint32_t s[4]) { return popc(s[0]) + popc(s[1]) + popc(s[2]) + popc(s[3]); }
+
+// Direction of slot `sl` for the piece on square `sq`.
+CK_HD uint32_t slot_dir(const Lane &L, const Gen &g, uint32_t sq, uint32_t sl) {
+  uint32_t king = getbit(L.kings, sq);
+  return (g.capture && !king && sl < 2u) ? (sl ^ 1u) : sl;
+}
+
+// k-th move (0-based) of the slot masks in square-major, slot-minor order.
+CK_HD void select_slot(const uint32_t s[4], uint32_t k, uint32_t &sq, uint32_t &sl) {
+  uint32_t lo = 0;
+#pragma unroll
+  for (uint32_t stride = 16; stride >= 1; stride >>= 1) {
+    uint32_t below = bit(lo + stride) - 1u;
+    uint32_t c = popc(s[0] & below) + popc(s[1] & below) + popc(s[2] & below) + popc(s[3] & below);
+    if (c <= k) lo += stride;
+  }
+  uint32_t below = bit(lo) - 1u;
+  uint32_t j = k - (popc(s[0] & below) + popc(s[1] & below) + popc(s[2] & below) + popc(s[3] & below));
+  uint32_t here = getbit(s[0], lo) | (getbit(s[1], lo) << 1) | (getbit(s[2], lo) << 2) |
+                  (getbit(s[3], lo) << 3);
+  if (j >= 1u) here &= here - 1u;
+  if (j >= 2u) here &= here - 1u;
+  if (j >= 3u) here &= here - 1u;
+  sq = lo;
+  sl = ctz(here);
+}
+
+// ------------------------------------------------------------------------------------------
+// capture-tree walk (src/state.cu:283-340 for men, 388-420 for kings)
+// ------------------------------------------------------------------------------------------
+// Children of the node on square x, as a bit mask in reference order.  Men: bit 0 = (+1,-1),
+// bit 1 = (+1,+1).  Kings: bit d = direction d, minus landings already used by the sequence
+// (the cycle check of src/state.cu:134-139).
+CK_HD uint32_t kids(uint32_t x, uint32_t king, const uint32_t cap[4], uint32_t visited) {
+  if (!king) return getbit(cap[1], x) | (getbit(cap[0], x) << 1);
+  uint32_t c0 = cap[0] & ~(visited >> 9);
+  uint32_t c1 = cap[1] & ~(visited >> 7);
+  uint32_t c2 = cap[2] & ~(visited << 7);
+  uint32_t c3 = cap[3] & ~(visited << 9);
+  return getbit(c0, x) | (getbit(c1, x) << 1) | (getbit(c2, x) << 2) | (getbit(c3, x) << 3);
+}
+
+// Depth-first walk of the capture sequences of the piece on `o`.  For each leaf, in order,
+// calls visit(to, removed_mask, jumps, dirs) where dirs holds the direction of jump t in bits
+// 2t..2t+1; the walk stops when visit returns true.  A king sequence has at most 7 jumps: all
+// its landing squares share one class of 8 squares ((row mod 2, (row+col) mod 4) is invariant
+// under a jump) with the origin, which stays occupied; men make at most 3.
+template <class Visit>
+CK_HD void walk(uint32_t o, uint32_t king, const uint32_t cap[4], Visit &visit) {
+  uint32_t x = o, depth = 0, visited = 0, removed = 0, dirs = 0;
+  uint32_t rem = kids(o, king, cap, 0);
+  for (;;) {
+    uint32_t cur = (rem >> (4u * depth)) & 15u;
+    if (cur) {
+      uint32_t s = ctz(cur);
+      rem ^= bit(4u * depth + s);
+      uint32_t d = king ? s : (s ^ 1u);
+      uint32_t land = (uint32_t)((int)x + jump_off(d));
+      visited |= bit(land);
+      removed |= bit(mid_sq(x, land));
+      dirs = (dirs & ~(3u << (2u * depth))) | (d << (2u * depth));
+      depth++;
+      x = land;
+      uint32_t ch = depth < 7u ? kids(x, king, cap, visited) : 0u;
+      if (ch) {
+        rem = (rem & ~(15u << (4u * depth))) | (ch << (4u * depth));
+        continue;
+      }
+      if (visit(x, removed, depth, dirs)) return;
+    } else if (depth == 0) {
+      return;
+    }
+    depth--;
+    uint32_t d = (dirs >> (2u * depth)) & 3u;
+    uint32_t prev = (uint32_t)((int)x - jump_off(d));
+    visited &= ~bit(x);
+    removed &= ~bit(mid_sq(prev, x));
+    x = prev;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// move application (src/state.cu:57-92), then hand the turn over (rotate into the new mover's
+// frame)
+// ------------------------------------------------------------------------------------------
+CK_HD void apply(Lane &L, uint32_t from, uint32_t to, uint32_t removed) {
+  const uint32_t fb = bit(from), tb = bit(to);
+  const uint32_t was_king = getbit(L.kings, from);
+  uint32_t own = L.own ^ (fb | tb);
+  uint32_t opp = L.opp & ~removed;
+  uint32_t kings = L.kings & ~removed;
+  kings = was_king ? (kings ^ (fb | tb)) : (kings | ((to >= FAR_ROW_FIRST) ? tb : 0u));
+  L.counter = removed ? 0u : L.counter + 1u;
+  L.own = brev(opp);
+  L.opp = brev(own);
+  L.kings = brev(kings);
+  L.turn ^= 1u;
+}
+
+// ------------------------------------------------------------------------------------------
+// heuristic weights (src/heuristic.cu:7-49): material with man 1, king 4
+// ------------------------------------------------------------------------------------------
+CK_HD uint32_t material(uint32_t pieces, uint32_t kings) {
+  return popc(pieces & ~kings) + 4u * popc(pieces & kings);
+}
+
+// ------------------------------------------------------------------------------------------
+// move-list hashing (verification mode).  Record layout shared with the oracle:
+//   move digest d = fmix(0x6A09E667 ^ (from | to<<5 | jumps<<10 | promoted<<14)), then per jump
+//                   d = fmix(d ^ (inter_t | removed_t<<5 | (t+1)<<10))        (absolute squares)
+//   list digest    = sum_i fmix(d_i + i * 0x9E3779B9)                         (reference index i)
+//   ply record     = fmix(list ^ fmix(n | chosen<<16)), chosen = 0xFFFF at game end
+//   chain          h' = fmix(h ^ record) + 0x9E3779B9, starting at 0
+// ------------------------------------------------------------------------------------------
+CK_HD uint32_t fmix(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  x *= 0xC2B2AE35u;
+  x ^= x >> 16;
+  return x;
+}
+CK_HD uint32_t digest_head(uint32_t from, uint32_t to, uint32_t jumps, uint32_t promoted) {
+  return fmix(0x6A09E667u ^ (from | (to << 5) | (jumps << 10) | (promoted << 14)));
+}
+CK_HD uint32_t digest_jump(uint32_t d, uint32_t inter, uint32_t removed, uint32_t t) {
+  return fmix(d ^ (inter | (removed << 5) | ((t + 1u) << 10)));
+}
+CK_HD uint32_t ply_record(uint32_t list, uint32_t n, uint32_t chosen) {
+  return fmix(list ^ fmix(n | (chosen << 16)));
+}
+CK_HD uint32_t hash_step(uint32_t h, uint32_t rec) { return fmix(h ^ rec) + 0x9E3779B9u; }
+
+// Visits every move of the position in mover-frame order:
+//   visit(index, from, to, removed_mask, jumps, dirs, king)
+// (jumps == 0 for direct moves).  Used by verification, hashing and the host move generator.
+template <class Visit>
+CK_HD uint32_t enumerate(const Lane &L, const Gen &g, Visit &visit) {
+  uint32_t idx = 0;
+  if (g.capture) {
+    uint32_t o = g.origins;
+    while (o) {
+      uint32_t s = ctz(o);
+      o &= o - 1u;
+      uint32_t king = getbit(L.kings, s);
+      auto leaf = [&](uint32_t to, uint32_t removed, uint32_t jumps, uint32_t dirs) {
+        visit(idx, s, to, removed, jumps, dirs, king);
+        idx++;
+        return false;
+      };
+      walk(s, king, g.cap, leaf);
+    }
+  } else {
+    uint32_t any = g.slot[0] | g.slot[1] | g.slot[2] | g.slot[3];
+    while (any) {
+      uint32_t s = ctz(any);
+      any &= any - 1u;
+      uint32_t king = getbit(L.kings, s);
+      for (uint32_t sl = 0; sl < 4u; sl++) {
+        if (!getbit(g.slot[sl], s)) continue;
+        visit(idx, s, step_to(s, sl), 0u, 0u, 0u, king);
+        idx++;
+      }
+    }
+  }
+  return idx;
+}
+
+// Digest of one move given its mover-frame description.
+CK_HD uint32_t move_digest(const Lane &L, uint32_t from, uint32_t to, uint32_t jumps,
+                           uint32_t dirs, uint32_t king) {
+  uint32_t promoted = (!king && to >= FAR_ROW_FIRST) ? 1u : 0u;
+  uint32_t d = digest_head(abs_sq(L, from), abs_sq(L, to), jumps, promoted);
+  uint32_t x = from;
+  for (uint32_t t = 0; t < jumps; t++) {
+    uint32_t dir = (dirs >> (2u * t)) & 3u;
+    uint32_t land = (uint32_t)((int)x + jump_off(dir));
+    d = digest_jump(d, abs_sq(L, land), abs_sq(L, mid_sq(x, land)), t);
+    x = land;
+  }
+  return d;
+}
+
+// List digest with reference indices (index n-1-i for player 2).
+CK_HD uint32_t list_digest(const Lane &L, const Gen &g, uint32_t n) {
+  uint32_t acc = 0;
+  auto v = [&](uint32_t i, uint32_t from, uint32_t to, uint32_t, uint32_t jumps, uint32_t dirs,
+               uint32_t king) {
+    uint32_t ref_i = L.turn ? n - 1u - i : i;
+    acc += fmix(move_digest(L, from, to, jumps, dirs, king) + ref_i * 0x9E3779B9u);
+  };
+  enumerate(L, g, v);
+  return acc;
+}
+
+// ------------------------------------------------------------------------------------------
+// one ply
+// ------------------------------------------------------------------------------------------
+// Result of choosing: the move in mover-frame squares.
+struct Choice {
+  uint32_t from, to, removed;
+};
+
+// Number of capture sequences of all origins (random policy, multi-jump ply).
+CK_HD uint32_t count_walks(const Lane &L, const Gen &g) {
+  uint32_t n = 0;
+  uint32_t o = g.origins;
+  auto leaf = [&](uint32_t, uint32_t, uint32_t, uint32_t) {
+    n++;
+    return false;
+  };
+  while (o) {
+    uint32_t s = ctz(o);
+    o &= o - 1u;
+    walk(s, getbit(L.kings, s), g.cap, leaf);
+  }
+  return n;
+}
+
+// k-th capture sequence in mover-frame order.
+CK_HD Choice select_walk(const Lane &L, const Gen &g, uint32_t k) {
+  Choice c = {0u, 0u, 0u};
+  uint32_t o = g.origins;
+  uint32_t left = k;
+  bool found = false;
+  auto leaf = [&](uint32_t to, uint32_t removed, uint32_t, uint32_t) {
+    if (left == 0u) {
+      c.to = to;
+      c.removed = removed;
+      found = true;
+      return true;
+    }
+    left--;
+    return false;
+  };
+  while (o && !found) {
+    uint32_t s = ctz(o);
+    o &= o - 1u;
+    c.from = s;
+    walk(s, getbit(L.kings, s), g.cap, leaf);
+  }
+  return c;
+}
+
+// Move of slot choice (square sq, slot sl).
+CK_HD Choice slot_choice(const Lane &L, const Gen &g, uint32_t sq, uint32_t sl) {
+  Choice c;
+  uint32_t d = slot_dir(L, g, sq, sl);
+  c.from = sq;
+  if (g.capture) {
+    c.to = (uint32_t)((int)sq + jump_off(d));
+    c.removed = bit(mid_sq(sq, c.to));
+  } else {
+    c.to = step_to(sq, d);
+    c.removed = 0u;
+  }
+  return c;
+}
+
+// Weight comparison a/b vs c/d for non-negative integers with a, c >= 1 (b == 0 is +inf).
+CK_HD int cmp_weight(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+  uint32_t l = a * d, r = c * b;
+  return (l > r) - (l < r);
+}
+
+// Outcome of a ply: n moves; the reference index chosen (or 0xFFFF at game end).
+struct PlyResult {
+  uint32_t n;
+  uint32_t chosen;
+  uint32_t over;    // 1 if the game ended before this ply
+  int32_t winner;   // valid when over
+};
+
+// Plays one ply of the random policy with word r (src/playout.cpp:24-27,
+// src/player.cpp:13-16).  With TRACE, also returns the list digest.
+template <bool TRACE>
+CK_HD PlyResult ply_random(Lane &L, uint32_t r, uint32_t *list_dig) {
+  Gen g;
+  analyse(L, g);
+  PlyResult res;
+  uint32_t n = g.multi ? count_walks(L, g) : slot_count(g.slot);
+  res.n = n;
+  if (TRACE) *list_dig = list_digest(L, g, n);
+  if (n == 0u || L.counter >= 50u) {  // src/state.cpp:16-23
+    res.over = 1u;
+    res.chosen = 0xFFFFu;
+    res.winner = L.counter < 50u ? (int32_t)(L.turn ^ 1u) : -1;
+    return res;
+  }
+  uint32_t k = mod_u32(r, n);
+  res.chosen = k;
+  res.over = 0u;
+  res.winner = 0;
+  uint32_t km = L.turn ? n - 1u - k : k;
+  Choice c;
+  if (g.multi) {
+    c = select_walk(L, g, km);
+  } else {
+    uint32_t sq, sl;
+    select_slot(g.slot, km, sq, sl);
+    c = slot_choice(L, g, sq, sl);
+  }
+  apply(L, c.from, c.to, c.removed);
+  return res;
+}
+
+// Plays one ply of the heuristic policy with word r (src/heuristicPlayout.cpp:20-42 without
+// the noise; ties broken by T[r mod |T|]).
+template <bool TRACE>
+CK_HD PlyResult ply_heuristic(Lane &L, uint32_t r, uint32_t *list_dig) {
+  Gen g;
+  analyse(L, g);
+  PlyResult res;
+  const uint32_t sm = material(L.own, L.kings);
+  const uint32_t so = material(L.opp, L.kings);
+  uint32_t n = 0, ties = 0, bn = 0, bd = 1;
+  uint32_t tslot[4];
+  if (g.capture) {
+    uint32_t o = g.origins;
+    uint32_t king = 0;
+    auto leaf = [&](uint32_t to, uint32_t removed, uint32_t, uint32_t) {
+      uint32_t num = sm + ((!king && to >= FAR_ROW_FIRST) ? 3u : 0u);
+      uint32_t den = so - material(removed, L.kings);
+      int c = cmp_weight(num, den, bn, bd);
+      if (c > 0) {
+        bn = num;
+        bd = den;
+        ties = 1u;
+      } else if (c == 0) {
+        ties++;
+      }
+      n++;
+      return false;
+    };
+    while (o) {
+      uint32_t s = ctz(o);
+      o &= o - 1u;
+      king = getbit(L.kings, s);
+      walk(s, king, g.cap, leaf);
+    }
+  } else {
+    n = slot_count(g.slot);
+    const uint32_t men = L.own & ~L.kings;
+    uint32_t pa = g.slot[0] & men & ROW6, pb = g.slot[1] & men & ROW6;
+    // a promotion weighs (sm + 3) / so > sm / so unless so == 0 (all weights +inf)
+    bool promo = (pa | pb) && so != 0u;
+    tslot[0] = promo ? pa : g.slot[0];
+    tslot[1] = promo ? pb : g.slot[1];
+    tslot[2] = promo ? 0u : g.slot[2];
+    tslot[3] = promo ? 0u : g.slot[3];
+    ties = slot_count(tslot);
+  }
+  res.n = n;
+  if (TRACE) *list_dig = list_digest(L, g, n);
+  if (n == 0u || L.counter >= 50u) {
+    res.over = 1u;
+    res.chosen = 0xFFFFu;
+    res.winner = L.counter < 50u ? (int32_t)(L.turn ^ 1u) : -1;
+    return res;
+  }
+  uint32_t k = mod_u32(r, ties);
+  uint32_t km = L.turn ? ties - 1u - k : k;
+  res.over = 0u;
+  res.winner = 0;
+  Choice c = {0u, 0u, 0u};
+  if (g.capture) {
+    uint32_t o = g.origins;
+    uint32_t king = 0, left = km, idx = 0;
+    bool found = false;
+    auto leaf = [&](uint32_t to, uint32_t removed, uint32_t, uint32_t) {
+      uint32_t num = sm + ((!king && to >= FAR_ROW_FIRST) ? 3u : 0u);
+      uint32_t den = so - material(removed, L.kings);
+      if (cmp_weight(num, den, bn, bd) == 0) {
+        if (left == 0u) {
+          c.to = to;
+          c.removed = removed;
+          found = true;
+          return true;
+        }
+        left--;
+      }
+      idx++;
+      return false;
+    };
+    while (o && !found) {
+      uint32_t s = ctz(o);
+      o &= o - 1u;
+      king = getbit(L.kings, s);
+      c.from = s;
+      walk(s, king, g.cap, leaf);
+    }
+    // reference index of the chosen move (for the trace record)
+    res.chosen = L.turn ? n - 1u - idx : idx;
+  } else {
+    uint32_t sq, sl;
+    select_slot(tslot, km, sq, sl);
+    c = slot_choice(L, g, sq, sl);
+    if (TRACE) {
+      // reference index: rank of (sq, sl) in the full slot order
+      uint32_t below = bit(sq) - 1u;
+      uint32_t i = popc(g.slot[0] & below) + popc(g.slot[1] & below) + popc(g.slot[2] & below) +
+                   popc(g.slot[3] & below);
+      for (uint32_t t = 0; t < sl; t++) i += getbit(g.slot[t], sq);
+      res.chosen = L.turn ? n - 1u - i : i;
+    } else {
+      res.chosen = 0u;
+    }
+  }
+  apply(L, c.from, c.to, c.removed);
+  return res;
+}
+
+// True when the position is finished (no legal move, or the draw counter reached 50).
+CK_HD bool game_over(const Lane &L) {
+  if (L.counter >= 50u) return true;
+  Gen g;
+  analyse(L, g);
+  return g.capture ? false : slot_count(g.slot) == 0u;
+}
+
+}  // namespace ck

@@ -1,0 +1,48 @@
+// kernels.h -- host-side launch interface of the gfx950 kernels (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gpuai/mi_playout.h"
+
+namespace gpuai {
+
+// Device-side per-launch work words (one 64-byte line, zeroed by launch_reset before use).
+struct WorkWords {
+  uint32_t queue;        // next unclaimed playout index (wave-level claims)
+  uint32_t max_moves;    // longest move list seen
+  uint32_t overflow;     // a move list exceeded a capacity
+  uint32_t pad0;
+  unsigned long long plies;  // plies applied over the launch
+  uint32_t pad1[10];
+};
+
+struct PlayoutArgs {
+  const mi_pos *pos;       // npos positions
+  const uint32_t *offs;    // npos+1 prefix offsets of playout counts, or null (uniform per_pos)
+  uint32_t npos;
+  uint32_t per_pos;
+  uint32_t total;          // total playouts (< 2^31)
+  uint32_t seed;
+  uint32_t epoch;
+  int8_t *winners;         // total
+  uint32_t *plies;         // total (trace mode only)
+  uint32_t *hashes;        // total (trace mode only)
+  WorkWords *work;
+};
+
+hipError_t launch_reset(WorkWords *work, hipStream_t stream);
+hipError_t launch_playouts(const PlayoutArgs &args, uint32_t policy, bool trace, uint32_t refill,
+                           uint32_t grid, uint32_t block, hipStream_t stream);
+hipError_t launch_gen_moves(const mi_pos *pos, uint32_t npos, mi_move *moves, uint32_t cap,
+                            uint32_t *counts, WorkWords *work, hipStream_t stream);
+hipError_t launch_start_positions(uint32_t seed, uint32_t n, uint32_t kmin, uint32_t kmax,
+                                  uint32_t reject_terminal, mi_pos *out, WorkWords *work,
+                                  hipStream_t stream);
+hipError_t launch_divmod(const uint32_t *r, const uint32_t *n, uint32_t count, uint32_t *q,
+                         uint32_t *rem, hipStream_t stream);
+hipError_t launch_tally(const int8_t *winners, const uint32_t *offs, uint32_t npos,
+                        uint32_t per_pos, uint32_t *tallies, hipStream_t stream);
+
+}  // namespace gpuai

@@ -1,0 +1,338 @@
+// playout_kernels.hip -- gfx950 kernels of the playout engine.
+//
+// k_playouts is a persistent kernel: each lane of a wave64 runs one playout at a time with the
+// position in VGPRs (ck::Lane) and, when its game ends, takes the next playout index.  New
+// playouts start only on wave iterations that are multiples of four, so every active lane is at
+// a ply index that is a multiple of four on those iterations and the Philox4x32-10 call that
+// yields the next four words is wave-uniform (one call per four plies instead of a divergent
+// call on almost every iteration).  Refill policy MI_REFILL_QUEUE claims playout indices for the
+// whole wave in chunks of 64 from a device counter (one atomic per 64 playouts, broadcast with a
+// lane read), MI_REFILL_STATIC strides lanes over the index space without atomics.
+#include <hip/hip_runtime.h>
+
+#include "checkers.h"
+#include "kernels.h"
+
+namespace gpuai {
+
+namespace {
+
+constexpr uint32_t kChunk = 64;
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__global__ void k_reset(WorkWords *work) {
+  if (threadIdx.x == 0) {
+    work->queue = 0;
+    work->max_moves = 0;
+    work->overflow = 0;
+    work->plies = 0ull;
+  }
+}
+
+// Position and playout index of global playout g.
+__device__ __forceinline__ void locate(const PlayoutArgs &a, uint32_t g, uint32_t &p, uint32_t &j) {
+  if (a.offs) {
+    uint32_t lo = 0, hi = a.npos;
+    while (hi - lo > 1u) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (a.offs[mid] <= g) lo = mid; else hi = mid;
+    }
+    p = lo;
+    j = g - a.offs[lo];
+  } else {
+    ck::divmod_u32(g, a.per_pos, p, j);
+  }
+}
+
+template <uint32_t POLICY, bool TRACE, uint32_t REFILL>
+__global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t nthreads = gridDim.x * blockDim.x;
+  bool active = false;
+  uint32_t g = 0, p = 0, j = 0, ply = 0, h = 0, maxn = 0;
+  uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+  uint64_t plies_acc = 0;
+  ck::Lane L = {0u, 0u, 0u, 0u, 0u};
+  // wave-uniform refill state
+  uint32_t res_next = 0, res_end = 0;
+  bool exhausted = false;
+  uint32_t static_next = blockIdx.x * blockDim.x + threadIdx.x;
+
+  for (uint32_t iter = 0;; iter++) {
+    if ((iter & 3u) == 0u) {
+      if (REFILL == MI_REFILL_STATIC) {
+        if (!active && static_next < a.total) {
+          g = static_next;
+          static_next += nthreads;
+          active = true;
+          locate(a, g, p, j);
+          mi_pos q = a.pos[p];
+          L = ck::lane_from_abs(q.p1, q.p2, q.kings, q.turn, q.counter);
+          ply = 0;
+          h = 0;
+        }
+      } else if (!exhausted) {
+        const uint64_t idle = __ballot(!active);
+        if (idle) {
+          const uint32_t cnt = (uint32_t)__popcll(idle);
+          const uint32_t avail = res_end - res_next;
+          uint32_t chunk = 0;
+          if (avail < cnt) {
+            if (lane == 0u) chunk = atomicAdd(&a.work->queue, kChunk);
+            chunk = __shfl(chunk, 0);
+          }
+          if (!active) {
+            const uint32_t rank = lane_rank(idle);
+            const uint32_t idx = rank < avail ? res_next + rank : chunk + (rank - avail);
+            if (idx < a.total) {
+              g = idx;
+              active = true;
+              locate(a, g, p, j);
+              mi_pos q = a.pos[p];
+              L = ck::lane_from_abs(q.p1, q.p2, q.kings, q.turn, q.counter);
+              ply = 0;
+              h = 0;
+            }
+          }
+          if (avail < cnt) {
+            res_next = chunk + (cnt - avail);
+            res_end = chunk + kChunk;
+          } else {
+            res_next += cnt;
+          }
+          if (res_next >= a.total) exhausted = true;
+        }
+      }
+      if (!__any(active)) break;
+      if (active) {
+        uint32_t w[4];
+        ck::philox(ply >> 2, j, ck::TAG_PLAYOUT, a.epoch, a.seed, p, w);
+        w0 = w[0];
+        w1 = w[1];
+        w2 = w[2];
+        w3 = w[3];
+      }
+    }
+    if (active) {
+      const uint32_t r = w0;
+      w0 = w1;
+      w1 = w2;
+      w2 = w3;
+      uint32_t ld = 0;
+      ck::PlyResult res = POLICY == MI_POLICY_RANDOM ? ck::ply_random<TRACE>(L, r, &ld)
+                                                     : ck::ply_heuristic<TRACE>(L, r, &ld);
+      if (TRACE) h = ck::hash_step(h, ck::ply_record(ld, res.n, res.chosen));
+      maxn = res.n > maxn ? res.n : maxn;
+      if (res.over) {
+        a.winners[g] = (int8_t)res.winner;
+        if (TRACE) {
+          if (a.plies) a.plies[g] = ply;
+          if (a.hashes) a.hashes[g] = h;
+        }
+        plies_acc += ply;
+        active = false;
+      } else {
+        ply++;
+      }
+    }
+  }
+  // per-wave reduction of the statistics, one atomic per wave
+  for (int off = 32; off >= 1; off >>= 1) {
+    plies_acc += __shfl_xor(plies_acc, off);
+    uint32_t m = __shfl_xor(maxn, off);
+    maxn = m > maxn ? m : maxn;
+  }
+  if (lane == 0u) {
+    atomicAdd(&a.work->plies, (unsigned long long)plies_acc);
+    atomicMax(&a.work->max_moves, maxn);
+  }
+}
+
+// One lane per position: the whole move list in reference order, absolute squares.
+__global__ void __launch_bounds__(256) k_gen_moves(const mi_pos *__restrict__ pos, uint32_t npos,
+                                                   mi_move *__restrict__ out, uint32_t cap,
+                                                   uint32_t *__restrict__ counts,
+                                                   WorkWords *work) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npos) return;
+  const mi_pos q = pos[i];
+  const ck::Lane L = ck::lane_from_abs(q.p1, q.p2, q.kings, q.turn, q.counter);
+  ck::Gen gen;
+  ck::analyse(L, gen);
+  const uint32_t n = gen.capture ? ck::count_walks(L, gen) : ck::slot_count(gen.slot);
+  counts[i] = n;
+  if (n > cap) {
+    atomicOr(&work->overflow, 1u);
+    return;
+  }
+  mi_move *dst = out + (size_t)i * cap;
+  auto v = [&](uint32_t idx, uint32_t from, uint32_t to, uint32_t, uint32_t jumps, uint32_t dirs,
+               uint32_t king) {
+    const uint32_t ref = L.turn ? n - 1u - idx : idx;
+    mi_move m;
+    m.from = (uint8_t)ck::abs_sq(L, from);
+    m.to = (uint8_t)ck::abs_sq(L, to);
+    m.jumps = (uint8_t)jumps;
+    m.promoted = (uint8_t)((!king && to >= ck::FAR_ROW_FIRST) ? 1 : 0);
+    uint32_t x = from;
+#pragma unroll
+    for (uint32_t t = 0; t < 8u; t++) {
+      m.inter[t] = 0;
+      m.removed[t] = 0;
+    }
+    for (uint32_t t = 0; t < jumps; t++) {
+      const uint32_t d = (dirs >> (2u * t)) & 3u;
+      const uint32_t land = (uint32_t)((int)x + ck::jump_off(d));
+      m.inter[t] = (uint8_t)ck::abs_sq(L, land);
+      m.removed[t] = (uint8_t)ck::abs_sq(L, ck::mid_sq(x, land));
+      x = land;
+    }
+    dst[ref] = m;
+  };
+  ck::enumerate(L, gen, v);
+}
+
+// One wave per position: tallies of draws, player-1 wins and player-2 wins.
+__global__ void __launch_bounds__(256) k_tally(const int8_t *__restrict__ winners,
+                                               const uint32_t *__restrict__ offs, uint32_t npos,
+                                               uint32_t per_pos, uint32_t *__restrict__ tallies) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
+  if (wave >= npos) return;
+  const uint64_t begin = offs ? offs[wave] : (uint64_t)wave * per_pos;
+  const uint64_t end = offs ? offs[wave + 1] : begin + per_pos;
+  uint32_t c[3] = {0u, 0u, 0u};
+  for (uint64_t g = begin + lane; g < end; g += 64u) {
+    const int w = winners[g];
+    c[0] += w < 0;
+    c[1] += w == 0;
+    c[2] += w == 1;
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    c[0] += __shfl_xor(c[0], off);
+    c[1] += __shfl_xor(c[1], off);
+    c[2] += __shfl_xor(c[2], off);
+  }
+  if (lane == 0u) {
+    tallies[3u * wave + 0u] = c[0];
+    tallies[3u * wave + 1u] = c[1];
+    tallies[3u * wave + 2u] = c[2];
+  }
+}
+
+// One lane per start position (the stream layout of mi_gen_start_positions).
+__global__ void __launch_bounds__(256) k_start_positions(uint32_t seed, uint32_t n, uint32_t kmin,
+                                                         uint32_t kmax, uint32_t reject,
+                                                         mi_pos *__restrict__ out,
+                                                         WorkWords *work) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ck::Lane L = ck::lane_from_abs(0x00000FFFu, 0xFFF00000u, 0u, 0u, 0u);
+  bool done = false;
+  for (uint32_t attempt = 0; attempt < 4096u && !done; attempt++) {
+    L = ck::lane_from_abs(0x00000FFFu, 0xFFF00000u, 0u, 0u, 0u);
+    uint32_t w[4];
+    ck::philox(0u, attempt, ck::TAG_START, 1u, seed, i, w);
+    const uint32_t k = kmin + ck::mod_u32(w[0], kmax - kmin + 1u);
+    for (uint32_t t = 0; t < k; t++) {
+      if ((t & 3u) == 0u) ck::philox(t >> 2, attempt, ck::TAG_START, 0u, seed, i, w);
+      uint32_t ld;
+      if (ck::ply_random<false>(L, w[t & 3u], &ld).over) break;
+    }
+    done = !reject || !ck::game_over(L);
+  }
+  if (!done) atomicOr(&work->overflow, 1u);
+  uint32_t p1, p2, kings;
+  ck::lane_to_abs(L, p1, p2, kings);
+  mi_pos q;
+  q.p1 = p1;
+  q.p2 = p2;
+  q.kings = kings;
+  q.turn = (uint8_t)L.turn;
+  q.counter = (uint8_t)(L.counter > 255u ? 255u : L.counter);
+  q.pad[0] = 0;
+  q.pad[1] = 0;
+  out[i] = q;
+}
+
+__global__ void __launch_bounds__(256) k_divmod(const uint32_t *__restrict__ r,
+                                                const uint32_t *__restrict__ n, uint32_t count,
+                                                uint32_t *__restrict__ q, uint32_t *__restrict__ rem) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t qq, rr;
+  ck::divmod_u32(r[i], n[i], qq, rr);
+  q[i] = qq;
+  rem[i] = ck::mod_u32(r[i], n[i]) == rr ? rr : 0xFFFFFFFFu;
+}
+
+template <uint32_t POLICY, bool TRACE>
+hipError_t launch_refill(const PlayoutArgs &args, uint32_t refill, uint32_t grid, uint32_t block,
+                         hipStream_t stream) {
+  if (refill == MI_REFILL_STATIC)
+    hipLaunchKernelGGL((k_playouts<POLICY, TRACE, MI_REFILL_STATIC>), dim3(grid), dim3(block), 0,
+                       stream, args);
+  else
+    hipLaunchKernelGGL((k_playouts<POLICY, TRACE, MI_REFILL_QUEUE>), dim3(grid), dim3(block), 0,
+                       stream, args);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_reset(WorkWords *work, hipStream_t stream) {
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, work);
+  return hipGetLastError();
+}
+
+hipError_t launch_playouts(const PlayoutArgs &args, uint32_t policy, bool trace, uint32_t refill,
+                           uint32_t grid, uint32_t block, hipStream_t stream) {
+  if (policy == MI_POLICY_RANDOM)
+    return trace ? launch_refill<MI_POLICY_RANDOM, true>(args, refill, grid, block, stream)
+                 : launch_refill<MI_POLICY_RANDOM, false>(args, refill, grid, block, stream);
+  return trace ? launch_refill<MI_POLICY_HEURISTIC, true>(args, refill, grid, block, stream)
+               : launch_refill<MI_POLICY_HEURISTIC, false>(args, refill, grid, block, stream);
+}
+
+hipError_t launch_gen_moves(const mi_pos *pos, uint32_t npos, mi_move *moves, uint32_t cap,
+                            uint32_t *counts, WorkWords *work, hipStream_t stream) {
+  const uint32_t block = 256;
+  const uint32_t grid = (npos + block - 1) / block;
+  hipLaunchKernelGGL(k_gen_moves, dim3(grid), dim3(block), 0, stream, pos, npos, moves, cap,
+                     counts, work);
+  return hipGetLastError();
+}
+
+hipError_t launch_start_positions(uint32_t seed, uint32_t n, uint32_t kmin, uint32_t kmax,
+                                  uint32_t reject_terminal, mi_pos *out, WorkWords *work,
+                                  hipStream_t stream) {
+  const uint32_t block = 256;
+  const uint32_t grid = (n + block - 1) / block;
+  hipLaunchKernelGGL(k_start_positions, dim3(grid), dim3(block), 0, stream, seed, n, kmin, kmax,
+                     reject_terminal, out, work);
+  return hipGetLastError();
+}
+
+hipError_t launch_divmod(const uint32_t *r, const uint32_t *n, uint32_t count, uint32_t *q,
+                         uint32_t *rem, hipStream_t stream) {
+  const uint32_t block = 256;
+  const uint32_t grid = (count + block - 1) / block;
+  hipLaunchKernelGGL(k_divmod, dim3(grid), dim3(block), 0, stream, r, n, count, q, rem);
+  return hipGetLastError();
+}
+
+hipError_t launch_tally(const int8_t *winners, const uint32_t *offs, uint32_t npos,
+                        uint32_t per_pos, uint32_t *tallies, hipStream_t stream) {
+  const uint32_t block = 256;
+  const uint64_t threads = (uint64_t)npos * 64u;
+  const uint32_t grid = (uint32_t)((threads + block - 1) / block);
+  hipLaunchKernelGGL(k_tally, dim3(grid), dim3(block), 0, stream, winners, offs, npos, per_pos,
+                     tallies);
+  return hipGetLastError();
+}
+
+}  // namespace gpuai

@@ -1,0 +1,253 @@
+"""ctypes bindings of the C ABI in include/gpuai/mi_playout.h.
+
+The library must have been built (python -m gpu_ai_amd.build); importing this module on a box
+without it raises at load time, so a missing extension never degrades into a silent fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .build import LIB_PATH
+
+POS_DTYPE = np.dtype([("p1", "<u4"), ("p2", "<u4"), ("kings", "<u4"), ("turn", "u1"),
+                      ("counter", "u1"), ("pad", "u1", (2,))])
+MOVE_DTYPE = np.dtype([("from", "u1"), ("to", "u1"), ("jumps", "u1"), ("promoted", "u1"),
+                       ("inter", "u1", (8,)), ("removed", "u1", (8,))])
+assert POS_DTYPE.itemsize == 16 and MOVE_DTYPE.itemsize == 20
+
+MI_OK = 0
+MI_ERR_INVALID_ARGUMENT = 1
+MI_ERR_NO_DEVICE = 2
+MI_ERR_DEVICE = 3
+MI_ERR_OUT_OF_MEMORY = 4
+MI_ERR_CAPACITY = 5
+MI_ERR_NOT_STAGED = 6
+
+POLICY_RANDOM = 0
+POLICY_HEURISTIC = 1
+REFILL_STATIC = 0
+REFILL_QUEUE = 1
+
+
+class MiTuning(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("refill", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class MiStats(ctypes.Structure):
+    _fields_ = [("playouts", ctypes.c_uint64), ("plies", ctypes.c_uint64),
+                ("kernel_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
+                ("max_moves", ctypes.c_uint32), ("grid_blocks", ctypes.c_uint32),
+                ("block_size", ctypes.c_uint32), ("refill", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MiError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_SIGS = {
+    "mi_create": ([ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(MiTuning)], ctypes.c_int),
+    "mi_destroy": ([_P], ctypes.c_int),
+    "mi_strerror": ([ctypes.c_int], ctypes.c_char_p),
+    "mi_last_error": ([_P], ctypes.c_char_p),
+    "mi_set_tuning": ([_P, ctypes.POINTER(MiTuning)], ctypes.c_int),
+    "mi_get_tuning": ([_P, ctypes.POINTER(MiTuning)], ctypes.c_int),
+    "mi_device_info": ([_P, _P, _P, _P], ctypes.c_int),
+    "mi_run_playouts": ([_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _P, ctypes.POINTER(MiStats)],
+                        ctypes.c_int),
+    "mi_run_playouts_traced": ([_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _P, _P, _P,
+                                ctypes.POINTER(MiStats)], ctypes.c_int),
+    "mi_run_playouts_tally": ([_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _P,
+                               ctypes.POINTER(MiStats)], ctypes.c_int),
+    "mi_gen_moves": ([_P, _P, _U32, _P, _U32, _P], ctypes.c_int),
+    "mi_stage": ([_P, _P, _U32, _P, _U32], ctypes.c_int),
+    "mi_launch_staged": ([_P, _U32, _U32, _U32, ctypes.POINTER(MiStats)], ctypes.c_int),
+    "mi_fetch_winners": ([_P, _P, ctypes.c_uint64], ctypes.c_int),
+    "mi_check_divmod": ([_P, _P, _P, _U32, _P, _P], ctypes.c_int),
+    "mi_gen_start_positions": ([_P, _U32, _U32, _U32, _U32, _U32, _P], ctypes.c_int),
+}
+
+_lib = None
+
+
+def library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads the device library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `python -m gpu_ai_amd.build`")
+        lib = ctypes.CDLL(path)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def as_positions(pos) -> np.ndarray:
+    arr = np.ascontiguousarray(pos)
+    if arr.dtype != POS_DTYPE:
+        raise TypeError(f"positions must have dtype POS_DTYPE, got {arr.dtype}")
+    return arr
+
+
+@dataclass
+class PlayoutResult:
+    winners: np.ndarray
+    stats: dict
+    plies: np.ndarray | None = None
+    hashes: np.ndarray | None = None
+
+
+class Engine:
+    """One device context (stream, events, buffers) of the playout engine."""
+
+    def __init__(self, device: int = 0, block_size: int = 0, blocks_per_cu: int = 0,
+                 refill: int = REFILL_QUEUE):
+        self.lib = library()
+        self.ctx = ctypes.c_void_p()
+        t = MiTuning(block_size, blocks_per_cu, refill, 0)
+        rc = self.lib.mi_create(ctypes.byref(self.ctx), device, ctypes.byref(t))
+        if rc != MI_OK:
+            raise MiError(rc, self.lib.mi_strerror(rc).decode())
+
+    # -- helpers --
+    def _check(self, rc: int) -> None:
+        if rc != MI_OK:
+            msg = self.lib.mi_last_error(self.ctx).decode() or self.lib.mi_strerror(rc).decode()
+            raise MiError(rc, msg)
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.mi_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def tuning(self) -> dict:
+        t = MiTuning()
+        self._check(self.lib.mi_get_tuning(self.ctx, ctypes.byref(t)))
+        return {"block_size": t.block_size, "blocks_per_cu": t.blocks_per_cu, "refill": t.refill}
+
+    def set_tuning(self, block_size: int = 0, blocks_per_cu: int = 0, refill: int = REFILL_QUEUE):
+        t = MiTuning(block_size, blocks_per_cu, refill, 0)
+        self._check(self.lib.mi_set_tuning(self.ctx, ctypes.byref(t)))
+
+    def device_info(self) -> dict:
+        cu, ws, clk = _U32(), _U32(), _U32()
+        self._check(self.lib.mi_device_info(self.ctx, ctypes.byref(cu), ctypes.byref(ws),
+                                            ctypes.byref(clk)))
+        return {"compute_units": cu.value, "wave_size": ws.value, "clock_khz": clk.value}
+
+    @staticmethod
+    def _counts(npos: int, per_pos: int | None, counts) -> tuple[np.ndarray | None, int, int]:
+        if counts is not None:
+            c = np.ascontiguousarray(counts, dtype=np.uint32)
+            if c.shape != (npos,):
+                raise ValueError("counts must have one entry per position")
+            return c, 0, int(c.sum(dtype=np.uint64))
+        return None, int(per_pos), npos * int(per_pos)
+
+    # -- entry points --
+    def run_playouts(self, pos, per_pos: int | None = None, counts=None, seed: int = 12345,
+                     epoch: int = 0, policy: int = POLICY_RANDOM) -> PlayoutResult:
+        pos = as_positions(pos)
+        c, pp, total = self._counts(len(pos), per_pos, counts)
+        winners = np.empty(total, dtype=np.int8)
+        st = MiStats()
+        self._check(self.lib.mi_run_playouts(self.ctx, _ptr(pos), len(pos), _ptr(c), pp, seed,
+                                             epoch, policy, _ptr(winners), ctypes.byref(st)))
+        return PlayoutResult(winners, st.as_dict())
+
+    def run_traced(self, pos, per_pos: int | None = None, counts=None, seed: int = 12345,
+                   epoch: int = 0, policy: int = POLICY_RANDOM) -> PlayoutResult:
+        pos = as_positions(pos)
+        c, pp, total = self._counts(len(pos), per_pos, counts)
+        winners = np.empty(total, dtype=np.int8)
+        plies = np.empty(total, dtype=np.uint32)
+        hashes = np.empty(total, dtype=np.uint32)
+        st = MiStats()
+        self._check(self.lib.mi_run_playouts_traced(self.ctx, _ptr(pos), len(pos), _ptr(c), pp,
+                                                    seed, epoch, policy, _ptr(winners),
+                                                    _ptr(plies), _ptr(hashes), ctypes.byref(st)))
+        return PlayoutResult(winners, st.as_dict(), plies, hashes)
+
+    def run_tally(self, pos, per_pos: int | None = None, counts=None, seed: int = 12345,
+                  epoch: int = 0, policy: int = POLICY_RANDOM) -> tuple[np.ndarray, dict]:
+        pos = as_positions(pos)
+        c, pp, _ = self._counts(len(pos), per_pos, counts)
+        tallies = np.zeros((len(pos), 3), dtype=np.uint32)
+        st = MiStats()
+        self._check(self.lib.mi_run_playouts_tally(self.ctx, _ptr(pos), len(pos), _ptr(c), pp,
+                                                   seed, epoch, policy, _ptr(tallies),
+                                                   ctypes.byref(st)))
+        return tallies, st.as_dict()
+
+    def gen_moves(self, pos, cap: int = 128) -> tuple[np.ndarray, np.ndarray]:
+        pos = as_positions(pos)
+        moves = np.zeros((len(pos), cap), dtype=MOVE_DTYPE)
+        counts = np.zeros(len(pos), dtype=np.uint32)
+        self._check(self.lib.mi_gen_moves(self.ctx, _ptr(pos), len(pos), _ptr(moves), cap,
+                                          _ptr(counts)))
+        return moves, counts
+
+    def start_positions(self, seed: int, n: int, kmin: int, kmax: int,
+                        reject_terminal: bool = False) -> np.ndarray:
+        out = np.zeros(n, dtype=POS_DTYPE)
+        self._check(self.lib.mi_gen_start_positions(self.ctx, seed, n, kmin, kmax,
+                                                    int(reject_terminal), _ptr(out)))
+        return out
+
+    def stage(self, pos, per_pos: int | None = None, counts=None) -> int:
+        pos = as_positions(pos)
+        c, pp, total = self._counts(len(pos), per_pos, counts)
+        self._check(self.lib.mi_stage(self.ctx, _ptr(pos), len(pos), _ptr(c), pp))
+        self._staged_total = total
+        return total
+
+    def launch_staged(self, seed: int = 12345, epoch: int = 0,
+                      policy: int = POLICY_RANDOM) -> dict:
+        st = MiStats()
+        self._check(self.lib.mi_launch_staged(self.ctx, seed, epoch, policy, ctypes.byref(st)))
+        return st.as_dict()
+
+    def fetch_winners(self) -> np.ndarray:
+        w = np.empty(self._staged_total, dtype=np.int8)
+        self._check(self.lib.mi_fetch_winners(self.ctx, _ptr(w), self._staged_total))
+        return w
+
+    def check_divmod(self, r, n) -> tuple[np.ndarray, np.ndarray]:
+        r = np.ascontiguousarray(r, dtype=np.uint32)
+        n = np.ascontiguousarray(n, dtype=np.uint32)
+        q = np.empty_like(r)
+        rem = np.empty_like(r)
+        self._check(self.lib.mi_check_divmod(self.ctx, _ptr(r), _ptr(n), len(r), _ptr(q),
+                                             _ptr(rem)))
+        return q, rem

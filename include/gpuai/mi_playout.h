@@ -1,0 +1,160 @@
+/*
+ * mi_playout.h -- C ABI of the MI355X (gfx950) Monte Carlo playout engine for 8x8 checkers.
+ *
+ * Plain pointers and sizes only.  The caller owns every host buffer; a context owns its device
+ * memory, its HIP stream and its timing events.  Every entry point returns MI_OK (0) or an
+ * error code that mi_strerror() describes; nothing in the library calls exit().
+ *
+ * Semantics follow the host path of gpu_ai (krame505/gpu_ai):
+ *   move generation   src/state.cu:122-180,239-245,255-438 (State::genMoves and helpers)
+ *   move application  src/state.cu:57-92 (State::move)
+ *   terminal rule     src/state.cpp:16-23 (isGameOver / getWinner)
+ *   random playout    src/playout.cpp:17-32 with src/player.cpp:13-16 (rand() % n)
+ *   heuristic playout src/heuristicPlayout.cpp:12-48 with src/heuristic.cu:7-49
+ * with each random draw taken from a counter-based Philox4x32-10 stream: the word for ply t of
+ * playout j of position i is word (t & 3) of Philox(key = (seed, i),
+ * counter = (t >> 2, j, 0, epoch)).  A random ply picks move (r mod n) of the reference-ordered
+ * list; a heuristic ply picks T[r mod |T|] of the reference-ordered set T of moves of maximal
+ * weight (S_mover + d_mover) / (S_opp + d_opp), compared exactly.
+ *
+ * Squares: s = 4*row + (col >> 1), col = 2*(s & 3) + (row even ? 1 : 0) (src/state.cu:173-176).
+ */
+#ifndef GPUAI_MI_PLAYOUT_H
+#define GPUAI_MI_PLAYOUT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MI_OK 0
+#define MI_ERR_INVALID_ARGUMENT 1
+#define MI_ERR_NO_DEVICE 2
+#define MI_ERR_DEVICE 3
+#define MI_ERR_OUT_OF_MEMORY 4
+#define MI_ERR_CAPACITY 5      /* a move list exceeded the caller's capacity */
+#define MI_ERR_NOT_STAGED 6
+
+/* ---- policies ---- */
+#define MI_POLICY_RANDOM 0     /* HostPlayoutDriver, src/playout.cpp:17-32 */
+#define MI_POLICY_HEURISTIC 1  /* HostHeuristicPlayoutDriver, src/heuristicPlayout.cpp:12-48 */
+
+/* ---- refill policies of the persistent playout kernel ---- */
+#define MI_REFILL_STATIC 0     /* lane-strided assignment, no atomics */
+#define MI_REFILL_QUEUE 1      /* wave-level claims of 64 playouts from a device counter */
+
+/* Packed position (16 bytes).  Replaces the 776-byte State (src/state.hpp:119-122). */
+typedef struct mi_pos {
+  uint32_t p1;       /* bit s: a player-1 piece stands on square s */
+  uint32_t p2;       /* bit s: a player-2 piece stands on square s */
+  uint32_t kings;    /* bit s: the piece on square s is a king */
+  uint8_t turn;      /* 0: player 1 to move (PLAYER_1), 1: player 2 (PLAYER_2) */
+  uint8_t counter;   /* movesSinceLastCapture */
+  uint8_t pad[2];
+} mi_pos;
+
+/* Canonical move record (20 bytes).  Replaces Move (src/state.hpp:253-260); equality as
+ * Move::operator== (src/state.cu:440-454): only the first `jumps` entries are meaningful. */
+typedef struct mi_move {
+  uint8_t from, to, jumps, promoted;
+  uint8_t inter[8];    /* intermediate[i]: landing square of jump i */
+  uint8_t removed[8];  /* removed[i]: square captured by jump i */
+} mi_move;
+
+/* Launch parameters of the persistent playout kernel.  Zero fields take the tuned defaults. */
+typedef struct mi_tuning {
+  uint32_t block_size;      /* threads per workgroup: 64..1024, a multiple of 64 */
+  uint32_t blocks_per_cu;   /* resident workgroups per compute unit for the persistent grid */
+  uint32_t refill;          /* MI_REFILL_* */
+  uint32_t reserved;
+} mi_tuning;
+
+/* Per-call statistics. */
+typedef struct mi_stats {
+  uint64_t playouts;       /* playouts run */
+  uint64_t plies;          /* moves applied over all playouts */
+  float kernel_ms;         /* device time of the playout kernel (HIP events on the ctx stream) */
+  float total_ms;          /* host wall time of the whole call */
+  uint32_t max_moves;      /* longest move list seen */
+  uint32_t grid_blocks;    /* workgroups launched */
+  uint32_t block_size;     /* threads per workgroup */
+  uint32_t refill;         /* refill policy used */
+} mi_stats;
+
+typedef struct mi_ctx mi_ctx;
+
+/* Creates a context on HIP device `device`.  `tuning` may be NULL.  No counterpart in the
+ * reference, whose drivers allocate on every call (src/singlePlayout.cu:76-84). */
+int mi_create(mi_ctx **ctx, int device, const mi_tuning *tuning);
+int mi_destroy(mi_ctx *ctx);
+const char *mi_strerror(int code);
+/* Text of the last error raised on this context (empty if none). */
+const char *mi_last_error(const mi_ctx *ctx);
+int mi_set_tuning(mi_ctx *ctx, const mi_tuning *tuning);
+int mi_get_tuning(const mi_ctx *ctx, mi_tuning *tuning);
+/* Device facts: compute units, wave size, clock in kHz. */
+int mi_device_info(const mi_ctx *ctx, uint32_t *compute_units, uint32_t *wave_size,
+                   uint32_t *clock_khz);
+
+/* Runs playouts and returns one winner per playout (0, 1, or -1 for a draw).
+ * Position i contributes counts[i] playouts (or per_pos each when counts == NULL); winners are
+ * laid out position-major.  Replaces PlayoutDriver::runPlayouts (src/playout.hpp:31) as
+ * implemented by DeviceSinglePlayoutDriver (src/singlePlayout.cu:71-120),
+ * DeviceMultiplePlayoutDriver (src/multiplePlayout.cu:53-98), DeviceCoarsePlayoutDriver
+ * (src/coarsePlayout.cu:91-163) and DeviceHeuristicPlayoutDriver
+ * (src/heuristicPlayout.cu:102-147).  A total of zero playouts returns MI_OK at once
+ * (src/singlePlayout.cu:73-75).  `stats` may be NULL. */
+int mi_run_playouts(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                    uint32_t per_pos, uint32_t seed, uint32_t epoch, uint32_t policy,
+                    int8_t *winners, mi_stats *stats);
+
+/* As mi_run_playouts, and also returns per playout the number of plies and the hash chain of
+ * every generated move list (verification mode; slower).  `plies` or `hashes` may be NULL. */
+int mi_run_playouts_traced(mi_ctx *ctx, const mi_pos *pos, uint32_t npos,
+                           const uint32_t *counts, uint32_t per_pos, uint32_t seed,
+                           uint32_t epoch, uint32_t policy, int8_t *winners, uint32_t *plies,
+                           uint32_t *hashes, mi_stats *stats);
+
+/* Per-position tallies instead of per-playout winners: tallies[3*i + 0] draws,
+ * [3*i + 1] player-1 wins, [3*i + 2] player-2 wins.  Same streams as mi_run_playouts. */
+int mi_run_playouts_tally(mi_ctx *ctx, const mi_pos *pos, uint32_t npos,
+                          const uint32_t *counts, uint32_t per_pos, uint32_t seed,
+                          uint32_t epoch, uint32_t policy, uint32_t *tallies, mi_stats *stats);
+
+/* Generates the legal move list of each position on the device, in the reference order of
+ * State::genMoves (src/state.cu:239-245).  moves[i*cap .. i*cap + counts[i]) receives the list
+ * of position i.  Returns MI_ERR_CAPACITY if a list is longer than cap.  Verification entry
+ * point, as genMovesTest (src/genMovesTest.cu:26-100). */
+int mi_gen_moves(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, mi_move *moves, uint32_t cap,
+                 uint32_t *counts);
+
+/* Seeded start positions, one per index i < n: k = kmin + (r mod (kmax - kmin + 1)) random plies
+ * from the initial position (r = word 0 of Philox(key (seed, i), counter (0, a, 1, 1))), each ply
+ * choosing move (r_t mod n_t) with r_t = word (t & 3) of Philox(key (seed, i),
+ * counter (t >> 2, a, 1, 0)), stopping early when the game is over.  With reject_terminal, a
+ * finished position is redrawn with attempt a + 1.  Follows genRandomStates
+ * (src/driver.cpp:76-104), whose shared engine is not reproducible. */
+int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin, uint32_t kmax,
+                           uint32_t reject_terminal, mi_pos *out);
+
+/* Device-resident batches (inputs stay in HBM between launches).
+ * mi_stage uploads positions and playout counts once; mi_launch_staged runs the staged batch
+ * and leaves winners on the device; mi_fetch_winners copies them back. */
+int mi_stage(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+             uint32_t per_pos);
+int mi_launch_staged(mi_ctx *ctx, uint32_t seed, uint32_t epoch, uint32_t policy,
+                     mi_stats *stats);
+int mi_fetch_winners(mi_ctx *ctx, int8_t *winners, uint64_t n);
+
+/* Diagnostic: runs the device's exact quotient/remainder routine (used to pick r mod n for
+ * every ply) on caller data, q[i] = r[i] / n[i] and rem[i] = r[i] % n[i], n[i] >= 1.  No
+ * counterpart in the reference, whose draws use the % operator on the host (src/player.cpp:15). */
+int mi_check_divmod(mi_ctx *ctx, const uint32_t *r, const uint32_t *n, uint32_t count,
+                    uint32_t *q, uint32_t *rem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

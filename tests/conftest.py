@@ -1,0 +1,56 @@
+"""Shared fixtures.  Tests marked `gpu` need an MI355X and the built device library."""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X) and the built library")
+
+
+def _ensure_oracle() -> None:
+    so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(so):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"], check=True)
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    _ensure_oracle()
+    from oracle import pyoracle
+    return pyoracle.Oracle()
+
+
+@pytest.fixture(scope="session")
+def golden():
+    def load(name):
+        with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as z:
+            return {k: z[k] for k in z.files}
+    g = {
+        "positions": load("positions.npz"),
+        "movelists": load("movelists.npz"),
+        "playouts": load("playouts.npz"),
+        "scores": load("scores.npz")["scores"],
+    }
+    with open(os.path.join(GOLDEN, "perft.json")) as f:
+        g["perft"] = json.load(f)
+    return g
+
+
+@pytest.fixture(scope="session")
+def engine():
+    from gpu_ai_amd.engine import Engine
+    eng = Engine(0)
+    yield eng
+    eng.close()

@@ -1,0 +1,158 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own host code.
+
+Run in the development container, after `make -C oracle ref` has compiled oracle/_ref from the
+reference sources (src/state.cu, src/state.cpp, src/heuristic.cu, src/mcts.cpp) with asserts on:
+
+    python tests/golden/make_golden.py
+
+Every vector below is produced by the reference build (oracle/_ref/libref_assert.so):
+  positions.npz   start positions of the benchmark configurations (genRandomStates-style,
+                  src/driver.cpp:76-104, with the Philox stream of the oracle)
+  movelists.npz   State::genMoves lists (src/state.cu:239-245) of ~4500 positions, flattened
+  perft.json      move-tree leaf counts from the start position and from midgame positions
+  playouts.npz    per-playout winner / ply count / move-list hash chain, random and heuristic
+  scores.npz      scoreState / scoreMove / getWeight (src/heuristic.cu:20-49) samples
+  MANIFEST.json   SHA-256 of each file
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import pyoracle  # noqa: E402
+
+SEED_POS = 2016
+SEED_PLAY = 12345
+
+
+def main() -> None:
+    ref = pyoracle.Reference(pyoracle.REF_ASSERT_SO)
+
+    # ---- start positions
+    c1 = ref.start_positions(SEED_POS, 1024, 0, 30, False)
+    c2 = ref.start_positions(SEED_POS, 1024, 20, 60, True)
+    np.savez_compressed(os.path.join(HERE, "positions.npz"), c1=c1, c2=c2)
+
+    # ---- move lists: configuration starts plus positions sampled along seeded random games
+    rng = np.random.Generator(np.random.Philox(7))
+    pool = [c1, c2]
+    walk = []
+    for start in np.concatenate([c1[:400], c2[:400]]):
+        p = start.copy()
+        for _ in range(int(rng.integers(1, 120))):
+            moves = ref.gen_moves(p)
+            if len(moves) == 0 or p["counter"] >= 50:
+                break
+            walk.append(p.copy())
+            p = ref.apply_index(p, int(rng.integers(0, len(moves))))
+    walk = np.array(walk, dtype=pyoracle.POS_DTYPE)
+    # keep every capture position and kings position, and a sample of the rest
+    keep = []
+    for i, p in enumerate(walk):
+        ms = ref.gen_moves(p)
+        if len(ms) and (ms[0]["jumps"] > 0 or p["kings"] or rng.random() < 0.05):
+            keep.append(i)
+    walk = walk[np.array(keep)]
+    # random placements: any mix of men and kings of both sides, either side to move, draw
+    # counters around the limit of 50 (src/state.cpp:16-23)
+    placed = np.zeros(3000, dtype=pyoracle.POS_DTYPE)
+    for i in range(len(placed)):
+        occ = rng.random(32) < rng.uniform(0.1, 0.6)
+        side = rng.random(32) < 0.5
+        king = rng.random(32) < rng.uniform(0.0, 0.7)
+        bits = lambda m: int(np.sum((1 << np.arange(32, dtype=np.uint64))[m]))
+        placed[i]["p1"] = bits(occ & side)
+        placed[i]["p2"] = bits(occ & ~side)
+        placed[i]["kings"] = bits(occ & king)
+        placed[i]["turn"] = int(rng.integers(0, 2))
+        placed[i]["counter"] = int(rng.choice([0, 3, 48, 49, 50, 51]))
+    # long king capture sequences (>= 4 jumps), where the landing-square cycle check of
+    # src/state.cu:134-139 decides the list: one or two kings among many opponent men
+    loops = []
+    tries = 0
+    while len(loops) < 300 and tries < 200000:
+        tries += 1
+        p = np.zeros(1, dtype=pyoracle.POS_DTYPE)[0]
+        mover = int(rng.integers(0, 2))
+        ks = rng.choice(32, size=int(rng.integers(1, 3)), replace=False)
+        others = [q for q in rng.choice(32, size=int(rng.integers(5, 12)), replace=False) if q not in ks]
+        own = sum(1 << int(q) for q in ks)
+        opp = sum(1 << int(q) for q in others)
+        p["p1" if mover == 0 else "p2"] = own
+        p["p2" if mover == 0 else "p1"] = opp
+        p["kings"] = own | (opp if rng.random() < 0.3 else 0)
+        p["turn"] = mover
+        ms = ref.gen_moves(p)
+        if len(ms) and ms["jumps"].max() >= 4:
+            loops.append(p)
+    loops = np.array(loops, dtype=pyoracle.POS_DTYPE)
+    positions = np.concatenate(pool + [walk, placed, loops])
+    lists = [ref.gen_moves(p) for p in positions]
+    counts = np.array([len(m) for m in lists], dtype=np.uint32)
+    offsets = np.concatenate([[0], np.cumsum(counts, dtype=np.uint64)]).astype(np.uint64)
+    moves = np.concatenate(lists) if len(lists) else np.zeros(0, dtype=pyoracle.MOVE_DTYPE)
+    np.savez_compressed(os.path.join(HERE, "movelists.npz"), positions=positions, counts=counts,
+                        offsets=offsets, moves=moves)
+
+    # ---- perft
+    perft = {"start": [ref.perft(pyoracle.start_position(), d) for d in range(1, 9)],
+             "midgame": []}
+    for p in c2[:8]:
+        perft["midgame"].append({"position": [int(p["p1"]), int(p["p2"]), int(p["kings"]),
+                                              int(p["turn"]), int(p["counter"])],
+                                 "counts": [ref.perft(p, d) for d in range(1, 6)]})
+    with open(os.path.join(HERE, "perft.json"), "w") as f:
+        json.dump(perft, f, indent=1)
+
+    # ---- playouts
+    out = {}
+    w, tr = ref.run_playouts(c1[:256], per_pos=16, seed=SEED_PLAY, epoch=0,
+                             policy=pyoracle.POLICY_RANDOM)
+    out["random_c1_256x16"] = tr
+    w, tr = ref.run_playouts(c2[:256], per_pos=16, seed=SEED_PLAY, epoch=0,
+                             policy=pyoracle.POLICY_HEURISTIC)
+    out["heuristic_c2_256x16"] = tr
+    counts_uneven = (np.arange(64, dtype=np.uint32) * 7) % 23
+    w, tr = ref.run_playouts(c2[:64], counts=counts_uneven, seed=99, epoch=7,
+                             policy=pyoracle.POLICY_RANDOM)
+    out["random_c2_64_uneven_e7"] = tr
+    out["uneven_counts"] = counts_uneven
+    np.savez_compressed(os.path.join(HERE, "playouts.npz"), **out)
+
+    # ---- heuristic scores
+    rows = []
+    for p in positions[::7]:
+        ms = ref.gen_moves(p)
+        for k in range(len(ms)):
+            s0, s1, wgt = ref.score_move_index(p, k)
+            ss = ref.score_state(p)
+            rows.append((p, k, s0, s1, wgt, ss[0], ss[1]))
+    sc = np.zeros(len(rows), dtype=[("pos", pyoracle.POS_DTYPE), ("index", "<u4"),
+                                    ("d0", "<i4"), ("d1", "<i4"), ("weight", "<f4"),
+                                    ("s0", "<u4"), ("s1", "<u4")])
+    for i, r in enumerate(rows):
+        sc[i] = r
+    np.savez_compressed(os.path.join(HERE, "scores.npz"), scores=sc)
+
+    manifest = {}
+    for name in ("positions.npz", "movelists.npz", "perft.json", "playouts.npz", "scores.npz"):
+        with open(os.path.join(HERE, name), "rb") as f:
+            manifest[name] = hashlib.sha256(f.read()).hexdigest()
+    manifest["source"] = "oracle/_ref/libref_assert.so (reference host code, asserts enabled)"
+    with open(os.path.join(HERE, "MANIFEST.json"), "w") as f:
+        json.dump(manifest, f, indent=1)
+    print(f"positions {len(positions)}, moves {len(moves)}, max n {counts.max()}, "
+          f"max jumps {moves['jumps'].max()}, king loops {len(loops)} ({tries} tries), "
+          f"scores {len(sc)}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,197 @@
+"""Parity of the gfx950 playout path with the oracle and with the reference's golden vectors.
+
+Every test calls the device through the C ABI (gpu_ai_amd.engine -> libgpuai_mi355x.so).  The
+bar is bit-exact: move lists, winners, ply counts and the per-playout hash chain over every
+generated move list and every chosen index.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from gpu_ai_amd.engine import POLICY_HEURISTIC, POLICY_RANDOM, REFILL_QUEUE, REFILL_STATIC
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg: str) -> None:
+    print(f"[gpu] {msg}", file=sys.stderr, flush=True)
+
+
+def trace_mismatches(res, tr) -> int:
+    return int(np.sum((res.winners != tr["winner"]) | (res.plies != tr["plies"]) |
+                      (res.hashes != tr["hash"])))
+
+
+def test_divmod_exact(engine):
+    announce("mi_check_divmod: n in [1, 1100], boundary and random r")
+    rng = np.random.default_rng(1)
+    ns, rs = [], []
+    for n in range(1, 1101):
+        base = [0, 1, n - 1, n, n + 1, 2**31 - 1, 2**31, 2**32 - 1, 2**32 - n, 2**32 - n - 1]
+        q = (2**32 - 1) // n
+        base += [q * n - 1, q * n, (q // 2) * n, (q // 2) * n - 1]
+        r = np.array([x % 2**32 for x in base] + list(rng.integers(0, 2**32, 64, dtype=np.uint64)),
+                     dtype=np.uint64)
+        rs.append(r)
+        ns.append(np.full(len(r), n, dtype=np.uint64))
+    r = np.concatenate(rs).astype(np.uint32)
+    n = np.concatenate(ns).astype(np.uint32)
+    q, rem = engine.check_divmod(r, n)
+    assert np.array_equal(q, r // n)
+    assert np.array_equal(rem, r % n)
+
+
+def test_gen_moves_golden(engine, golden):
+    ml = golden["movelists"]
+    pos, counts, offs, moves = ml["positions"], ml["counts"], ml["offsets"], ml["moves"]
+    announce(f"mi_gen_moves on {len(pos)} golden positions")
+    got, got_counts = engine.gen_moves(pos, cap=32)
+    assert np.array_equal(got_counts, counts)
+    flat = np.concatenate([got[i, :counts[i]] for i in range(len(pos))])
+    assert flat.tobytes() == moves.tobytes()
+
+
+def test_gen_moves_capacity_error(engine, golden):
+    from gpu_ai_amd.engine import MI_ERR_CAPACITY, MiError
+    pos = golden["positions"]["c1"][:64]
+    with pytest.raises(MiError) as e:
+        engine.gen_moves(pos, cap=1)
+    assert e.value.code == MI_ERR_CAPACITY
+
+
+def golden_sets(golden):
+    g, pl = golden["positions"], golden["playouts"]
+    return [
+        ("random c1[:256]x16", g["c1"][:256], dict(per_pos=16, seed=12345, epoch=0, policy=0),
+         pl["random_c1_256x16"]),
+        ("heuristic c2[:256]x16", g["c2"][:256], dict(per_pos=16, seed=12345, epoch=0, policy=1),
+         pl["heuristic_c2_256x16"]),
+        ("random c2[:64] uneven e7", g["c2"][:64],
+         dict(counts=pl["uneven_counts"], seed=99, epoch=7, policy=0), pl["random_c2_64_uneven_e7"]),
+    ]
+
+
+def test_traced_playouts_match_reference_golden(engine, golden):
+    for name, pos, kw, expect in golden_sets(golden):
+        announce(f"traced {name}")
+        res = engine.run_traced(pos, **kw)
+        assert trace_mismatches(res, expect) == 0, name
+        assert res.stats["plies"] == int(expect["plies"].sum(dtype=np.uint64))
+
+
+@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
+def test_traced_full_positions_vs_oracle(engine, oracle, golden, policy):
+    pos = golden["positions"]["c1" if policy == POLICY_RANDOM else "c2"]
+    announce(f"traced 1024 positions x 64, policy {policy}")
+    res = engine.run_traced(pos, per_pos=64, seed=2024, epoch=1, policy=policy)
+    _, tr = oracle.run_playouts(pos, per_pos=64, seed=2024, epoch=1, policy=policy)
+    assert trace_mismatches(res, tr) == 0
+    assert res.stats["max_moves"] == int(tr["max_n"].max())
+
+
+def test_config1_random_full_bitexact(engine, oracle, golden):
+    pos = golden["positions"]["c1"]
+    announce("config 1: 1024 positions x 1024 random playouts")
+    res = engine.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=POLICY_RANDOM)
+    w, tr = oracle.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=POLICY_RANDOM)
+    assert np.array_equal(res.winners, w)
+    assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
+
+
+def test_config2_heuristic_full_bitexact(engine, oracle, golden):
+    pos = golden["positions"]["c2"]
+    announce("config 2: 1024 positions x 1024 heuristic playouts")
+    res = engine.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=POLICY_HEURISTIC)
+    w, tr = oracle.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=POLICY_HEURISTIC)
+    assert np.array_equal(res.winners, w)
+    assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
+
+
+def test_config0_bitexact(engine, oracle, golden):
+    pos = golden["positions"]["c1"][:64]
+    announce("config 0: 64 positions x 4096 random playouts")
+    res = engine.run_playouts(pos, per_pos=4096, seed=12345, epoch=0, policy=POLICY_RANDOM)
+    w, _ = oracle.run_playouts(pos, per_pos=4096, seed=12345, epoch=0, policy=POLICY_RANDOM,
+                               traces=False)
+    assert np.array_equal(res.winners, w)
+
+
+@pytest.mark.parametrize("block,bpc,refill", [(64, 1, REFILL_STATIC), (64, 16, REFILL_QUEUE),
+                                              (256, 8, REFILL_STATIC), (1024, 2, REFILL_QUEUE),
+                                              (512, 4, REFILL_QUEUE)])
+def test_launch_shape_does_not_change_results(engine, golden, block, bpc, refill):
+    pos = golden["positions"]["c2"][:256]
+    announce(f"tuning block {block} blocks/CU {bpc} refill {refill}")
+    base = engine.run_traced(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
+    engine.set_tuning(block, bpc, refill)
+    try:
+        other = engine.run_traced(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
+        assert other.stats["block_size"] == block and other.stats["refill"] == refill
+    finally:
+        engine.set_tuning(0, 0, REFILL_QUEUE)
+    assert np.array_equal(base.winners, other.winners)
+    assert np.array_equal(base.hashes, other.hashes)
+
+
+def test_uneven_counts_empty_and_single(engine, oracle, golden):
+    pos = golden["positions"]["c1"][:40]
+    counts = np.array([0, 1, 0, 0, 5, 63, 64, 65, 0, 1000] + [3] * 30, dtype=np.uint32)
+    announce("uneven counts with zeros")
+    res = engine.run_traced(pos, counts=counts, seed=5, epoch=0, policy=POLICY_RANDOM)
+    _, tr = oracle.run_playouts(pos, counts=counts, seed=5, epoch=0, policy=POLICY_RANDOM)
+    assert trace_mismatches(res, tr) == 0
+    announce("empty batch")
+    empty = engine.run_playouts(pos, counts=np.zeros(40, dtype=np.uint32))
+    assert len(empty.winners) == 0 and empty.stats["playouts"] == 0
+    none = engine.run_playouts(pos[:0], per_pos=10)
+    assert len(none.winners) == 0
+    announce("single playout")
+    one = engine.run_traced(pos[3:4], per_pos=1, seed=11, epoch=4, policy=POLICY_HEURISTIC)
+    _, tr1 = oracle.run_playouts(pos[3:4], per_pos=1, seed=11, epoch=4, policy=POLICY_HEURISTIC)
+    assert trace_mismatches(one, tr1) == 0
+
+
+def test_terminal_and_odd_positions(engine, oracle, golden):
+    ml = golden["movelists"]["positions"]
+    # random placements and king-loop positions sit at the end of the golden set; they include
+    # finished games (counter >= 50, no moves) and men on rows a game never leaves them on
+    odd = ml[-3300:]
+    for policy in (POLICY_RANDOM, POLICY_HEURISTIC):
+        announce(f"odd positions x 8, policy {policy}")
+        res = engine.run_traced(odd, per_pos=8, seed=31, epoch=0, policy=policy)
+        _, tr = oracle.run_playouts(odd, per_pos=8, seed=31, epoch=0, policy=policy)
+        assert trace_mismatches(res, tr) == 0
+    finished = odd[odd["counter"] >= 50]
+    assert len(finished) > 0
+    res = engine.run_traced(finished, per_pos=2, seed=1)
+    assert np.all(res.winners == -1) and np.all(res.plies == 0)
+
+
+def test_tally_staged_and_repeat(engine, golden):
+    pos = golden["positions"]["c1"][:300]
+    counts = (np.arange(300, dtype=np.uint32) * 37) % 200
+    announce("tally / staged / repeat")
+    res = engine.run_playouts(pos, counts=counts, seed=77, epoch=3, policy=POLICY_HEURISTIC)
+    tallies, _ = engine.run_tally(pos, counts=counts, seed=77, epoch=3, policy=POLICY_HEURISTIC)
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    for i in range(len(pos)):
+        w = res.winners[offs[i]:offs[i + 1]]
+        assert list(tallies[i]) == [int((w == -1).sum()), int((w == 0).sum()), int((w == 1).sum())]
+    again = engine.run_playouts(pos, counts=counts, seed=77, epoch=3, policy=POLICY_HEURISTIC)
+    assert np.array_equal(res.winners, again.winners)
+    engine.stage(pos, counts=counts)
+    engine.launch_staged(seed=77, epoch=3, policy=POLICY_HEURISTIC)
+    assert np.array_equal(engine.fetch_winners(), res.winners)
+    other = engine.run_playouts(pos, counts=counts, seed=77, epoch=4, policy=POLICY_HEURISTIC)
+    assert not np.array_equal(other.winners, res.winners)
+
+
+def test_device_start_positions_match_reference_golden(engine, golden):
+    announce("mi_gen_start_positions for configs 0/1 and 2")
+    c1 = engine.start_positions(2016, 1024, 0, 30, False)
+    c2 = engine.start_positions(2016, 1024, 20, 60, True)
+    assert c1.tobytes() == golden["positions"]["c1"].tobytes()
+    assert c2.tobytes() == golden["positions"]["c2"].tobytes()
