@@ -98,13 +98,27 @@ constexpr uint32_t TAG_START = 1u;
 constexpr uint32_t TAG_SEARCH = 2u;
 constexpr uint32_t TAG_MLP = 3u;
 
-// Exact r mod n for 1 <= n.  On the device: one double-precision reciprocal estimate, a floor,
-// an exact fused remainder, and one correction each way (the estimate's relative error is far
-// below 2^-33, so the floor is off by at most one).
+// Reciprocal of n in double precision: the hardware estimate (v_rcp_f64, a seed of roughly
+// single precision) refined by two Newton-Raphson steps, each of which squares the relative
+// error, so the result is within a few units of 2^-52 of 1/n.
+CK_HD double recip(double dn) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double x = __builtin_amdgcn_rcp(dn);
+#else
+  double x = 1.0 / dn;
+#endif
+  x = __builtin_fma(x, __builtin_fma(-dn, x, 1.0), x);
+  x = __builtin_fma(x, __builtin_fma(-dn, x, 1.0), x);
+  return x;
+}
+
+// Exact r mod n for 1 <= n.  On the device: the refined reciprocal, a floor, an exact fused
+// remainder, and one correction each way (r / n is estimated to far better than one unit, so the
+// floor is off by at most one).
 CK_HD uint32_t mod_u32(uint32_t r, uint32_t n) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double dn = (double)n, dr = (double)r;
-  double q = __builtin_floor(dr * __builtin_amdgcn_rcp(dn));
+  double q = __builtin_floor(dr * recip(dn));
   int32_t rem = (int32_t)__builtin_fma(-q, dn, dr);
   rem += (rem < 0) ? (int32_t)n : 0;
   rem -= (rem >= (int32_t)n) ? (int32_t)n : 0;
@@ -118,7 +132,7 @@ CK_HD uint32_t mod_u32(uint32_t r, uint32_t n) {
 CK_HD void divmod_u32(uint32_t r, uint32_t n, uint32_t &q, uint32_t &rem) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double dn = (double)n, dr = (double)r;
-  double qd = __builtin_floor(dr * __builtin_amdgcn_rcp(dn));
+  double qd = __builtin_floor(dr * recip(dn));
   int32_t m = (int32_t)__builtin_fma(-qd, dn, dr);
   uint32_t qq = (uint32_t)(int64_t)qd;
   if (m < 0) {
