@@ -176,7 +176,7 @@ def test_tally_staged_and_repeat(engine, golden):
     announce("tally / staged / repeat")
     res = engine.run_playouts(pos, counts=counts, seed=77, epoch=3, policy=POLICY_HEURISTIC)
     tallies, _ = engine.run_tally(pos, counts=counts, seed=77, epoch=3, policy=POLICY_HEURISTIC)
-    offs = np.concatenate([[0], np.cumsum(counts)])
+    offs = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
     for i in range(len(pos)):
         w = res.winners[offs[i]:offs[i + 1]]
         assert list(tallies[i]) == [int((w == -1).sum()), int((w == 0).sum()), int((w == 1).sum())]
