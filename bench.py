@@ -85,6 +85,8 @@ def main() -> int:
     ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run a plain copy kernel of BYTES (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the playout path runs on a single MI355X here "
@@ -148,6 +150,10 @@ def main() -> int:
                      "device": {"compute_units": info["compute_units"],
                                 "clock_mhz": info["clock_khz"] / 1e3}},
     }
+    if args.copy_calibration:
+        ms = eng.copy_calibration(args.copy_calibration, 5)
+        line["copy_calibration"] = {"bytes": args.copy_calibration, "avg_ms": round(ms, 4),
+                                    "GB/s": round(2 * args.copy_calibration / ms / 1e6, 1)}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:

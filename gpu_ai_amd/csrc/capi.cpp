@@ -493,4 +493,30 @@ int mi_check_divmod(mi_ctx *ctx, const uint32_t *r, const uint32_t *n, uint32_t 
   return rc;
 }
 
+int mi_copy_calibration(mi_ctx *ctx, uint64_t bytes, uint32_t repeats, float *avg_ms) {
+  if (!ctx || !avg_ms || bytes == 0 || bytes % 16 || repeats == 0) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  void *buf = nullptr;
+  MI_HIP(ctx, hipMalloc(&buf, 2 * bytes));
+  const uint4 *src = (const uint4 *)buf;
+  uint4 *dst = (uint4 *)((char *)buf + bytes);
+  hipError_t e = hipMemsetAsync(buf, 1, bytes, ctx->stream);
+  float total = 0.f;
+  for (uint32_t i = 0; i < repeats && e == hipSuccess; i++) {
+    e = hipEventRecord(ctx->ev_start, ctx->stream);
+    if (e == hipSuccess) e = gpuai::launch_copy(src, dst, bytes / 16, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_stop);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    total += ms;
+  }
+  int rc = e == hipSuccess ? MI_OK : hip_fail(ctx, e, "copy calibration");
+  (void)hipFree(buf);
+  *avg_ms = total / (float)repeats;
+  return rc;
+}
+
 }  // extern "C"

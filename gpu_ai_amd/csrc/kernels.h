@@ -42,6 +42,7 @@ hipError_t launch_start_positions(uint32_t seed, uint32_t n, uint32_t kmin, uint
                                   hipStream_t stream);
 hipError_t launch_divmod(const uint32_t *r, const uint32_t *n, uint32_t count, uint32_t *q,
                          uint32_t *rem, hipStream_t stream);
+hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n16, hipStream_t stream);
 hipError_t launch_tally(const int8_t *winners, const uint32_t *offs, uint32_t npos,
                         uint32_t per_pos, uint32_t *tallies, hipStream_t stream);
 

@@ -270,6 +270,14 @@ __global__ void __launch_bounds__(256) k_divmod(const uint32_t *__restrict__ r,
   rem[i] = ck::mod_u32(r[i], n[i]) == rr ? rr : 0xFFFFFFFFu;
 }
 
+// Plain grid-stride copy of 16-byte words (traffic-counter calibration).
+__global__ void __launch_bounds__(256) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                              uint64_t n16) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    dst[i] = src[i];
+}
+
 template <uint32_t POLICY, bool TRACE>
 hipError_t launch_refill(const PlayoutArgs &args, uint32_t refill, uint32_t grid, uint32_t block,
                          hipStream_t stream) {
@@ -322,6 +330,14 @@ hipError_t launch_divmod(const uint32_t *r, const uint32_t *n, uint32_t count, u
   const uint32_t block = 256;
   const uint32_t grid = (count + block - 1) / block;
   hipLaunchKernelGGL(k_divmod, dim3(grid), dim3(block), 0, stream, r, n, count, q, rem);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n16, hipStream_t stream) {
+  const uint32_t block = 256;
+  const uint64_t want = (n16 + block - 1) / block;
+  const uint32_t grid = (uint32_t)(want < 16384u ? want : 16384u);
+  hipLaunchKernelGGL(k_copy, dim3(grid ? grid : 1u), dim3(block), 0, stream, src, dst, n16);
   return hipGetLastError();
 }
 

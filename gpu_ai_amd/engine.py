@@ -76,6 +76,8 @@ _SIGS = {
     "mi_fetch_winners": ([_P, _P, ctypes.c_uint64], ctypes.c_int),
     "mi_check_divmod": ([_P, _P, _P, _U32, _P, _P], ctypes.c_int),
     "mi_gen_start_positions": ([_P, _U32, _U32, _U32, _U32, _U32, _P], ctypes.c_int),
+    "mi_copy_calibration": ([_P, ctypes.c_uint64, _U32, ctypes.POINTER(ctypes.c_float)],
+                            ctypes.c_int),
 }
 
 _lib = None
@@ -242,6 +244,11 @@ class Engine:
         w = np.empty(self._staged_total, dtype=np.int8)
         self._check(self.lib.mi_fetch_winners(self.ctx, _ptr(w), self._staged_total))
         return w
+
+    def copy_calibration(self, nbytes: int, repeats: int = 5) -> float:
+        ms = ctypes.c_float()
+        self._check(self.lib.mi_copy_calibration(self.ctx, nbytes, repeats, ctypes.byref(ms)))
+        return ms.value
 
     def check_divmod(self, r, n) -> tuple[np.ndarray, np.ndarray]:
         r = np.ascontiguousarray(r, dtype=np.uint32)

@@ -154,6 +154,11 @@ int mi_fetch_winners(mi_ctx *ctx, int8_t *winners, uint64_t n);
 int mi_check_divmod(mi_ctx *ctx, const uint32_t *r, const uint32_t *n, uint32_t count,
                     uint32_t *q, uint32_t *rem);
 
+/* Diagnostic: copies `bytes` (a multiple of 16) between two device buffers with a plain
+ * vectorised copy kernel (k_copy), `repeats` times, and returns the average kernel time.  Used to
+ * calibrate the HBM traffic counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
+int mi_copy_calibration(mi_ctx *ctx, uint64_t bytes, uint32_t repeats, float *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
