@@ -416,6 +416,66 @@ int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin
   return MI_OK;
 }
 
+int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                uint32_t per_pos, uint32_t policy, uint32_t reps, mi_tune_result *results,
+                uint32_t max_results, uint32_t *nresults, mi_tuning *best) {
+  if (!ctx) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  int rc;
+  if ((rc = bad_policy(ctx, policy))) return rc;
+  if (reps == 0) reps = 3;
+  uint64_t total = 0;
+  if ((rc = plan_batch(ctx, pos, npos, counts, per_pos, total))) return rc;
+  if (total == 0) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "empty tuning batch");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  if ((rc = upload_batch(ctx, pos, npos, counts, total))) return rc;
+  ctx->staged = false;
+  const mi_tuning saved = ctx->tuning;
+  const uint32_t blocks[] = {64u, 128u, 256u, 512u, 1024u};
+  const uint32_t waves_per_simd[] = {1u, 2u, 4u, 8u};
+  mi_tuning best_t = saved;
+  float best_ms = 3.0e38f;
+  uint32_t n = 0;
+  for (uint32_t refill = 0; refill < 2; refill++) {
+    for (uint32_t b : blocks) {
+      for (uint32_t w : waves_per_simd) {
+        const uint32_t waves_per_cu = 4u * w;           // four SIMDs per CU
+        if ((b / 64u) > waves_per_cu) continue;          // a block larger than the residency
+        mi_tuning t = {b, waves_per_cu * 64u / b, refill, 0u};
+        ctx->tuning = t;
+        std::vector<float> ms;
+        for (uint32_t r = 0; r < reps; r++) {
+          mi_stats st;
+          rc = run_kernel(ctx, npos, per_pos, total, counts != nullptr, 0x7A5Eu, 0xFFFF0000u + r,
+                          policy, false, &st);
+          if (rc) {
+            ctx->tuning = saved;
+            return rc;
+          }
+          ms.push_back(st.kernel_ms);
+        }
+        std::sort(ms.begin(), ms.end());
+        const float med = ms[ms.size() / 2];
+        if (results && n < max_results) {
+          results[n].tuning = t;
+          results[n].kernel_ms = med;
+          results[n].playouts_per_s = (float)((double)total / (med * 1e-3));
+        }
+        n++;
+        if (med < best_ms) {
+          best_ms = med;
+          best_t = t;
+        }
+      }
+    }
+  }
+  ctx->tuning = best_t;
+  if (nresults) *nresults = n;
+  if (best) *best = best_t;
+  return MI_OK;
+}
+
 int mi_stage(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
              uint32_t per_pos) {
   if (!ctx) return MI_ERR_INVALID_ARGUMENT;

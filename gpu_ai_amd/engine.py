@@ -38,6 +38,11 @@ class MiTuning(ctypes.Structure):
                 ("refill", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
+class MiTuneResult(ctypes.Structure):
+    _fields_ = [("tuning", MiTuning), ("kernel_ms", ctypes.c_float),
+                ("playouts_per_s", ctypes.c_float)]
+
+
 class MiStats(ctypes.Structure):
     _fields_ = [("playouts", ctypes.c_uint64), ("plies", ctypes.c_uint64),
                 ("kernel_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
@@ -76,6 +81,8 @@ _SIGS = {
     "mi_fetch_winners": ([_P, _P, ctypes.c_uint64], ctypes.c_int),
     "mi_check_divmod": ([_P, _P, _P, _U32, _P, _P], ctypes.c_int),
     "mi_gen_start_positions": ([_P, _U32, _U32, _U32, _U32, _U32, _P], ctypes.c_int),
+    "mi_autotune": ([_P, _P, _U32, _P, _U32, _U32, _U32, _P, _U32, _P, ctypes.POINTER(MiTuning)],
+                    ctypes.c_int),
     "mi_copy_calibration": ([_P, ctypes.c_uint64, _U32, ctypes.POINTER(ctypes.c_float)],
                             ctypes.c_int),
 }
@@ -244,6 +251,22 @@ class Engine:
         w = np.empty(self._staged_total, dtype=np.int8)
         self._check(self.lib.mi_fetch_winners(self.ctx, _ptr(w), self._staged_total))
         return w
+
+    def autotune(self, pos, per_pos: int | None = None, counts=None, policy: int = POLICY_RANDOM,
+                 reps: int = 3) -> tuple[dict, list[dict]]:
+        """Sweeps launch shapes on this batch; the fastest becomes the context's tuning."""
+        pos = as_positions(pos)
+        c, pp, _ = self._counts(len(pos), per_pos, counts)
+        res = (MiTuneResult * 64)()
+        nres = _U32()
+        best = MiTuning()
+        self._check(self.lib.mi_autotune(self.ctx, _ptr(pos), len(pos), _ptr(c), pp, policy, reps,
+                                         res, 64, ctypes.byref(nres), ctypes.byref(best)))
+        rows = [{"block_size": r.tuning.block_size, "blocks_per_cu": r.tuning.blocks_per_cu,
+                 "refill": r.tuning.refill, "kernel_ms": r.kernel_ms,
+                 "playouts_per_s": r.playouts_per_s} for r in res[:min(nres.value, 64)]]
+        return ({"block_size": best.block_size, "blocks_per_cu": best.blocks_per_cu,
+                 "refill": best.refill}, rows)
 
     def copy_calibration(self, nbytes: int, repeats: int = 5) -> float:
         ms = ctypes.c_float()

@@ -83,6 +83,13 @@ typedef struct mi_stats {
   uint32_t refill;         /* refill policy used */
 } mi_stats;
 
+/* One candidate launch shape measured by mi_autotune. */
+typedef struct mi_tune_result {
+  mi_tuning tuning;
+  float kernel_ms;        /* median device time over the repetitions */
+  float playouts_per_s;   /* batch size / kernel_ms */
+} mi_tune_result;
+
 typedef struct mi_ctx mi_ctx;
 
 /* Creates a context on HIP device `device`.  `tuning` may be NULL.  No counterpart in the
@@ -138,6 +145,16 @@ int mi_gen_moves(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, mi_move *moves, 
  * (src/driver.cpp:76-104), whose shared engine is not reproducible. */
 int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin, uint32_t kmax,
                            uint32_t reject_terminal, mi_pos *out);
+
+/* Autotuner (no counterpart in the reference, whose OptimalPlayoutDriver picks between whole
+ * drivers by a runtime model, src/playout.cpp:80-166).  Runs the given batch under every
+ * candidate launch shape -- workgroup sizes 64..1024, 1..8 resident waves per SIMD (which sets the
+ * playouts each lane runs) and both refill policies -- `reps` times each, and makes the one with
+ * the smallest median kernel time the context's tuning.  `results` (may be NULL) receives up to
+ * `max_results` candidates; `nresults` the number measured; `best` (may be NULL) the winner. */
+int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                uint32_t per_pos, uint32_t policy, uint32_t reps, mi_tune_result *results,
+                uint32_t max_results, uint32_t *nresults, mi_tuning *best);
 
 /* Device-resident batches (inputs stay in HBM between launches).
  * mi_stage uploads positions and playout counts once; mi_launch_staged runs the staged batch
