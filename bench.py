@@ -84,6 +84,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shape instead of tuning on the batch")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a plain copy kernel of BYTES (traffic-counter calibration)")
@@ -99,6 +101,10 @@ def main() -> int:
     eng = Engine(0)
     info = eng.device_info()
     pos = eng.start_positions(POS_SEED, cfg["n"], cfg["kmin"], cfg["kmax"], cfg["reject"])
+    tuned = None
+    if not args.no_autotune:
+        # the autotuner times every candidate launch shape on this batch (its own Philox epochs)
+        tuned, _ = eng.autotune(pos, per_pos=cfg["per_pos"], policy=cfg["policy"], reps=3)
     total = eng.stage(pos, per_pos=cfg["per_pos"])
     for i in range(args.warmup):
         eng.launch_staged(seed=PLAY_SEED, epoch=1000 + i, policy=cfg["policy"])
@@ -140,7 +146,8 @@ def main() -> int:
                    "policy": "random" if cfg["policy"] == 0 else "heuristic",
                    "position_seed": POS_SEED, "playout_seed": PLAY_SEED,
                    "block_size": tuning["block_size"], "blocks_per_cu": tuning["blocks_per_cu"],
-                   "refill": "queue" if tuning["refill"] == 1 else "static"},
+                   "refill": "queue" if tuning["refill"] == 1 else "static",
+                   "defer_min": tuning["defer_min"], "autotuned": tuned is not None},
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 4),
                      "peak": round(peak_spec / 1e12, 3), "unit": "Tops/s (int32 lane-ops)",
                      "frac": round(achieved / peak_spec, 4), "traffic": traffic,

@@ -30,7 +30,7 @@ struct mi_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-  mi_tuning tuning = {256u, 8u, MI_REFILL_QUEUE, 0u};
+  mi_tuning tuning = {512u, 2u, MI_REFILL_QUEUE, 8u};
   uint32_t cus = 0, wave = 64, clock_khz = 0;
   DevBuf pos, offs, winners, plies, hashes, work, tallies, moves, counts;
   // staged batch
@@ -83,6 +83,8 @@ int check_tuning(mi_ctx *ctx, const mi_tuning &t) {
     return fail(ctx, MI_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 64]");
   if (t.refill != MI_REFILL_STATIC && t.refill != MI_REFILL_QUEUE)
     return fail(ctx, MI_ERR_INVALID_ARGUMENT, "unknown refill policy");
+  if (t.defer_min < 1 || t.defer_min > 64)
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "defer_min must be in [1, 64]");
   return MI_OK;
 }
 
@@ -153,6 +155,7 @@ int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
   a.plies = trace ? (uint32_t *)ctx->plies.ptr : nullptr;
   a.hashes = trace ? (uint32_t *)ctx->hashes.ptr : nullptr;
   a.work = (gpuai::WorkWords *)ctx->work.ptr;
+  a.defer_min = ctx->tuning.defer_min;
   const uint32_t grid = grid_for(ctx, total);
   MI_HIP(ctx, gpuai::launch_reset(a.work, ctx->stream));
   MI_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
@@ -271,6 +274,7 @@ int mi_create(mi_ctx **out, int device, const mi_tuning *tuning) {
     mi_tuning t = *tuning;
     if (!t.block_size) t.block_size = ctx->tuning.block_size;
     if (!t.blocks_per_cu) t.blocks_per_cu = ctx->tuning.blocks_per_cu;
+    if (!t.defer_min) t.defer_min = ctx->tuning.defer_min;
     if (check_tuning(ctx, t)) {
       delete ctx;
       return MI_ERR_INVALID_ARGUMENT;
@@ -310,8 +314,9 @@ int mi_set_tuning(mi_ctx *ctx, const mi_tuning *tuning) {
   if (!ctx || !tuning) return MI_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->mu);
   mi_tuning t = *tuning;
-  if (!t.block_size) t.block_size = 256;
-  if (!t.blocks_per_cu) t.blocks_per_cu = 8;
+  if (!t.block_size) t.block_size = 512;
+  if (!t.blocks_per_cu) t.blocks_per_cu = 2;
+  if (!t.defer_min) t.defer_min = 8;
   int rc = check_tuning(ctx, t);
   if (rc) return rc;
   ctx->tuning = t;
@@ -437,12 +442,15 @@ int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
   mi_tuning best_t = saved;
   float best_ms = 3.0e38f;
   uint32_t n = 0;
+  const uint32_t defers[] = {1u, 8u, 16u};
   for (uint32_t refill = 0; refill < 2; refill++) {
     for (uint32_t b : blocks) {
       for (uint32_t w : waves_per_simd) {
+       for (uint32_t dm : defers) {
         const uint32_t waves_per_cu = 4u * w;           // four SIMDs per CU
         if ((b / 64u) > waves_per_cu) continue;          // a block larger than the residency
-        mi_tuning t = {b, waves_per_cu * 64u / b, refill, 0u};
+        if (policy != MI_POLICY_RANDOM && dm != 8u) continue;  // deferral is a random-policy knob
+        mi_tuning t = {b, waves_per_cu * 64u / b, refill, dm};
         ctx->tuning = t;
         std::vector<float> ms;
         for (uint32_t r = 0; r < reps; r++) {
@@ -467,6 +475,7 @@ int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
           best_ms = med;
           best_t = t;
         }
+       }
       }
     }
   }

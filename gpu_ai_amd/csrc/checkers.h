@@ -502,6 +502,75 @@ CK_HD Choice slot_choice(const Lane &L, const Gen &g, uint32_t sq, uint32_t sl) 
   return c;
 }
 
+// Leaf count of the capture tree of one origin.
+CK_HD uint32_t count_walk(const Lane &L, const Gen &g, uint32_t s) {
+  uint32_t n = 0;
+  auto leaf = [&](uint32_t, uint32_t, uint32_t, uint32_t) {
+    n++;
+    return false;
+  };
+  walk(s, getbit(L.kings, s), g.cap, leaf);
+  return n;
+}
+
+// Leaf `k` (mover-frame order) of the capture tree of origin s.
+CK_HD Choice select_in_walk(const Lane &L, const Gen &g, uint32_t s, uint32_t k) {
+  Choice c = {s, 0u, 0u};
+  uint32_t left = k;
+  auto leaf = [&](uint32_t to, uint32_t removed, uint32_t, uint32_t) {
+    if (left == 0u) {
+      c.to = to;
+      c.removed = removed;
+      return true;
+    }
+    left--;
+    return false;
+  };
+  walk(s, getbit(L.kings, s), g.cap, leaf);
+  return c;
+}
+
+// Move count of a capture ply with multi-jump origins: single-jump origins contribute their slot
+// bits, each multi-jump origin its walked leaf count.  The first four such counts are returned
+// packed one per byte (in square order) for the selection pass.
+CK_HD uint32_t count_mixed(const Lane &L, const Gen &g, uint32_t &packed) {
+  uint32_t n = popc(g.slot[0] & ~g.multi) + popc(g.slot[1] & ~g.multi) +
+               popc(g.slot[2] & ~g.multi) + popc(g.slot[3] & ~g.multi);
+  uint32_t o = g.multi, i = 0;
+  packed = 0;
+  while (o) {
+    const uint32_t s = ctz(o);
+    o &= o - 1u;
+    const uint32_t c = count_walk(L, g, s);
+    if (i < 4u) packed |= (c < 255u ? c : 255u) << (8u * i);
+    i++;
+    n += c;
+  }
+  return n;
+}
+
+// k-th move (mover-frame order) of a capture ply with multi-jump origins.
+CK_HD Choice select_mixed(const Lane &L, const Gen &g, uint32_t k, uint32_t packed) {
+  uint32_t sm[4] = {g.slot[0] & ~g.multi, g.slot[1] & ~g.multi, g.slot[2] & ~g.multi,
+                    g.slot[3] & ~g.multi};
+  uint32_t o = g.multi, acc = 0, i = 0;
+  while (o) {
+    const uint32_t s = ctz(o);
+    o &= o - 1u;
+    const uint32_t below = bit(s) - 1u;
+    const uint32_t start = acc + popc(sm[0] & below) + popc(sm[1] & below) +
+                           popc(sm[2] & below) + popc(sm[3] & below);
+    if (k < start) break;
+    const uint32_t c = i < 4u ? (packed >> (8u * i)) & 255u : count_walk(L, g, s);
+    if (k < start + c) return select_in_walk(L, g, s, k - start);
+    acc += c;
+    i++;
+  }
+  uint32_t sq, sl;
+  select_slot(sm, k - acc, sq, sl);
+  return slot_choice(L, g, sq, sl);
+}
+
 // Weight comparison a/b vs c/d for non-negative integers with a, c >= 1 (b == 0 is +inf).
 CK_HD int cmp_weight(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
   uint32_t l = a * d, r = c * b;
@@ -523,7 +592,8 @@ CK_HD PlyResult ply_random(Lane &L, uint32_t r, uint32_t *list_dig) {
   Gen g;
   analyse(L, g);
   PlyResult res;
-  uint32_t n = g.multi ? count_walks(L, g) : slot_count(g.slot);
+  uint32_t packed = 0;
+  uint32_t n = g.multi ? count_mixed(L, g, packed) : slot_count(g.slot);
   res.n = n;
   if (TRACE) *list_dig = list_digest(L, g, n);
   if (n == 0u || L.counter >= 50u) {  // src/state.cpp:16-23
@@ -539,12 +609,48 @@ CK_HD PlyResult ply_random(Lane &L, uint32_t r, uint32_t *list_dig) {
   uint32_t km = L.turn ? n - 1u - k : k;
   Choice c;
   if (g.multi) {
-    c = select_walk(L, g, km);
+    c = select_mixed(L, g, km, packed);
   } else {
     uint32_t sq, sl;
     select_slot(g.slot, km, sq, sl);
     c = slot_choice(L, g, sq, sl);
   }
+  apply(L, c.from, c.to, c.removed);
+  return res;
+}
+
+// The common part of a random ply: plays it unless it is a capture ply with a multi-jump origin,
+// in which case the position is left untouched and `deferred` is returned set (the caller plays
+// it later with ply_random and the same word).
+template <bool TRACE>
+CK_HD PlyResult ply_random_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool &deferred) {
+  Gen g;
+  analyse(L, g);
+  PlyResult res;
+  deferred = g.multi != 0u;
+  if (deferred) {
+    res.n = 0u;
+    res.chosen = 0u;
+    res.over = 0u;
+    res.winner = 0;
+    return res;
+  }
+  const uint32_t n = slot_count(g.slot);
+  res.n = n;
+  if (TRACE) *list_dig = list_digest(L, g, n);
+  if (n == 0u || L.counter >= 50u) {
+    res.over = 1u;
+    res.chosen = 0xFFFFu;
+    res.winner = L.counter < 50u ? (int32_t)(L.turn ^ 1u) : -1;
+    return res;
+  }
+  const uint32_t k = mod_u32(r, n);
+  res.chosen = k;
+  res.over = 0u;
+  res.winner = 0;
+  uint32_t sq, sl;
+  select_slot(g.slot, L.turn ? n - 1u - k : k, sq, sl);
+  const Choice c = slot_choice(L, g, sq, sl);
   apply(L, c.from, c.to, c.removed);
   return res;
 }

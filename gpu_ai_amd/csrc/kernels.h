@@ -30,6 +30,7 @@ struct PlayoutArgs {
   uint32_t *plies;         // total (trace mode only)
   uint32_t *hashes;        // total (trace mode only)
   WorkWords *work;
+  uint32_t defer_min;      // deferred multi-jump lanes that trigger a walk phase
 };
 
 hipError_t launch_reset(WorkWords *work, hipStream_t stream);

@@ -48,32 +48,52 @@ __device__ __forceinline__ void locate(const PlayoutArgs &a, uint32_t g, uint32_
   }
 }
 
+// Philox words of a lane: `cur` holds block `blk` of its stream (words 4*blk .. 4*blk+3),
+// `nxt` block blk+1 when nxt_ok.  Blocks are generated only on wave iterations that are
+// multiples of four (wave-uniform work); since a lane consumes at most one word per iteration,
+// holding the current and the next block always covers the next four iterations.
+struct Words {
+  uint32_t c0, c1, c2, c3, n0, n1, n2, n3;
+  uint32_t blk;
+  bool cur_ok, nxt_ok;
+};
+
+__device__ __forceinline__ uint32_t take_word(Words &w, uint32_t ply) {
+  if ((ply >> 2) != w.blk) {
+    w.c0 = w.n0;
+    w.c1 = w.n1;
+    w.c2 = w.n2;
+    w.c3 = w.n3;
+    w.blk++;
+    w.nxt_ok = false;
+  }
+  const uint32_t q = ply & 3u;
+  return q == 0u ? w.c0 : q == 1u ? w.c1 : q == 2u ? w.c2 : w.c3;
+}
+
 template <uint32_t POLICY, bool TRACE, uint32_t REFILL>
 __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nthreads = gridDim.x * blockDim.x;
-  bool active = false;
+  bool active = false, deferred = false;
   uint32_t g = 0, p = 0, j = 0, ply = 0, h = 0, maxn = 0;
-  uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+  Words w = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, false, false};
   uint64_t plies_acc = 0;
   ck::Lane L = {0u, 0u, 0u, 0u, 0u};
-  // wave-uniform refill state
-  uint32_t res_next = 0, res_end = 0;
+  // wave-uniform refill and deferral state
+  uint32_t res_next = 0, res_end = 0, since_walk = 0;
   bool exhausted = false;
   uint32_t static_next = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t defer_min = a.defer_min;
 
   for (uint32_t iter = 0;; iter++) {
     if ((iter & 3u) == 0u) {
+      bool start = false;
       if (REFILL == MI_REFILL_STATIC) {
         if (!active && static_next < a.total) {
           g = static_next;
           static_next += nthreads;
-          active = true;
-          locate(a, g, p, j);
-          mi_pos q = a.pos[p];
-          L = ck::lane_from_abs(q.p1, q.p2, q.kings, q.turn, q.counter);
-          ply = 0;
-          h = 0;
+          start = true;
         }
       } else if (!exhausted) {
         const uint64_t idle = __ballot(!active);
@@ -90,12 +110,7 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
             const uint32_t idx = rank < avail ? res_next + rank : chunk + (rank - avail);
             if (idx < a.total) {
               g = idx;
-              active = true;
-              locate(a, g, p, j);
-              mi_pos q = a.pos[p];
-              L = ck::lane_from_abs(q.p1, q.p2, q.kings, q.turn, q.counter);
-              ply = 0;
-              h = 0;
+              start = true;
             }
           }
           if (avail < cnt) {
@@ -107,24 +122,75 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
           if (res_next >= a.total) exhausted = true;
         }
       }
+      if (start) {
+        active = true;
+        deferred = false;
+        locate(a, g, p, j);
+        const mi_pos q = a.pos[p];
+        L = ck::lane_from_abs(q.p1, q.p2, q.kings, q.turn, q.counter);
+        ply = 0;
+        h = 0;
+        w.cur_ok = false;
+        w.nxt_ok = false;
+      }
       if (!__any(active)) break;
-      if (active) {
-        uint32_t w[4];
-        ck::philox(ply >> 2, j, ck::TAG_PLAYOUT, a.epoch, a.seed, p, w);
-        w0 = w[0];
-        w1 = w[1];
-        w2 = w[2];
-        w3 = w[3];
+      if (active && (!w.cur_ok || !w.nxt_ok)) {
+        const uint32_t b = w.cur_ok ? w.blk + 1u : (ply >> 2);
+        uint32_t o[4];
+        ck::philox(b, j, ck::TAG_PLAYOUT, a.epoch, a.seed, p, o);
+        if (w.cur_ok) {
+          w.n0 = o[0];
+          w.n1 = o[1];
+          w.n2 = o[2];
+          w.n3 = o[3];
+          w.nxt_ok = true;
+        } else {
+          w.c0 = o[0];
+          w.c1 = o[1];
+          w.c2 = o[2];
+          w.c3 = o[3];
+          w.blk = b;
+          w.cur_ok = true;
+        }
       }
     }
-    if (active) {
-      const uint32_t r = w0;
-      w0 = w1;
-      w1 = w2;
-      w2 = w3;
-      uint32_t ld = 0;
-      ck::PlyResult res = POLICY == MI_POLICY_RANDOM ? ck::ply_random<TRACE>(L, r, &ld)
-                                                     : ck::ply_heuristic<TRACE>(L, r, &ld);
+    // common part of the ply; multi-jump plies of the random policy wait for the walk phase
+    bool play = false;
+    ck::PlyResult res;
+    uint32_t ld = 0;
+    if (active && !deferred) {
+      const uint32_t r = take_word(w, ply);
+      if (POLICY == MI_POLICY_RANDOM) {
+        // a deferred ply takes the same word again in the walk phase (take_word is idempotent
+        // for a given ply)
+        bool d = false;
+        res = ck::ply_random_fast<TRACE>(L, r, &ld, d);
+        deferred = d;
+        play = !d;
+      } else {
+        res = ck::ply_heuristic<TRACE>(L, r, &ld);
+        play = true;
+      }
+    }
+    if (POLICY == MI_POLICY_RANDOM) {
+      const uint64_t dm = __ballot(deferred);
+      if (dm) {
+        const uint32_t nd = (uint32_t)__popcll(dm);
+        const uint32_t na = (uint32_t)__popcll(__ballot(active));
+        if (nd >= defer_min || nd == na || since_walk >= 8u) {
+          since_walk = 0;
+          if (deferred) {
+            const uint32_t r = take_word(w, ply);
+            res = ck::ply_random<TRACE>(L, r, &ld);
+            deferred = false;
+            play = true;
+          }
+        } else {
+          since_walk++;
+        }
+      }
+    }
+    if (play) {
       if (TRACE) h = ck::hash_step(h, ck::ply_record(ld, res.n, res.chosen));
       maxn = res.n > maxn ? res.n : maxn;
       if (res.over) {

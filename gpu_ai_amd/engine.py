@@ -35,7 +35,7 @@ REFILL_QUEUE = 1
 
 class MiTuning(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
-                ("refill", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("refill", ctypes.c_uint32), ("defer_min", ctypes.c_uint32)]
 
 
 class MiTuneResult(ctypes.Structure):
@@ -128,10 +128,10 @@ class Engine:
     """One device context (stream, events, buffers) of the playout engine."""
 
     def __init__(self, device: int = 0, block_size: int = 0, blocks_per_cu: int = 0,
-                 refill: int = REFILL_QUEUE):
+                 refill: int = REFILL_QUEUE, defer_min: int = 0):
         self.lib = library()
         self.ctx = ctypes.c_void_p()
-        t = MiTuning(block_size, blocks_per_cu, refill, 0)
+        t = MiTuning(block_size, blocks_per_cu, refill, defer_min)
         rc = self.lib.mi_create(ctypes.byref(self.ctx), device, ctypes.byref(t))
         if rc != MI_OK:
             raise MiError(rc, self.lib.mi_strerror(rc).decode())
@@ -163,10 +163,12 @@ class Engine:
     def tuning(self) -> dict:
         t = MiTuning()
         self._check(self.lib.mi_get_tuning(self.ctx, ctypes.byref(t)))
-        return {"block_size": t.block_size, "blocks_per_cu": t.blocks_per_cu, "refill": t.refill}
+        return {"block_size": t.block_size, "blocks_per_cu": t.blocks_per_cu, "refill": t.refill,
+                "defer_min": t.defer_min}
 
-    def set_tuning(self, block_size: int = 0, blocks_per_cu: int = 0, refill: int = REFILL_QUEUE):
-        t = MiTuning(block_size, blocks_per_cu, refill, 0)
+    def set_tuning(self, block_size: int = 0, blocks_per_cu: int = 0, refill: int = REFILL_QUEUE,
+                   defer_min: int = 0):
+        t = MiTuning(block_size, blocks_per_cu, refill, defer_min)
         self._check(self.lib.mi_set_tuning(self.ctx, ctypes.byref(t)))
 
     def device_info(self) -> dict:
@@ -257,16 +259,17 @@ class Engine:
         """Sweeps launch shapes on this batch; the fastest becomes the context's tuning."""
         pos = as_positions(pos)
         c, pp, _ = self._counts(len(pos), per_pos, counts)
-        res = (MiTuneResult * 64)()
+        res = (MiTuneResult * 128)()
         nres = _U32()
         best = MiTuning()
         self._check(self.lib.mi_autotune(self.ctx, _ptr(pos), len(pos), _ptr(c), pp, policy, reps,
-                                         res, 64, ctypes.byref(nres), ctypes.byref(best)))
+                                         res, 128, ctypes.byref(nres), ctypes.byref(best)))
         rows = [{"block_size": r.tuning.block_size, "blocks_per_cu": r.tuning.blocks_per_cu,
-                 "refill": r.tuning.refill, "kernel_ms": r.kernel_ms,
-                 "playouts_per_s": r.playouts_per_s} for r in res[:min(nres.value, 64)]]
+                 "refill": r.tuning.refill, "defer_min": r.tuning.defer_min,
+                 "kernel_ms": r.kernel_ms, "playouts_per_s": r.playouts_per_s}
+                for r in res[:min(nres.value, 128)]]
         return ({"block_size": best.block_size, "blocks_per_cu": best.blocks_per_cu,
-                 "refill": best.refill}, rows)
+                 "refill": best.refill, "defer_min": best.defer_min}, rows)
 
     def copy_calibration(self, nbytes: int, repeats: int = 5) -> float:
         ms = ctypes.c_float()

@@ -68,7 +68,8 @@ typedef struct mi_tuning {
   uint32_t block_size;      /* threads per workgroup: 64..1024, a multiple of 64 */
   uint32_t blocks_per_cu;   /* resident workgroups per compute unit for the persistent grid */
   uint32_t refill;          /* MI_REFILL_* */
-  uint32_t reserved;
+  uint32_t defer_min;       /* random policy: a wave plays its parked multi-jump plies once this
+                               many lanes wait (or all active lanes wait, or 8 iterations pass) */
 } mi_tuning;
 
 /* Per-call statistics. */
@@ -149,8 +150,8 @@ int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin
 /* Autotuner (no counterpart in the reference, whose OptimalPlayoutDriver picks between whole
  * drivers by a runtime model, src/playout.cpp:80-166).  Runs the given batch under every
  * candidate launch shape -- workgroup sizes 64..1024, 1..8 resident waves per SIMD (which sets the
- * playouts each lane runs) and both refill policies -- `reps` times each, and makes the one with
- * the smallest median kernel time the context's tuning.  `results` (may be NULL) receives up to
+ * playouts each lane runs), both refill policies and deferral thresholds 1, 8 and 16 -- `reps`
+ * times each, and makes the one with the smallest median kernel time the context's tuning.  `results` (may be NULL) receives up to
  * `max_results` candidates; `nresults` the number measured; `best` (may be NULL) the winner. */
 int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
                 uint32_t per_pos, uint32_t policy, uint32_t reps, mi_tune_result *results,

@@ -119,21 +119,25 @@ def test_config0_bitexact(engine, oracle, golden):
     assert np.array_equal(res.winners, w)
 
 
-@pytest.mark.parametrize("block,bpc,refill", [(64, 1, REFILL_STATIC), (64, 16, REFILL_QUEUE),
-                                              (256, 8, REFILL_STATIC), (1024, 2, REFILL_QUEUE),
-                                              (512, 4, REFILL_QUEUE)])
-def test_launch_shape_does_not_change_results(engine, golden, block, bpc, refill):
+@pytest.mark.parametrize("block,bpc,refill,defer", [(64, 1, REFILL_STATIC, 1), (64, 16, REFILL_QUEUE, 16),
+                                                    (256, 8, REFILL_STATIC, 8), (1024, 2, REFILL_QUEUE, 1),
+                                                    (512, 4, REFILL_QUEUE, 64)])
+def test_launch_shape_does_not_change_results(engine, golden, block, bpc, refill, defer):
     pos = golden["positions"]["c2"][:256]
-    announce(f"tuning block {block} blocks/CU {bpc} refill {refill}")
+    announce(f"tuning block {block} blocks/CU {bpc} refill {refill} defer {defer}")
     base = engine.run_traced(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
-    engine.set_tuning(block, bpc, refill)
+    fast = engine.run_playouts(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
+    engine.set_tuning(block, bpc, refill, defer)
     try:
         other = engine.run_traced(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
+        other_fast = engine.run_playouts(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
         assert other.stats["block_size"] == block and other.stats["refill"] == refill
     finally:
-        engine.set_tuning(0, 0, REFILL_QUEUE)
+        engine.set_tuning()
     assert np.array_equal(base.winners, other.winners)
     assert np.array_equal(base.hashes, other.hashes)
+    assert np.array_equal(fast.winners, base.winners)
+    assert np.array_equal(other_fast.winners, base.winners)
 
 
 def test_uneven_counts_empty_and_single(engine, oracle, golden):
