@@ -34,6 +34,10 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I" + os.path
 
 # (source, object) pairs of the device library
 LIB_SOURCES = ["playout_kernels.hip", "capi.cpp"]
+# the C++ host layer (caller side: State, PlayoutDriver, GameTree, players, CLI)
+HOST_SOURCES = ["state.cpp", "playout.cpp", "mcts.cpp", "player.cpp", "capi_host.cpp"]
+HOST_LIB_PATH = os.path.join(PKG, "libgpuai_host.so")
+RUN_AI_PATH = os.path.join(BIN, "run_ai")
 
 
 def _run(cmd: list[str]) -> None:
@@ -63,6 +67,25 @@ def build_library(jobs: int = 8) -> str:
     return LIB_PATH
 
 
+def build_host(jobs: int = 8) -> None:
+    """Builds libgpuai_host.so and bin/run_ai over the device library."""
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    extra = ["-fopenmp", "-I" + HOST]
+    srcs = HOST_SOURCES + ["run_ai.cpp"]
+    pairs = [(os.path.join(HOST, s), os.path.join(OBJ, "host_" + s.replace(".", "_") + ".o"))
+             for s in srcs]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda p: _compile(p[0], p[1], extra), pairs))
+    link = ["-fopenmp", "-L" + PKG, "-lgpuai_mi355x", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."]
+    tmp = HOST_LIB_PATH + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs[:-1], "-o", tmp] + link)
+    os.replace(tmp, HOST_LIB_PATH)
+    tmp = RUN_AI_PATH + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", objs[-1], "-o", tmp, "-L" + PKG, "-lgpuai_host"] + link)
+    os.replace(tmp, RUN_AI_PATH)
+
+
 def build_oracle() -> None:
     """Builds oracle/liboracle.so and, where the reference sources exist, oracle/_ref."""
     oracle = os.path.join(ROOT, "oracle")
@@ -73,14 +96,16 @@ def build_oracle() -> None:
 
 def build_all(jobs: int = 8) -> None:
     build_library(jobs)
+    build_host(jobs)
     build_oracle()
 
 
 def clean() -> None:
     for p in (OBJ, BIN):
         shutil.rmtree(p, ignore_errors=True)
-    if os.path.exists(LIB_PATH):
-        os.remove(LIB_PATH)
+    for p in (LIB_PATH, HOST_LIB_PATH):
+        if os.path.exists(p):
+            os.remove(p)
 
 
 def main() -> int:

@@ -1,0 +1,107 @@
+// capi_host.cpp -- C entry points of the host layer (include/gpuai/gpuai_host.h).
+#include <cstring>
+#include <exception>
+
+#include "gpuai/gpuai_host.h"
+#include "mcts.hpp"
+#include "state.hpp"
+
+namespace {
+
+mi_move toMi(const Move &m) {
+  mi_move o;
+  std::memset(&o, 0, sizeof(o));
+  o.from = m.from.square();
+  o.to = m.to.square();
+  o.jumps = m.jumps;
+  o.promoted = m.promoted ? 1 : 0;
+  for (unsigned i = 0; i < m.jumps; i++) {
+    o.inter[i] = m.intermediate[i].square();
+    o.removed[i] = m.removed[i].square();
+  }
+  return o;
+}
+
+uint32_t fmix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  x *= 0xC2B2AE35u;
+  x ^= x >> 16;
+  return x;
+}
+
+uint32_t rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+uint32_t stateKey(const mi_pos &p) {
+  return p.p1 ^ rotl(p.p2, 7) ^ rotl(p.kings, 13) ^ ((uint32_t)p.turn << 30) ^
+         ((uint32_t)p.counter << 20);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpuai_host_gen_moves(const mi_pos *pos, mi_move *out, uint32_t cap) {
+  Move result[MAX_MOVES];
+  const State s = State::unpack(*pos);
+  const int n = s.genMoves(result);
+  if ((uint32_t)n > cap) return -1;
+  for (int i = 0; i < n; i++) out[i] = toMi(result[i]);
+  return n;
+}
+
+int gpuai_host_apply(mi_pos *pos, const mi_move *move) {
+  State s = State::unpack(*pos);
+  Move m(Loc::ofSquare(move->from), Loc::ofSquare(move->to), move->jumps, move->promoted != 0);
+  for (unsigned i = 0; i < move->jumps && i < MAX_MOVE_JUMPS; i++) {
+    m.intermediate[i] = Loc::ofSquare(move->inter[i]);
+    m.removed[i] = Loc::ofSquare(move->removed[i]);
+  }
+  for (const Move &legal : s.getMoves()) {
+    if (legal == m) {
+      s.move(m);
+      *pos = s.pack();
+      return 0;
+    }
+  }
+  return -1;
+}
+
+int gpuai_mcts_trace(const mi_pos *root, uint32_t iterations, uint32_t initial, double scale,
+                     uint32_t *trace, uint32_t trace_cap) {
+  if (trace_cap < 5u * iterations + 4u) return -1;
+  try {
+    GameTree tree(State::unpack(*root));
+    uint32_t w = 0;
+    for (uint32_t it = 0; it < iterations; it++) {
+      unsigned n = (unsigned)(tree.getTotalTrials() * scale);
+      if (n == 0) n = initial;
+      std::vector<State> leaves = tree.select(n);
+      std::vector<PlayerId> results(leaves.size());
+      uint32_t digest = 0;
+      for (size_t i = 0; i < leaves.size(); i++) {
+        const uint32_t key = stateKey(leaves[i].pack());
+        digest = fmix32(digest ^ key) + 0x9E3779B9u;
+        const uint32_t h = fmix32(key ^ (it * 0x9E3779B9u) ^ ((uint32_t)i * 0x85EBCA6Bu));
+        results[i] = (PlayerId)((int)(h % 3u) - 1);
+      }
+      tree.update(results);
+      trace[w++] = (uint32_t)leaves.size();
+      trace[w++] = digest;
+      trace[w++] = tree.getTotalTrials();
+      trace[w++] = (uint32_t)tree.getWins(PLAYER_1);
+      trace[w++] = (uint32_t)tree.getWins(PLAYER_2);
+    }
+    for (PlayerId p : {PLAYER_1, PLAYER_2}) {
+      Move m = tree.getOptMove(p);
+      trace[w++] = m.from.square();
+      trace[w++] = m.to.square();
+    }
+    return (int)w;
+  } catch (const std::exception &) {
+    return -2;
+  }
+}
+
+}  // extern "C"

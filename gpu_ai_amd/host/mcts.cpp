@@ -1,0 +1,147 @@
+// mcts.cpp -- batch UCB1 tree (behaviour of src/mcts.cpp).
+#include "mcts.hpp"
+
+#include <cassert>
+#include <cmath>
+#include <stdexcept>
+
+GameTree::GameTree(State s, GameTree *parent) : state(s), parent(parent) { moves = s.getMoves(); }
+
+GameTree::~GameTree() {
+  for (auto &c : children) c->parent = nullptr;
+}
+
+std::shared_ptr<GameTree> GameTree::move(const Move &m) {
+  if (expanded) {
+    std::shared_ptr<GameTree> found;
+    for (size_t i = 0; i < children.size(); i++)
+      if (moves[i] == m) found = children[i];
+    if (!found) throw std::runtime_error("move is not legal in this position");
+    return found;
+  }
+  State next(state);
+  next.move(m);
+  return std::make_shared<GameTree>(next);
+}
+
+double GameTree::getScore(PlayerId player) const {
+  if (state.isGameOver()) return state.getNextTurn() == player ? 1.0 : 0.0;
+  return wins[player] / totalTrials;
+}
+
+Move GameTree::getOptMove(PlayerId player) const {
+  if (!expanded || children.empty()) throw std::runtime_error("the tree has no expanded moves");
+  double best = -INFINITY;
+  Move bestMove;
+  for (size_t i = 0; i < children.size(); i++) {
+    const double s = children[i]->getScore(player);
+    if (s > best) {
+      best = s;
+      bestMove = moves[i];
+    }
+  }
+  return bestMove;
+}
+
+void GameTree::expand(unsigned numPlayouts, PlayoutDriver *driver) {
+  std::vector<State> leaves = select(numPlayouts);
+  update(driver->runPlayouts(leaves));
+}
+
+std::vector<State> GameTree::select(unsigned trials) {
+  if (state.isGameOver()) {  // every trial of a finished game is played from it
+    assignedTrials = trials;
+    return std::vector<State>(trials, state);
+  }
+  if (!expanded) {
+    if (trials > 1) {
+      for (const Move &m : moves) {
+        State next = state;
+        next.move(m);
+        children.push_back(std::make_shared<GameTree>(next, this));
+      }
+      expanded = true;
+    } else {
+      assignedTrials = trials;
+      return trials == 1 ? std::vector<State>{state} : std::vector<State>{};
+    }
+  }
+  const size_t n = children.size();
+  std::vector<unsigned> share(n, 0);
+  std::vector<double> weight(n);
+  double weightSum = 0;
+  unsigned untried = 0;
+  for (size_t i = 0; i < n; i++) {
+    weight[i] = children[i]->ucb1();
+    weightSum += weight[i];
+    if (children[i]->totalTrials == 0) untried++;
+  }
+  assignedTrials = 0;
+  for (size_t i = 0; i < n; i++) {
+    if (untried > 0)  // untried children share the batch evenly
+      share[i] = children[i]->totalTrials == 0 ? trials / untried : 0;
+    else if (trials > 0)  // proportional to UCB1, rounded down
+      share[i] = (unsigned)(trials * (weight[i] / weightSum));
+    assignedTrials += share[i];
+  }
+  // the remainder goes one trial each to the children of largest weight, first index first
+  std::vector<bool> topped(n, false);
+  while (assignedTrials < trials) {
+    double maxWeight = -INFINITY;
+    int pick = -1;
+    for (size_t i = 0; i < n; i++) {
+      if (!topped[i] && weight[i] > maxWeight) {
+        maxWeight = weight[i];
+        pick = (int)i;
+      }
+    }
+    if (pick < 0) throw std::runtime_error("trial allocation failed");
+    topped[pick] = true;
+    share[pick]++;
+    assignedTrials++;
+  }
+  std::vector<std::vector<State>> parts(n);
+#pragma omp parallel for
+  for (long i = 0; i < (long)n; i++) parts[i] = children[i]->select(share[i]);
+  std::vector<State> out;
+  out.reserve(trials);
+  for (auto &p : parts) out.insert(out.end(), p.begin(), p.end());
+  return out;
+}
+
+template <class R, class Add>
+void GameTree::updateImpl(const std::vector<R> &results, Add add) {
+  if (results.size() != assignedTrials)
+    throw std::runtime_error("update: result count differs from the trials handed out");
+  totalTrials += assignedTrials;
+  if (expanded) {
+    size_t at = 0;
+    for (auto &c : children) {
+      std::vector<R> part(results.begin() + at, results.begin() + at + c->assignedTrials);
+      at += c->assignedTrials;
+      c->updateImpl(part, add);
+    }
+  }
+  for (const R &r : results) add(wins, r);
+}
+
+void GameTree::update(const std::vector<PlayerId> &results) {
+  updateImpl(results, [](double *w, PlayerId r) {
+    if (r != PLAYER_NONE) w[r] += 1.0;
+  });
+}
+
+void GameTree::updateValues(const std::vector<float> &p1WinProb) {
+  updateImpl(p1WinProb, [](double *w, float p) {
+    w[PLAYER_1] += p;
+    w[PLAYER_2] += 1.0 - p;
+  });
+}
+
+// The reference's arithmetic, including its long-double exploration term: the win rate of the
+// side to move in this child (src/mcts.cpp:189) plus sqrt(2 ln N / n).
+double GameTree::ucb1() const {
+  if (totalTrials == 0) return INFINITY;
+  return (double)wins[state.turn] / totalTrials +
+         std::sqrt(2.0L * std::log(parent->totalTrials) / totalTrials);
+}

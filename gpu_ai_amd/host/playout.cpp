@@ -1,0 +1,65 @@
+// playout.cpp -- MI355X playout drivers behind PlayoutDriver::runPlayouts.
+#include "playout.hpp"
+
+#include <stdexcept>
+
+void checkMi(int rc, const mi_ctx *ctx, const char *what) {
+  if (rc == MI_OK) return;
+  std::string msg = std::string(what) + ": " + mi_strerror(rc);
+  const char *detail = ctx ? mi_last_error(ctx) : "";
+  if (detail && *detail) msg += std::string(" (") + detail + ")";
+  throw std::runtime_error(msg);
+}
+
+DeviceMI355XPlayoutDriver::DeviceMI355XPlayoutDriver(uint32_t policy, int device, uint32_t seed,
+                                                     const mi_tuning *tuning)
+    : policy_(policy), seed_(seed) {
+  if (policy != MI_POLICY_RANDOM && policy != MI_POLICY_HEURISTIC)
+    throw std::runtime_error("unknown playout policy");
+  checkMi(mi_create(&ctx_, device, tuning), nullptr, "mi_create");
+}
+
+DeviceMI355XPlayoutDriver::~DeviceMI355XPlayoutDriver() { mi_destroy(ctx_); }
+
+void DeviceMI355XPlayoutDriver::runGrouped(const std::vector<mi_pos> &pos,
+                                           const std::vector<uint32_t> &counts,
+                                           std::vector<int8_t> &winners, mi_stats *stats) {
+  uint64_t total = 0;
+  for (uint32_t c : counts) total += c;
+  winners.resize(total);
+  checkMi(mi_run_playouts(ctx_, pos.data(), (uint32_t)pos.size(), counts.data(), 0, seed_,
+                          nextEpoch(), policy_, winners.data(), stats),
+          ctx_, "mi_run_playouts");
+}
+
+std::vector<PlayerId> DeviceMI355XPlayoutDriver::runPlayouts(std::vector<State> states) {
+  std::vector<PlayerId> results(states.size());
+  if (states.empty()) return results;  // src/singlePlayout.cu:73-75
+  std::vector<mi_pos> pos;
+  std::vector<uint32_t> counts;
+  for (size_t i = 0; i < states.size(); i++) {
+    if (i > 0 && states[i] == states[i - 1]) {
+      counts.back()++;
+    } else {
+      pos.push_back(states[i].pack());
+      counts.push_back(1);
+    }
+  }
+  std::vector<int8_t> winners;
+  runGrouped(pos, counts, winners);
+  for (size_t i = 0; i < states.size(); i++) results[i] = (PlayerId)winners[i];
+  return results;
+}
+
+std::unique_ptr<PlayoutDriver> getPlayoutDriver(std::string name) {
+  static const char *random_names[] = {"host", "device_single", "device_multiple", "device_coarse",
+                                       "hybrid", "optimal", "device_mi355x"};
+  static const char *heuristic_names[] = {"host_heuristic", "device_heuristic", "hybrid_heuristic",
+                                          "hybrid_heruistic", "optimal_heuristic",
+                                          "device_mi355x_heuristic"};
+  for (const char *n : random_names)
+    if (name == n) return std::make_unique<DeviceMI355XPlayoutDriver>(MI_POLICY_RANDOM);
+  for (const char *n : heuristic_names)
+    if (name == n) return std::make_unique<DeviceMI355XPlayoutDriver>(MI_POLICY_HEURISTIC);
+  throw std::runtime_error("Unknown playout type");
+}

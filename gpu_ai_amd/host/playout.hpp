@@ -1,0 +1,63 @@
+// playout.hpp -- the playout-driver interface of gpu_ai (src/playout.hpp:27-33) and the MI355X
+// drivers that implement it through the C ABI of include/gpuai/mi_playout.h.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gpuai/mi_playout.h"
+#include "state.hpp"
+
+class PlayoutDriver {
+ public:
+  virtual ~PlayoutDriver() {}
+  // One playout from each state; result i is the winner of the playout from states[i].
+  virtual std::vector<PlayerId> runPlayouts(std::vector<State>) = 0;
+  virtual std::string getName() const = 0;
+};
+
+// Random (src/playout.cpp:17-32) or heuristic (src/heuristicPlayout.cpp:12-48) playouts on an
+// MI355X.  Runs of equal consecutive states become one position with a playout count (terminal
+// leaves repeat their state, src/mcts.cpp:65-68).  Each call draws from its own Philox epoch, so
+// repeated calls on the same states are independent samples; the sequence is reproducible from
+// the seed.  Instances are independent (one HIP context each) and may be used from different
+// threads; errors throw std::runtime_error.
+class DeviceMI355XPlayoutDriver : public PlayoutDriver {
+ public:
+  explicit DeviceMI355XPlayoutDriver(uint32_t policy = MI_POLICY_RANDOM, int device = 0,
+                                     uint32_t seed = 12345, const mi_tuning *tuning = nullptr);
+  ~DeviceMI355XPlayoutDriver() override;
+  DeviceMI355XPlayoutDriver(const DeviceMI355XPlayoutDriver &) = delete;
+  DeviceMI355XPlayoutDriver &operator=(const DeviceMI355XPlayoutDriver &) = delete;
+
+  std::vector<PlayerId> runPlayouts(std::vector<State> states) override;
+  std::string getName() const override {
+    return policy_ == MI_POLICY_RANDOM ? "device_mi355x" : "device_mi355x_heuristic";
+  }
+
+  // Grouped form used by the batched searches: positions with playout counts, winners in
+  // position-major order.
+  void runGrouped(const std::vector<mi_pos> &pos, const std::vector<uint32_t> &counts,
+                  std::vector<int8_t> &winners, mi_stats *stats = nullptr);
+  mi_ctx *context() const { return ctx_; }
+  uint32_t nextEpoch() { return epoch_++; }
+  uint32_t seed() const { return seed_; }
+  uint32_t policy() const { return policy_; }
+
+ private:
+  mi_ctx *ctx_ = nullptr;
+  uint32_t policy_;
+  uint32_t seed_;
+  uint32_t epoch_ = 0;
+};
+
+// Name -> driver.  The reference's driver names (src/playout.cpp:358-392) all select the MI355X
+// engine: host, device_single, device_multiple, device_coarse, hybrid and optimal run random
+// playouts; host_heuristic, device_heuristic, hybrid_heuristic and optimal_heuristic run
+// heuristic playouts.  device_mi355x and device_mi355x_heuristic name them directly.
+// Unknown names throw std::runtime_error("Unknown playout type").
+std::unique_ptr<PlayoutDriver> getPlayoutDriver(std::string name);
+
+// Throws std::runtime_error with the library's message when rc != MI_OK.
+void checkMi(int rc, const mi_ctx *ctx, const char *what);

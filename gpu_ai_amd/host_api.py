@@ -1,0 +1,65 @@
+"""ctypes bindings of the host layer's C entry points (include/gpuai/gpuai_host.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .build import HOST_LIB_PATH, RUN_AI_PATH
+from .engine import MOVE_DTYPE, POS_DTYPE, library
+
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_lib = None
+
+
+def host_library() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise RuntimeError(f"{HOST_LIB_PATH} is missing: build it with `python -m gpu_ai_amd.build`")
+        library()  # the device library it links against
+        lib = ctypes.CDLL(HOST_LIB_PATH)
+        lib.gpuai_host_gen_moves.argtypes = [_P, _P, _U32]
+        lib.gpuai_host_gen_moves.restype = ctypes.c_int
+        lib.gpuai_host_apply.argtypes = [_P, _P]
+        lib.gpuai_host_apply.restype = ctypes.c_int
+        lib.gpuai_mcts_trace.argtypes = [_P, _U32, _U32, ctypes.c_double, _P, _U32]
+        lib.gpuai_mcts_trace.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def gen_moves(pos) -> np.ndarray:
+    p = np.ascontiguousarray(pos, dtype=POS_DTYPE).reshape(1)
+    out = np.zeros(128, dtype=MOVE_DTYPE)
+    n = host_library().gpuai_host_gen_moves(_ptr(p), _ptr(out), 128)
+    if n < 0:
+        raise RuntimeError("move list longer than 128")
+    return out[:n].copy()
+
+
+def apply(pos, move) -> np.ndarray:
+    p = np.array(pos, dtype=POS_DTYPE).reshape(1).copy()
+    m = np.array(move, dtype=MOVE_DTYPE).reshape(1)
+    if host_library().gpuai_host_apply(_ptr(p), _ptr(m)) != 0:
+        raise ValueError("illegal move")
+    return p[0]
+
+
+def mcts_trace(root, iterations: int, initial: int, scale: float) -> np.ndarray:
+    p = np.ascontiguousarray(root, dtype=POS_DTYPE).reshape(1)
+    out = np.zeros(5 * iterations + 4, dtype=np.uint32)
+    n = host_library().gpuai_mcts_trace(_ptr(p), iterations, initial, scale, _ptr(out), len(out))
+    if n < 0:
+        raise RuntimeError(f"gpuai_mcts_trace failed ({n})")
+    return out[:n]
+
+
+def run_ai_path() -> str:
+    return RUN_AI_PATH

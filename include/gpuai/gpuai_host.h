@@ -1,0 +1,42 @@
+/*
+ * gpuai_host.h -- C entry points of the C++ host layer (libgpuai_host.so).
+ *
+ * The host layer is the drop-in caller side: State/Move (src/state.hpp), PlayoutDriver
+ * (src/playout.hpp:27-33), GameTree (src/mcts.hpp:12-64) and the players (src/player.hpp).
+ * These functions expose parts of it to bindings and tests; positions and moves use the
+ * layouts of mi_playout.h.
+ */
+#ifndef GPUAI_HOST_H
+#define GPUAI_HOST_H
+
+#include <stdint.h>
+
+#include "gpuai/mi_playout.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* State::genMoves on the host (tree expansion uses it); returns the count, -1 if > cap.
+ * Replaces State::genMoves (src/state.cu:239-245) for the callers. */
+int gpuai_host_gen_moves(const mi_pos *pos, mi_move *out, uint32_t cap);
+
+/* State::move on the host (src/state.cu:57-92); returns 0, or -1 for an illegal move. */
+int gpuai_host_apply(mi_pos *pos, const mi_move *move);
+
+/* Grows a GameTree from `root` for `iterations` rounds of select / update, the trial count per
+ * round following MCTSPlayer::worker (src/player.cpp:137-139: totalTrials * scale, or `initial`
+ * when that is 0).  Each trial's winner is a fixed function of (round, trial index, position)
+ * instead of a playout: fmix32(p1 ^ rot(p2,7) ^ rot(kings,13) ^ turn<<30 ^ counter<<20 ^
+ * round*0x9E3779B9 ^ trial*0x85EBCA6B) mod 3, minus 1.  Writes 5 words per round (trials
+ * handed out, a digest of the selected positions in order, root totalTrials, root wins of player 1
+ * and of player 2) followed by the from/to squares of getOptMove for player 1 and player 2.
+ * Returns the number of words written, or -1 if trace_cap is too small.  Test hook: the same
+ * function over the reference's GameTree lives in the oracle. */
+int gpuai_mcts_trace(const mi_pos *root, uint32_t iterations, uint32_t initial, double scale,
+                     uint32_t *trace, uint32_t trace_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -187,6 +187,18 @@ class Reference(_Lib):
         L.ref_score_move_index.argtypes = [_P, ctypes.c_int, _P, _P]
         L.ref_score_move_index.restype = ctypes.c_int
 
+    def mcts_trace(self, root, iterations: int, initial: int, scale: float) -> np.ndarray:
+        """ref_mcts_trace: the reference's GameTree driven by a fixed winner function."""
+        fn = self.lib.ref_mcts_trace
+        fn.argtypes = [_P, _U32, _U32, ctypes.c_double, _P, _U32]
+        fn.restype = ctypes.c_int
+        p = np.ascontiguousarray(root, dtype=POS_DTYPE).reshape(1)
+        out = np.zeros(5 * iterations + 4, dtype=np.uint32)
+        n = fn(_ptr(p), iterations, initial, scale, _ptr(out), len(out))
+        if n < 0:
+            raise RuntimeError("ref_mcts_trace failed")
+        return out[:n]
+
     def apply_index(self, pos, index: int) -> np.ndarray:
         p = np.array(pos, dtype=POS_DTYPE).reshape(1).copy()
         if self.lib.ref_apply_index(_ptr(p), index) != 0:

@@ -15,6 +15,7 @@
 #include "oracle.h"
 
 #include <array>
+#include <cmath>
 #include <cstring>
 #include <vector>
 #include <omp.h>
@@ -273,6 +274,45 @@ int ref_gen_start_positions(uint32_t seed, uint32_t n, uint32_t kmin, uint32_t k
     out[posIndex] = fromState(state);
   }
   return failed ? -1 : 0;
+}
+
+// The trace of gpuai_mcts_trace (include/gpuai/gpuai_host.h) over the reference's own GameTree
+// (src/mcts.cpp): identical trial counts and winner function, so the two traces must agree.
+int ref_mcts_trace(const or_pos *root, uint32_t iterations, uint32_t initial, double scale,
+                   uint32_t *trace, uint32_t traceCap) {
+  if (traceCap < 5u * iterations + 4u) return -1;
+  auto rotl = [](uint32_t x, int r) { return (x << r) | (x >> (32 - r)); };
+  GameTree tree(toState(*root));
+  uint32_t w = 0;
+  for (uint32_t it = 0; it < iterations; it++) {
+    unsigned n = (unsigned)(tree.getTotalTrials() * scale);
+    if (n == 0) n = initial;
+    std::vector<State> leaves = tree.select(n);
+    std::vector<PlayerId> results(leaves.size());
+    uint32_t digest = 0;
+    for (size_t i = 0; i < leaves.size(); i++) {
+      const or_pos p = fromState(leaves[i]);
+      const uint32_t key = p.p1 ^ rotl(p.p2, 7) ^ rotl(p.kings, 13) ^ ((uint32_t)p.turn << 30) ^
+                           ((uint32_t)p.counter << 20);
+      digest = or_fmix32(digest ^ key) + 0x9E3779B9u;
+      const uint32_t h = or_fmix32(key ^ (it * 0x9E3779B9u) ^ ((uint32_t)i * 0x85EBCA6Bu));
+      results[i] = (PlayerId)((int)(h % 3u) - 1);
+    }
+    tree.update(results);
+    trace[w++] = (uint32_t)leaves.size();
+    trace[w++] = digest;
+    const unsigned total = tree.getTotalTrials();
+    trace[w++] = total;
+    // wins are private in the reference; getScore(p) = wins[p] / totalTrials recovers them
+    trace[w++] = (uint32_t)std::llround(tree.getScore(PLAYER_1) * total);
+    trace[w++] = (uint32_t)std::llround(tree.getScore(PLAYER_2) * total);
+  }
+  for (PlayerId p : {PLAYER_1, PLAYER_2}) {
+    Move m = tree.getOptMove(p);
+    trace[w++] = (uint32_t)sqOf(m.from);
+    trace[w++] = (uint32_t)sqOf(m.to);
+  }
+  return (int)w;
 }
 
 }  // extern "C"

@@ -1,0 +1,68 @@
+"""The C++ host layer (caller side of the drop-in): State/Move on bitboards, GameTree."""
+from __future__ import annotations
+
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+
+
+def test_host_gen_moves_golden(golden):
+    from gpu_ai_amd import host_api
+    ml = golden["movelists"]
+    pos, counts, offs, moves = ml["positions"], ml["counts"], ml["offsets"], ml["moves"]
+    for i in range(0, len(pos), 3):
+        got = host_api.gen_moves(pos[i])
+        ref = moves[int(offs[i]):int(offs[i]) + int(counts[i])]
+        assert len(got) == counts[i] and got.tobytes() == ref.tobytes(), i
+
+
+def test_host_apply_matches_oracle(oracle, golden):
+    from gpu_ai_amd import host_api
+    rng = np.random.default_rng(5)
+    pos = golden["positions"]["c2"][:200]
+    for p in pos:
+        for _ in range(30):
+            ms = oracle.gen_moves(p)
+            if len(ms) == 0 or p["counter"] >= 50:
+                break
+            m = ms[rng.integers(len(ms))]
+            a = host_api.apply(p, m)
+            b = oracle.apply(p, m)
+            assert a.tobytes() == b.tobytes()
+            p = b
+
+
+def test_host_apply_rejects_illegal(golden):
+    from gpu_ai_amd import host_api
+    p = golden["positions"]["c1"][0]
+    m = np.zeros(1, dtype=pyoracle.MOVE_DTYPE)[0]
+    m["from"], m["to"] = 31, 0
+    with pytest.raises(ValueError):
+        host_api.apply(p, m)
+
+
+@pytest.mark.skipif(not pyoracle.reference_available(), reason="oracle/_ref not built here")
+@pytest.mark.parametrize("which,iters,initial,scale", [("start", 40, 50, 0.02), ("c2_0", 30, 200, 0.25),
+                                                      ("c2_5", 12, 64, 1.0), ("c1_9", 60, 7, 0.1)])
+def test_mcts_matches_reference_gametree(golden, which, iters, initial, scale):
+    """GameTree select/update/UCB1/getOptMove against the reference's own src/mcts.cpp."""
+    from gpu_ai_amd import host_api
+    if which == "start":
+        root = pyoracle.start_position()
+    else:
+        name, idx = which.split("_")
+        root = golden["positions"][name][int(idx)]
+    ref = pyoracle.Reference().mcts_trace(root, iters, initial, scale)
+    mine = host_api.mcts_trace(root, iters, initial, scale)
+    assert len(mine) == len(ref)
+    assert np.array_equal(mine, ref)
+    assert mine[5 * (iters - 1) + 2] > initial  # the tree grew
+
+
+def test_run_ai_usage():
+    from gpu_ai_amd import host_api
+    out = subprocess.run([host_api.run_ai_path(), "-h"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "gen_moves_test" in out.stdout
