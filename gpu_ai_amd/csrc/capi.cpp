@@ -156,6 +156,7 @@ int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
   a.hashes = trace ? (uint32_t *)ctx->hashes.ptr : nullptr;
   a.work = (gpuai::WorkWords *)ctx->work.ptr;
   a.defer_min = ctx->tuning.defer_min;
+  a.desc = nullptr;
   const uint32_t grid = grid_for(ctx, total);
   MI_HIP(ctx, gpuai::launch_reset(a.work, ctx->stream));
   MI_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
@@ -586,6 +587,178 @@ int mi_copy_calibration(mi_ctx *ctx, uint64_t bytes, uint32_t repeats, float *av
   (void)hipFree(buf);
   *avg_ms = total / (float)repeats;
   return rc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// graph-replayed batches
+// ---------------------------------------------------------------------------------------------
+struct mi_graph {
+  mi_ctx *ctx = nullptr;
+  uint32_t max_pos = 0, max_play = 0, policy = 0;
+  // pinned host blob and its device twin: LaunchDesc | offsets (max_pos + 1) | positions
+  void *h_blob = nullptr, *d_blob = nullptr;
+  size_t offs_off = 0, pos_off = 0, blob_bytes = 0;
+  // pinned host output and its device twin: WorkWords | winners (max_play)
+  void *h_out = nullptr, *d_out = nullptr;
+  size_t out_bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraphNode_t n_h2d = nullptr, n_reset = nullptr, n_play = nullptr, n_d2h = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+void graph_release(mi_graph *g) {
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->e0) (void)hipEventDestroy(g->e0);
+  if (g->e1) (void)hipEventDestroy(g->e1);
+  if (g->d_blob) (void)hipFree(g->d_blob);
+  if (g->d_out) (void)hipFree(g->d_out);
+  if (g->h_blob) (void)hipHostFree(g->h_blob);
+  if (g->h_out) (void)hipHostFree(g->h_out);
+  delete g;
+}
+
+}  // namespace
+
+int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, uint32_t policy,
+                    mi_graph **out) {
+  if (!ctx || !out) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  *out = nullptr;
+  int rc;
+  if ((rc = bad_policy(ctx, policy))) return rc;
+  if (max_positions == 0 || max_playouts == 0 || max_playouts >= (1u << 31))
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "graph capacities must be in [1, 2^31)");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  mi_graph *g = new mi_graph();
+  g->ctx = ctx;
+  g->max_pos = max_positions;
+  g->max_play = max_playouts;
+  g->policy = policy;
+  g->offs_off = sizeof(gpuai::LaunchDesc);
+  g->pos_off = (g->offs_off + sizeof(uint32_t) * ((size_t)max_positions + 1) + 15) & ~(size_t)15;
+  g->blob_bytes = g->pos_off + sizeof(mi_pos) * (size_t)max_positions;
+  g->out_bytes = sizeof(gpuai::WorkWords) + (size_t)max_playouts;
+  hipError_t e = hipHostMalloc(&g->h_blob, g->blob_bytes, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&g->h_out, g->out_bytes, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&g->d_blob, g->blob_bytes);
+  if (e == hipSuccess) e = hipMalloc(&g->d_out, g->out_bytes);
+  if (e == hipSuccess) e = hipEventCreate(&g->e0);
+  if (e == hipSuccess) e = hipEventCreate(&g->e1);
+  if (e != hipSuccess) {
+    rc = hip_fail(ctx, e, "graph buffers");
+    graph_release(g);
+    return rc;
+  }
+  std::memset(g->h_blob, 0, g->blob_bytes);
+  gpuai::WorkWords *work = (gpuai::WorkWords *)g->d_out;
+  gpuai::PlayoutArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.pos = (const mi_pos *)((char *)g->d_blob + g->pos_off);
+  a.offs = (const uint32_t *)((char *)g->d_blob + g->offs_off);
+  a.winners = (int8_t *)((char *)g->d_out + sizeof(gpuai::WorkWords));
+  a.work = work;
+  a.defer_min = ctx->tuning.defer_min;
+  a.desc = (const gpuai::LaunchDesc *)g->d_blob;
+  const uint32_t grid = grid_for(ctx, max_playouts);
+  void *reset_args[] = {&work};
+  void *play_args[] = {&a};
+  hipKernelNodeParams rp;
+  std::memset(&rp, 0, sizeof(rp));
+  rp.func = const_cast<void *>(gpuai::reset_kernel());
+  rp.gridDim = dim3(1);
+  rp.blockDim = dim3(64);
+  rp.kernelParams = reset_args;
+  hipKernelNodeParams pp;
+  std::memset(&pp, 0, sizeof(pp));
+  pp.func = const_cast<void *>(gpuai::playout_kernel(policy, ctx->tuning.refill));
+  pp.gridDim = dim3(grid);
+  pp.blockDim = dim3(ctx->tuning.block_size);
+  pp.kernelParams = play_args;
+  e = hipGraphCreate(&g->graph, 0);
+  if (e == hipSuccess)
+    e = hipGraphAddMemcpyNode1D(&g->n_h2d, g->graph, nullptr, 0, g->d_blob, g->h_blob,
+                                g->blob_bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipGraphAddKernelNode(&g->n_reset, g->graph, &g->n_h2d, 1, &rp);
+  if (e == hipSuccess) e = hipGraphAddKernelNode(&g->n_play, g->graph, &g->n_reset, 1, &pp);
+  if (e == hipSuccess)
+    e = hipGraphAddMemcpyNode1D(&g->n_d2h, g->graph, &g->n_play, 1, g->h_out, g->d_out,
+                                g->out_bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    rc = hip_fail(ctx, e, "graph construction");
+    graph_release(g);
+    return rc;
+  }
+  *out = g;
+  return MI_OK;
+}
+
+int mi_graph_run(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                 uint32_t seed, uint32_t epoch, int8_t *winners, mi_stats *stats) {
+  if (!g) return MI_ERR_INVALID_ARGUMENT;
+  mi_ctx *ctx = g->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const auto t0 = Clock::now();
+  ctx->last_error.clear();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (npos > 0 && (!pos || !counts)) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "null batch");
+  if (npos > g->max_pos) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "more positions than the graph holds");
+  uint64_t total = 0;
+  int rc;
+  if ((rc = plan_batch(ctx, pos, npos, counts, 0, total))) return rc;
+  if (total > g->max_play) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "more playouts than the graph holds");
+  if (total == 0) {
+    finish_stats(stats, t0);
+    return MI_OK;
+  }
+  if (!winners) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "winners pointer is null");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  gpuai::LaunchDesc *d = (gpuai::LaunchDesc *)g->h_blob;
+  d->npos = npos;
+  d->total = (uint32_t)total;
+  d->seed = seed;
+  d->epoch = epoch;
+  std::memcpy((char *)g->h_blob + g->offs_off, ctx->host_offs.data(), sizeof(uint32_t) * ((size_t)npos + 1));
+  std::memcpy((char *)g->h_blob + g->pos_off, pos, sizeof(mi_pos) * (size_t)npos);
+  const size_t in_bytes = g->pos_off + sizeof(mi_pos) * (size_t)npos;
+  const size_t out_bytes = sizeof(gpuai::WorkWords) + (size_t)total;
+  MI_HIP(ctx, hipGraphExecMemcpyNodeSetParams1D(g->exec, g->n_h2d, g->d_blob, g->h_blob, in_bytes,
+                                                hipMemcpyHostToDevice));
+  MI_HIP(ctx, hipGraphExecMemcpyNodeSetParams1D(g->exec, g->n_d2h, g->h_out, g->d_out, out_bytes,
+                                                hipMemcpyDeviceToHost));
+  MI_HIP(ctx, hipEventRecord(g->e0, ctx->stream));
+  MI_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+  MI_HIP(ctx, hipEventRecord(g->e1, ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(winners, (const char *)g->h_out + sizeof(gpuai::WorkWords), (size_t)total);
+  if (stats) {
+    const gpuai::WorkWords *w = (const gpuai::WorkWords *)g->h_out;
+    float ms = 0.f;
+    MI_HIP(ctx, hipEventElapsedTime(&ms, g->e0, g->e1));
+    stats->playouts = total;
+    stats->plies = w->plies;
+    stats->kernel_ms = ms;
+    stats->max_moves = w->max_moves;
+    stats->grid_blocks = grid_for(ctx, g->max_play);
+    stats->block_size = ctx->tuning.block_size;
+    stats->refill = ctx->tuning.refill;
+  }
+  finish_stats(stats, t0);
+  return MI_OK;
+}
+
+int mi_graph_destroy(mi_graph *g) {
+  if (!g) return MI_OK;
+  std::lock_guard<std::mutex> lock(g->ctx->mu);
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  graph_release(g);
+  return MI_OK;
 }
 
 }  // extern "C"

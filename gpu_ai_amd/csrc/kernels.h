@@ -18,6 +18,16 @@ struct WorkWords {
   uint32_t pad1[10];
 };
 
+// Batch description read by the kernel from device memory (graph replays change it between
+// launches without touching the captured kernel arguments).
+struct LaunchDesc {
+  uint32_t npos;
+  uint32_t total;
+  uint32_t seed;
+  uint32_t epoch;
+  uint32_t pad[12];
+};
+
 struct PlayoutArgs {
   const mi_pos *pos;       // npos positions
   const uint32_t *offs;    // npos+1 prefix offsets of playout counts, or null (uniform per_pos)
@@ -31,7 +41,12 @@ struct PlayoutArgs {
   uint32_t *hashes;        // total (trace mode only)
   WorkWords *work;
   uint32_t defer_min;      // deferred multi-jump lanes that trigger a walk phase
+  const LaunchDesc *desc;  // when set, npos / total / seed / epoch come from here
 };
+
+// Kernel entry of the playout kernel for a policy and refill policy (graph construction).
+const void *playout_kernel(uint32_t policy, uint32_t refill);
+const void *reset_kernel();
 
 hipError_t launch_reset(WorkWords *work, hipStream_t stream);
 hipError_t launch_playouts(const PlayoutArgs &args, uint32_t policy, bool trace, uint32_t refill,

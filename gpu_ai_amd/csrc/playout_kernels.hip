@@ -73,6 +73,12 @@ __device__ __forceinline__ uint32_t take_word(Words &w, uint32_t ply) {
 
 template <uint32_t POLICY, bool TRACE, uint32_t REFILL>
 __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
+  if (a.desc) {  // graph replay: the batch description lives in device memory
+    a.npos = a.desc->npos;
+    a.total = a.desc->total;
+    a.seed = a.desc->seed;
+    a.epoch = a.desc->epoch;
+  }
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nthreads = gridDim.x * blockDim.x;
   bool active = false, deferred = false;
@@ -357,6 +363,18 @@ hipError_t launch_refill(const PlayoutArgs &args, uint32_t refill, uint32_t grid
 }
 
 }  // namespace
+
+const void *playout_kernel(uint32_t policy, uint32_t refill) {
+  if (policy == MI_POLICY_RANDOM)
+    return refill == MI_REFILL_STATIC
+               ? (const void *)k_playouts<MI_POLICY_RANDOM, false, MI_REFILL_STATIC>
+               : (const void *)k_playouts<MI_POLICY_RANDOM, false, MI_REFILL_QUEUE>;
+  return refill == MI_REFILL_STATIC
+             ? (const void *)k_playouts<MI_POLICY_HEURISTIC, false, MI_REFILL_STATIC>
+             : (const void *)k_playouts<MI_POLICY_HEURISTIC, false, MI_REFILL_QUEUE>;
+}
+
+const void *reset_kernel() { return (const void *)k_reset; }
 
 hipError_t launch_reset(WorkWords *work, hipStream_t stream) {
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, work);

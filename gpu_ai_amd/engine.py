@@ -83,6 +83,9 @@ _SIGS = {
     "mi_gen_start_positions": ([_P, _U32, _U32, _U32, _U32, _U32, _P], ctypes.c_int),
     "mi_autotune": ([_P, _P, _U32, _P, _U32, _U32, _U32, _P, _U32, _P, ctypes.POINTER(MiTuning)],
                     ctypes.c_int),
+    "mi_graph_create": ([_P, _U32, _U32, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+    "mi_graph_run": ([_P, _P, _U32, _P, _U32, _U32, _P, ctypes.POINTER(MiStats)], ctypes.c_int),
+    "mi_graph_destroy": ([_P], ctypes.c_int),
     "mi_copy_calibration": ([_P, ctypes.c_uint64, _U32, ctypes.POINTER(ctypes.c_float)],
                             ctypes.c_int),
 }
@@ -271,6 +274,10 @@ class Engine:
         return ({"block_size": best.block_size, "blocks_per_cu": best.blocks_per_cu,
                  "refill": best.refill, "defer_min": best.defer_min}, rows)
 
+    def graph(self, max_positions: int, max_playouts: int,
+              policy: int = POLICY_RANDOM) -> "GraphBatch":
+        return GraphBatch(self, max_positions, max_playouts, policy)
+
     def copy_calibration(self, nbytes: int, repeats: int = 5) -> float:
         ms = ctypes.c_float()
         self._check(self.lib.mi_copy_calibration(self.ctx, nbytes, repeats, ctypes.byref(ms)))
@@ -284,3 +291,34 @@ class Engine:
         self._check(self.lib.mi_check_divmod(self.ctx, _ptr(r), _ptr(n), len(r), _ptr(q),
                                              _ptr(rem)))
         return q, rem
+
+
+class GraphBatch:
+    """A hipGraph of (copy in, reset, persistent playout kernel, copy out) replayed per batch."""
+
+    def __init__(self, engine: Engine, max_positions: int, max_playouts: int, policy: int):
+        self.engine = engine
+        self.handle = ctypes.c_void_p()
+        engine._check(engine.lib.mi_graph_create(engine.ctx, max_positions, max_playouts, policy,
+                                                 ctypes.byref(self.handle)))
+
+    def run(self, pos, counts, seed: int = 12345, epoch: int = 0) -> PlayoutResult:
+        pos = as_positions(pos)
+        c = np.ascontiguousarray(counts, dtype=np.uint32)
+        winners = np.empty(int(c.sum(dtype=np.uint64)), dtype=np.int8)
+        st = MiStats()
+        self.engine._check(self.engine.lib.mi_graph_run(self.handle, _ptr(pos), len(pos), _ptr(c),
+                                                        seed, epoch, _ptr(winners),
+                                                        ctypes.byref(st)))
+        return PlayoutResult(winners, st.as_dict())
+
+    def close(self) -> None:
+        if self.handle:
+            self.engine.lib.mi_graph_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -157,6 +157,21 @@ int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
                 uint32_t per_pos, uint32_t policy, uint32_t reps, mi_tune_result *results,
                 uint32_t max_results, uint32_t *nresults, mi_tuning *best);
 
+/* Graph-replayed batches for search loops (no counterpart in the reference, which launches,
+ * copies and frees per call).  mi_graph_create fixes the capacities and builds one hipGraph on
+ * the context's stream: copy of the batch (positions, playout offsets, seed, epoch) from pinned
+ * host memory, a reset kernel, the persistent playout kernel with the context's tuning, and the
+ * copy of the winners back to pinned memory.  mi_graph_run fills the pinned buffers, resizes the
+ * two copies to the batch, replays the graph and returns the winners as mi_run_playouts would
+ * (same streams).  kernel_ms in the stats is the device time of the whole replay.  A graph is
+ * used by one thread at a time and must be destroyed before its context. */
+typedef struct mi_graph mi_graph;
+int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, uint32_t policy,
+                    mi_graph **graph);
+int mi_graph_run(mi_graph *graph, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                 uint32_t seed, uint32_t epoch, int8_t *winners, mi_stats *stats);
+int mi_graph_destroy(mi_graph *graph);
+
 /* Device-resident batches (inputs stay in HBM between launches).
  * mi_stage uploads positions and playout counts once; mi_launch_staged runs the staged batch
  * and leaves winners on the device; mi_fetch_winners copies them back. */

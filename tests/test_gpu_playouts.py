@@ -199,3 +199,23 @@ def test_device_start_positions_match_reference_golden(engine, golden):
     c2 = engine.start_positions(2016, 1024, 20, 60, True)
     assert c1.tobytes() == golden["positions"]["c1"].tobytes()
     assert c2.tobytes() == golden["positions"]["c2"].tobytes()
+
+
+@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
+def test_graph_replays_match_eager(engine, golden, policy):
+    """Replays one captured graph on batches of different sizes, with eager launches and
+    synchronisations in between, and checks every replay against mi_run_playouts."""
+    pos = golden["positions"]["c2" if policy else "c1"]
+    rng = np.random.default_rng(9 + policy)
+    with engine.graph(max_positions=1024, max_playouts=300000, policy=policy) as g:
+        for rep in range(6):
+            npos = int(rng.integers(1, 1024))
+            sel = pos[rng.choice(len(pos), npos, replace=False)]
+            counts = rng.integers(0, 400, npos).astype(np.uint32)
+            announce(f"graph replay {rep}: {npos} positions, {int(counts.sum())} playouts, policy {policy}")
+            got = g.run(sel, counts, seed=31 + rep, epoch=rep)
+            eager = engine.run_playouts(sel, counts=counts, seed=31 + rep, epoch=rep, policy=policy)
+            assert np.array_equal(got.winners, eager.winners), rep
+            assert got.stats["plies"] == eager.stats["plies"]
+        with pytest.raises(Exception):
+            g.run(pos[:2], np.array([200000, 200000], dtype=np.uint32))
