@@ -77,12 +77,34 @@ def cpu_baseline(cfg: dict, pos, threads: int) -> dict:
                       f"({npos * per} playouts, {dt:.2f} s wall on {threads} threads)"}
 
 
+def bench_multisearch() -> int:
+    """Configuration 3: 256 concurrent searches, budgets uniform in [1000, 64000] playouts,
+    UCT c = sqrt(2).  One graph replay (persistent refill kernel) per iteration versus one
+    playout launch per search per iteration."""
+    from gpu_ai_amd import host_api
+    runs = {m: host_api.multisearch(mode=m) for m in ("graph", "per_search")}
+    g, p = runs["graph"], runs["per_search"]
+    line = {
+        "metric": "aggregate playouts/s of 256 concurrent MCTS searches on 1 MI355X",
+        "value": round(g["playouts_per_second"], 1), "unit": "playouts/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "int32", "data": "synthetic (seeded positions)",
+        "config": {"workload": "config 3: 256 searches from seeded distinct positions, budgets "
+                               "uniform in [1000, 64000] playouts, trials per iteration "
+                               "max(512, totalTrials/2), UCT c=sqrt(2)"},
+        "graph": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in g.items()},
+        "per_search": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()},
+        "speedup_vs_per_search": round(g["playouts_per_second"] / p["playouts_per_second"], 3),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS) + ["c3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shape instead of tuning on the batch")
@@ -94,6 +116,9 @@ def main() -> int:
         print("bench.py measures one GPU only: the playout path runs on a single MI355X here "
               "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
         return 2
+
+    if args.config == "c3":
+        return bench_multisearch()
 
     from gpu_ai_amd.engine import Engine
 

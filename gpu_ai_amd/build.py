@@ -35,7 +35,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I" + os.path
 # (source, object) pairs of the device library
 LIB_SOURCES = ["playout_kernels.hip", "capi.cpp"]
 # the C++ host layer (caller side: State, PlayoutDriver, GameTree, players, CLI)
-HOST_SOURCES = ["state.cpp", "playout.cpp", "mcts.cpp", "player.cpp", "capi_host.cpp"]
+HOST_SOURCES = ["state.cpp", "playout.cpp", "mcts.cpp", "player.cpp", "multisearch.cpp",
+                "capi_host.cpp"]
 HOST_LIB_PATH = os.path.join(PKG, "libgpuai_host.so")
 RUN_AI_PATH = os.path.join(BIN, "run_ai")
 

@@ -1,9 +1,11 @@
 // capi_host.cpp -- C entry points of the host layer (include/gpuai/gpuai_host.h).
 #include <cstring>
 #include <exception>
+#include <stdexcept>
 
 #include "gpuai/gpuai_host.h"
 #include "mcts.hpp"
+#include "multisearch.hpp"
 #include "state.hpp"
 
 namespace {
@@ -101,6 +103,42 @@ int gpuai_mcts_trace(const mi_pos *root, uint32_t iterations, uint32_t initial, 
     return (int)w;
   } catch (const std::exception &) {
     return -2;
+  }
+}
+
+int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch_result *out,
+                          char *err, uint32_t err_len) {
+  try {
+    MultiSearchConfig c;
+    c.searches = cfg->searches;
+    c.seed = cfg->seed;
+    c.playoutSeed = cfg->playout_seed;
+    c.minBudget = cfg->min_budget;
+    c.maxBudget = cfg->max_budget;
+    c.initial = cfg->initial;
+    c.scale = cfg->scale;
+    c.policy = cfg->policy;
+    c.mode = (MultiSearchMode)cfg->mode;
+    if (c.maxBudget < c.minBudget || c.searches == 0 || cfg->mode > 2)
+      throw std::runtime_error("invalid multi-search configuration");
+    MultiSearchResult r = runMultiSearch(c);
+    out->playouts = r.playouts;
+    out->iterations = r.iterations;
+    out->pad = 0;
+    out->wall_seconds = r.wallSeconds;
+    out->device_ms = r.deviceMs;
+    out->p50_ms = r.p50Ms;
+    out->p99_ms = r.p99Ms;
+    out->max_ms = r.maxMs;
+    out->playouts_per_second = r.playoutsPerSecond;
+    out->root_digest = r.rootDigest;
+    return 0;
+  } catch (const std::exception &e) {
+    if (err && err_len) {
+      std::strncpy(err, e.what(), err_len - 1);
+      err[err_len - 1] = 0;
+    }
+    return -1;
   }
 }
 

@@ -1,0 +1,164 @@
+// multisearch.cpp -- many concurrent searches sharing one device (see multisearch.hpp).
+#include "multisearch.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <set>
+#include <stdexcept>
+#include <tuple>
+
+#include "checkers.h"
+#include "mcts.hpp"
+#include "playout.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+uint64_t mix64(uint64_t h, uint64_t v) {
+  h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h;
+}
+
+}  // namespace
+
+void multiSearchSetup(const MultiSearchConfig &cfg, std::vector<mi_pos> &roots,
+                      std::vector<uint32_t> &budgets) {
+  mi_ctx *ctx = nullptr;
+  checkMi(mi_create(&ctx, cfg.device, nullptr), nullptr, "mi_create");
+  // draw twice as many candidates as needed and keep the first distinct ones
+  std::vector<mi_pos> cand((size_t)cfg.searches * 2 + 64);
+  int rc = mi_gen_start_positions(ctx, cfg.seed, (uint32_t)cand.size(), 1, 30, 1, cand.data());
+  std::string err = rc ? mi_last_error(ctx) : "";
+  mi_destroy(ctx);
+  if (rc) throw std::runtime_error("mi_gen_start_positions: " + err);
+  std::set<std::tuple<uint32_t, uint32_t, uint32_t, uint8_t, uint8_t>> seen;
+  roots.clear();
+  for (const mi_pos &p : cand) {
+    if (roots.size() == cfg.searches) break;
+    if (seen.insert(std::make_tuple(p.p1, p.p2, p.kings, p.turn, p.counter)).second) roots.push_back(p);
+  }
+  if (roots.size() < cfg.searches) throw std::runtime_error("not enough distinct start positions");
+  budgets.resize(cfg.searches);
+  const uint32_t span = cfg.maxBudget - cfg.minBudget + 1;
+  for (uint32_t s = 0; s < cfg.searches; s++) {
+    uint32_t w[4];
+    ck::philox(0u, 0u, ck::TAG_SEARCH, 0u, cfg.seed, s, w);
+    budgets[s] = cfg.minBudget + w[0] % span;
+  }
+}
+
+MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
+  std::vector<mi_pos> roots;
+  std::vector<uint32_t> budgets;
+  multiSearchSetup(cfg, roots, budgets);
+  const uint32_t S = cfg.searches;
+  std::vector<std::unique_ptr<GameTree>> trees;
+  for (const mi_pos &p : roots) trees.emplace_back(new GameTree(State::unpack(p)));
+  std::vector<uint32_t> remaining = budgets;
+  uint64_t maxIter = 0;
+  for (uint32_t b : budgets) maxIter += b;
+
+  DeviceMI355XPlayoutDriver driver(cfg.policy, cfg.device, cfg.playoutSeed);
+  mi_graph *graph = nullptr;
+  if (cfg.mode == MultiSearchMode::GRAPH) {
+    // the largest batch: every search's largest possible iteration
+    uint64_t cap = 0;
+    for (uint32_t b : budgets) cap += b;
+    cap = std::min<uint64_t>(cap, (uint64_t)S * std::max<uint64_t>(cfg.initial, cfg.maxBudget));
+    checkMi(mi_graph_create(driver.context(), (uint32_t)std::min<uint64_t>(cap, 1u << 22),
+                            (uint32_t)std::min<uint64_t>(cap, (1u << 31) - 1), cfg.policy, &graph),
+            driver.context(), "mi_graph_create");
+  }
+
+  MultiSearchResult res;
+  std::vector<std::vector<State>> leaves(S);
+  std::vector<std::vector<PlayerId>> results(S);
+  std::vector<mi_pos> pos;
+  std::vector<uint32_t> counts;
+  std::vector<int8_t> winners;
+  const auto wall0 = Clock::now();
+  for (uint32_t it = 0;; it++) {
+    const auto t0 = Clock::now();
+    bool any = false;
+    for (uint32_t s = 0; s < S; s++) any |= remaining[s] > 0;
+    if (!any) break;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long s = 0; s < (long)S; s++) {
+      leaves[s].clear();
+      if (remaining[s] == 0) continue;
+      unsigned n = (unsigned)(trees[s]->getTotalTrials() * cfg.scale);
+      n = std::max<unsigned>(n, cfg.initial);
+      n = std::min<unsigned>(n, remaining[s]);
+      leaves[s] = trees[s]->select(n);
+    }
+    uint64_t iterPlayouts = 0;
+    if (cfg.mode == MultiSearchMode::PER_SEARCH) {
+      for (uint32_t s = 0; s < S; s++) {
+        if (leaves[s].empty()) continue;
+        results[s] = driver.runPlayouts(leaves[s]);
+        iterPlayouts += leaves[s].size();
+      }
+    } else {
+      pos.clear();
+      counts.clear();
+      for (uint32_t s = 0; s < S; s++) {
+        const std::vector<State> &L = leaves[s];
+        for (size_t i = 0; i < L.size(); i++) {
+          if (i > 0 && L[i] == L[i - 1]) {
+            counts.back()++;
+          } else {
+            pos.push_back(L[i].pack());
+            counts.push_back(1);
+          }
+        }
+        iterPlayouts += L.size();
+      }
+      mi_stats st;
+      if (cfg.mode == MultiSearchMode::GRAPH) {
+        winners.resize(iterPlayouts);
+        checkMi(mi_graph_run(graph, pos.data(), (uint32_t)pos.size(), counts.data(), driver.seed(),
+                             driver.nextEpoch(), winners.data(), &st),
+                driver.context(), "mi_graph_run");
+      } else {
+        driver.runGrouped(pos, counts, winners, &st);
+      }
+      res.deviceMs += st.kernel_ms;
+      size_t at = 0;
+      for (uint32_t s = 0; s < S; s++) {
+        results[s].resize(leaves[s].size());
+        for (size_t i = 0; i < leaves[s].size(); i++) results[s][i] = (PlayerId)winners[at++];
+      }
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long s = 0; s < (long)S; s++) {
+      if (leaves[s].empty()) continue;
+      trees[s]->update(results[s]);
+      remaining[s] -= (uint32_t)leaves[s].size();
+    }
+    res.playouts += iterPlayouts;
+    res.iterations++;
+    res.iterationMs.push_back(std::chrono::duration<double, std::milli>(Clock::now() - t0).count());
+    if (res.iterations > maxIter) throw std::runtime_error("multi-search did not converge");
+  }
+  res.wallSeconds = std::chrono::duration<double>(Clock::now() - wall0).count();
+  if (graph) mi_graph_destroy(graph);
+  std::vector<double> lat = res.iterationMs;
+  std::sort(lat.begin(), lat.end());
+  if (!lat.empty()) {
+    res.p50Ms = lat[lat.size() / 2];
+    res.p99Ms = lat[std::min(lat.size() - 1, (size_t)std::ceil(0.99 * lat.size()) - 1)];
+    res.maxMs = lat.back();
+  }
+  res.playoutsPerSecond = res.wallSeconds > 0 ? res.playouts / res.wallSeconds : 0;
+  uint64_t h = 0;
+  for (auto &t : trees) {
+    h = mix64(h, t->getTotalTrials());
+    h = mix64(h, (uint64_t)t->getWins(PLAYER_1));
+    h = mix64(h, (uint64_t)t->getWins(PLAYER_2));
+  }
+  res.rootDigest = h;
+  return res;
+}

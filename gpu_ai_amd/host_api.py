@@ -13,6 +13,22 @@ _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
 _lib = None
 
+MODES = {"graph": 0, "batched": 1, "per_search": 2}
+
+
+class MultiSearchConfig(ctypes.Structure):
+    _fields_ = [("searches", _U32), ("seed", _U32), ("playout_seed", _U32), ("min_budget", _U32),
+                ("max_budget", _U32), ("initial", _U32), ("scale", ctypes.c_double),
+                ("policy", _U32), ("mode", _U32)]
+
+
+class MultiSearchResult(ctypes.Structure):
+    _fields_ = [("playouts", ctypes.c_uint64), ("iterations", _U32), ("pad", _U32),
+                ("wall_seconds", ctypes.c_double), ("device_ms", ctypes.c_double),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("playouts_per_second", ctypes.c_double),
+                ("root_digest", ctypes.c_uint64)]
+
 
 def host_library() -> ctypes.CDLL:
     global _lib
@@ -27,6 +43,10 @@ def host_library() -> ctypes.CDLL:
         lib.gpuai_host_apply.restype = ctypes.c_int
         lib.gpuai_mcts_trace.argtypes = [_P, _U32, _U32, ctypes.c_double, _P, _U32]
         lib.gpuai_mcts_trace.restype = ctypes.c_int
+        lib.gpuai_multisearch_run.argtypes = [ctypes.POINTER(MultiSearchConfig),
+                                              ctypes.POINTER(MultiSearchResult), ctypes.c_char_p,
+                                              _U32]
+        lib.gpuai_multisearch_run.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -63,3 +83,16 @@ def mcts_trace(root, iterations: int, initial: int, scale: float) -> np.ndarray:
 
 def run_ai_path() -> str:
     return RUN_AI_PATH
+
+
+def multisearch(searches: int = 256, seed: int = 2016, playout_seed: int = 12345,
+                min_budget: int = 1000, max_budget: int = 64000, initial: int = 512,
+                scale: float = 0.5, policy: int = 0, mode: str = "graph") -> dict:
+    """Configuration 3: concurrent searches sharing the device; returns the run's statistics."""
+    cfg = MultiSearchConfig(searches, seed, playout_seed, min_budget, max_budget, initial, scale,
+                            policy, MODES[mode])
+    res = MultiSearchResult()
+    err = ctypes.create_string_buffer(512)
+    if host_library().gpuai_multisearch_run(ctypes.byref(cfg), ctypes.byref(res), err, 512) != 0:
+        raise RuntimeError(err.value.decode())
+    return {k: getattr(res, k) for k, _ in MultiSearchResult._fields_ if k != "pad"}

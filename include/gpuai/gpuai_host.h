@@ -36,6 +36,31 @@ int gpuai_host_apply(mi_pos *pos, const mi_move *move);
 int gpuai_mcts_trace(const mi_pos *root, uint32_t iterations, uint32_t initial, double scale,
                      uint32_t *trace, uint32_t trace_cap);
 
+/* Configuration 3: concurrent MCTS searches sharing the device (multisearch.hpp). */
+typedef struct gpuai_multisearch_config {
+  uint32_t searches;      /* 256 */
+  uint32_t seed;          /* start positions and budgets (2016) */
+  uint32_t playout_seed;  /* playout streams (12345) */
+  uint32_t min_budget;    /* 1000 */
+  uint32_t max_budget;    /* 64000 */
+  uint32_t initial;       /* trials per iteration: max(initial, totalTrials * scale) */
+  double scale;
+  uint32_t policy;        /* MI_POLICY_* */
+  uint32_t mode;          /* 0 graph replay, 1 one plain launch, 2 one launch per search */
+} gpuai_multisearch_config;
+
+typedef struct gpuai_multisearch_result {
+  uint64_t playouts;
+  uint32_t iterations;
+  uint32_t pad;
+  double wall_seconds, device_ms, p50_ms, p99_ms, max_ms, playouts_per_second;
+  uint64_t root_digest;
+} gpuai_multisearch_result;
+
+/* Runs configuration 3 on device 0; returns 0, or -1 with a message in `err` (err_len bytes). */
+int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch_result *out,
+                          char *err, uint32_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

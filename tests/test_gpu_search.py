@@ -1,0 +1,52 @@
+"""Searches on the device: the drop-in driver under GameTree, batched multi-search, run_ai."""
+from __future__ import annotations
+
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg: str) -> None:
+    print(f"[gpu] {msg}", file=sys.stderr, flush=True)
+
+
+def test_multisearch_graph_equals_plain_batched():
+    from gpu_ai_amd import host_api
+    announce("multi-search 24 searches, graph vs batched")
+    kw = dict(searches=24, min_budget=200, max_budget=3000, initial=64, scale=0.5)
+    g = host_api.multisearch(mode="graph", **kw)
+    b = host_api.multisearch(mode="batched", **kw)
+    assert g["playouts"] == b["playouts"] > 0
+    assert g["iterations"] == b["iterations"]
+    assert g["root_digest"] == b["root_digest"]
+    announce("multi-search per-search launches")
+    p = host_api.multisearch(mode="per_search", **kw)
+    assert p["playouts"] == g["playouts"]
+
+
+def test_run_ai_modes():
+    from gpu_ai_amd import host_api
+    exe = host_api.run_ai_path()
+    announce("run_ai gen_moves_test")
+    out = subprocess.run([exe, "-m", "gen_moves_test", "-n", "3000"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0 and "Passed" in out.stdout, out.stdout
+    announce("run_ai playout_test")
+    out = subprocess.run([exe, "-m", "playout_test", "-n", "20000", "-1", "device_mi355x", "-2",
+                          "device_coarse"], capture_output=True, text=True, timeout=120)
+    # both names select the same engine and the same streams except for the epoch: results are
+    # statistically alike, and the tallies are printed as the reference prints them
+    assert out.returncode == 0 and "Similar results" in out.stdout, out.stdout
+    announce("run_ai game_test random vs mcts_device_mi355x")
+    out = subprocess.run([exe, "-m", "game_test", "-n", "2", "-1", "random", "-2",
+                          "mcts_device_mi355x", "--timeout", "0.02"], capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0 and "=== Results ===" in out.stdout, out.stdout
+    announce("run_ai unknown driver")
+    out = subprocess.run([exe, "-m", "playout_test", "-n", "10", "-1", "nope"], capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 1 and "Unknown playout type" in out.stdout
