@@ -89,8 +89,8 @@ def bench_multisearch() -> int:
         "value": round(g["playouts_per_second"], 1), "unit": "playouts/s", "n_gpus": 1,
         "higher_is_better": True, "dtype": "int32", "data": "synthetic (seeded positions)",
         "config": {"workload": "config 3: 256 searches from seeded distinct positions, budgets "
-                               "uniform in [1000, 64000] playouts, trials per iteration "
-                               "max(512, totalTrials/2), UCT c=sqrt(2)"},
+                               "uniform in [1000, 64000] playouts, 256 trials per search "
+                               "per iteration, UCT c=sqrt(2)"},
         "graph": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in g.items()},
         "per_search": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()},
         "speedup_vs_per_search": round(g["playouts_per_second"] / p["playouts_per_second"], 3),

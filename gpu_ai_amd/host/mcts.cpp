@@ -5,7 +5,10 @@
 #include <cmath>
 #include <stdexcept>
 
-GameTree::GameTree(State s, GameTree *parent) : state(s), parent(parent) { moves = s.getMoves(); }
+GameTree::GameTree(State s, GameTree *parent) : state(s), parent(parent) {
+  moves = s.getMoves();
+  over = moves.empty() || s.movesSinceLastCapture >= NUM_DRAW_MOVES;  // src/state.cpp:16-18
+}
 
 GameTree::~GameTree() {
   for (auto &c : children) c->parent = nullptr;
@@ -25,7 +28,7 @@ std::shared_ptr<GameTree> GameTree::move(const Move &m) {
 }
 
 double GameTree::getScore(PlayerId player) const {
-  if (state.isGameOver()) return state.getNextTurn() == player ? 1.0 : 0.0;
+  if (over) return state.getNextTurn() == player ? 1.0 : 0.0;
   return wins[player] / totalTrials;
 }
 
@@ -49,12 +52,21 @@ void GameTree::expand(unsigned numPlayouts, PlayoutDriver *driver) {
 }
 
 std::vector<State> GameTree::select(unsigned trials) {
-  if (state.isGameOver()) {  // every trial of a finished game is played from it
+  std::vector<State> out;
+  out.reserve(trials);
+  selectInto(trials, out);
+  return out;
+}
+
+void GameTree::selectInto(unsigned trials, std::vector<State> &out) {
+  if (over) {  // every trial of a finished game is played from it
     assignedTrials = trials;
-    return std::vector<State>(trials, state);
+    out.insert(out.end(), trials, state);
+    return;
   }
   if (!expanded) {
     if (trials > 1) {
+      children.reserve(moves.size());
       for (const Move &m : moves) {
         State next = state;
         next.move(m);
@@ -63,18 +75,22 @@ std::vector<State> GameTree::select(unsigned trials) {
       expanded = true;
     } else {
       assignedTrials = trials;
-      return trials == 1 ? std::vector<State>{state} : std::vector<State>{};
+      if (trials == 1) out.push_back(state);
+      return;
     }
   }
   const size_t n = children.size();
-  std::vector<unsigned> share(n, 0);
-  std::vector<double> weight(n);
+  unsigned share[MAX_MOVES];
+  double weight[MAX_MOVES];
+  bool topped[MAX_MOVES];
   double weightSum = 0;
   unsigned untried = 0;
   for (size_t i = 0; i < n; i++) {
     weight[i] = children[i]->ucb1();
     weightSum += weight[i];
     if (children[i]->totalTrials == 0) untried++;
+    share[i] = 0;
+    topped[i] = false;
   }
   assignedTrials = 0;
   for (size_t i = 0; i < n; i++) {
@@ -85,7 +101,6 @@ std::vector<State> GameTree::select(unsigned trials) {
     assignedTrials += share[i];
   }
   // the remainder goes one trial each to the children of largest weight, first index first
-  std::vector<bool> topped(n, false);
   while (assignedTrials < trials) {
     double maxWeight = -INFINITY;
     int pick = -1;
@@ -100,39 +115,46 @@ std::vector<State> GameTree::select(unsigned trials) {
     share[pick]++;
     assignedTrials++;
   }
+  if (!parallel) {
+    for (size_t i = 0; i < n; i++) children[i]->selectInto(share[i], out);
+    return;
+  }
   std::vector<std::vector<State>> parts(n);
 #pragma omp parallel for
-  for (long i = 0; i < (long)n; i++) parts[i] = children[i]->select(share[i]);
-  std::vector<State> out;
-  out.reserve(trials);
+  for (long i = 0; i < (long)n; i++) {
+    parts[i].reserve(share[i]);
+    children[i]->selectInto(share[i], parts[i]);
+  }
   for (auto &p : parts) out.insert(out.end(), p.begin(), p.end());
-  return out;
 }
 
 template <class R, class Add>
-void GameTree::updateImpl(const std::vector<R> &results, Add add) {
-  if (results.size() != assignedTrials)
+void GameTree::updateImpl(const R *results, size_t count, Add add) {
+  if (count != assignedTrials)
     throw std::runtime_error("update: result count differs from the trials handed out");
   totalTrials += assignedTrials;
   if (expanded) {
     size_t at = 0;
     for (auto &c : children) {
-      std::vector<R> part(results.begin() + at, results.begin() + at + c->assignedTrials);
+      c->updateImpl(results + at, c->assignedTrials, add);
       at += c->assignedTrials;
-      c->updateImpl(part, add);
     }
   }
-  for (const R &r : results) add(wins, r);
+  for (size_t i = 0; i < count; i++) add(wins, results[i]);
 }
 
-void GameTree::update(const std::vector<PlayerId> &results) {
-  updateImpl(results, [](double *w, PlayerId r) {
+void GameTree::update(const std::vector<PlayerId> &results) { update(results.data(), results.size()); }
+
+void GameTree::update(const PlayerId *results, size_t n) {
+  updateImpl(results, n, [](double *w, PlayerId r) {
     if (r != PLAYER_NONE) w[r] += 1.0;
   });
 }
 
-void GameTree::updateValues(const std::vector<float> &p1WinProb) {
-  updateImpl(p1WinProb, [](double *w, float p) {
+void GameTree::updateValues(const std::vector<float> &p) { updateValues(p.data(), p.size()); }
+
+void GameTree::updateValues(const float *p1WinProb, size_t n) {
+  updateImpl(p1WinProb, n, [](double *w, float p) {
     w[PLAYER_1] += p;
     w[PLAYER_2] += 1.0 - p;
   });

@@ -28,11 +28,18 @@ class GameTree {
   void expand(unsigned numPlayouts, PlayoutDriver *playoutDriver);
   // Distributes `trials` by UCB1, creating nodes as needed (src/mcts.cpp:63-157).
   std::vector<State> select(unsigned trials);
+  // As select, appending the trial states to `out` (no per-node allocation).
+  void selectInto(unsigned trials, std::vector<State> &out);
   // Adds the winners of the trials handed out by the last select (src/mcts.cpp:159-180).
   void update(const std::vector<PlayerId> &results);
+  void update(const PlayerId *results, size_t n);
   // As update, with each trial scored by player 1's win probability p: player 1 gains p and
   // player 2 gains 1 - p (value-network leaves).
   void updateValues(const std::vector<float> &p1WinProb);
+  void updateValues(const float *p1WinProb, size_t n);
+  // Whether select parallelises over the root's children with OpenMP (as the reference does at
+  // every node, src/mcts.cpp:146-149); searches run in parallel with each other switch it off.
+  void setParallel(bool on) { parallel = on; }
 
   const State state;
 
@@ -46,10 +53,12 @@ class GameTree {
  private:
   double ucb1() const;  // src/mcts.cpp:182-191
   template <class R, class Add>
-  void updateImpl(const std::vector<R> &results, Add add);
+  void updateImpl(const R *results, size_t n, Add add);
 
   const GameTree *parent;
   bool expanded = false;
+  bool over = false;       // isGameOver() of the node's state, fixed at construction
+  bool parallel = false;
   std::vector<Move> moves;
   std::vector<std::shared_ptr<GameTree>> children;
   unsigned assignedTrials = 0;

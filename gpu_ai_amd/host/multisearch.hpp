@@ -25,8 +25,8 @@ struct MultiSearchConfig {
   uint32_t playoutSeed = 12345;  // playout streams
   uint32_t minBudget = 1000;     // playouts per search, uniform in [minBudget, maxBudget]
   uint32_t maxBudget = 64000;
-  uint32_t initial = 512;        // trials per iteration: max(initial, totalTrials * scale),
-  double scale = 0.5;            //   capped by the remaining budget
+  uint32_t initial = 256;        // trials per iteration: max(initial, totalTrials * scale),
+  double scale = 0.0;            //   capped by the remaining budget
   uint32_t policy = MI_POLICY_RANDOM;
   MultiSearchMode mode = MultiSearchMode::GRAPH;
   int device = 0;

@@ -17,7 +17,9 @@ MCTSPlayer::MCTSPlayer(unsigned initialNumPlayouts, float numPlayoutsScale, doub
       numPlayoutsScale(numPlayoutsScale),
       timeout(timeout),
       playoutDriver(std::move(playoutDriver)),
-      tree(std::make_shared<GameTree>(getStartingState())) {}
+      tree(std::make_shared<GameTree>(getStartingState())) {
+  tree->setParallel(true);
+}
 
 MCTSPlayer::~MCTSPlayer() {
   if (running) stop();
@@ -28,6 +30,7 @@ Move MCTSPlayer::getMove(const State &state, bool verbose) {
     std::lock_guard<std::mutex> lock(treeMutex);
     if (state != tree->state) {
       tree = std::make_shared<GameTree>(state);
+      tree->setParallel(true);
       updateCancelled = true;
     }
   }
@@ -48,6 +51,7 @@ Move MCTSPlayer::getMove(const State &state, bool verbose) {
 void MCTSPlayer::move(const Move &m) {
   std::lock_guard<std::mutex> lock(treeMutex);
   tree = tree->move(m);
+  tree->setParallel(true);
   updateCancelled = true;
 }
 

@@ -86,8 +86,8 @@ def run_ai_path() -> str:
 
 
 def multisearch(searches: int = 256, seed: int = 2016, playout_seed: int = 12345,
-                min_budget: int = 1000, max_budget: int = 64000, initial: int = 512,
-                scale: float = 0.5, policy: int = 0, mode: str = "graph") -> dict:
+                min_budget: int = 1000, max_budget: int = 64000, initial: int = 256,
+                scale: float = 0.0, policy: int = 0, mode: str = "graph") -> dict:
     """Configuration 3: concurrent searches sharing the device; returns the run's statistics."""
     cfg = MultiSearchConfig(searches, seed, playout_seed, min_budget, max_budget, initial, scale,
                             policy, MODES[mode])
