@@ -17,31 +17,10 @@
 #include "gpuai/mi_playout.h"
 #include "kernels.h"
 
-namespace {
+#include "ctx.h"
 
-struct DevBuf {
-  void *ptr = nullptr;
-  size_t bytes = 0;
-};
-
-}  // namespace
-
-struct mi_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-  mi_tuning tuning = {512u, 2u, MI_REFILL_QUEUE, 8u};
-  uint32_t cus = 0, wave = 64, clock_khz = 0;
-  DevBuf pos, offs, winners, plies, hashes, work, tallies, moves, counts;
-  // staged batch
-  uint32_t staged_npos = 0, staged_per_pos = 0, staged_total = 0;
-  bool staged = false, staged_offs = false;
-  std::vector<uint32_t> host_offs;
-  std::string last_error;
-  std::mutex mu;
-};
-
-namespace {
+namespace gpuai {
+namespace ctxutil {
 
 int fail(mi_ctx *ctx, int code, const std::string &msg) {
   if (ctx) ctx->last_error = msg;
@@ -54,11 +33,6 @@ int hip_fail(mi_ctx *ctx, hipError_t e, const char *what) {
   return fail(ctx, code, msg);
 }
 
-#define MI_HIP(ctx, call)                                 \
-  do {                                                    \
-    hipError_t e_ = (call);                               \
-    if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
-  } while (0)
 
 int ensure(mi_ctx *ctx, DevBuf &b, size_t bytes) {
   if (bytes <= b.bytes && b.ptr) return MI_OK;
@@ -75,6 +49,16 @@ int ensure(mi_ctx *ctx, DevBuf &b, size_t bytes) {
   b.bytes = want;
   return MI_OK;
 }
+
+}  // namespace ctxutil
+}  // namespace gpuai
+
+using gpuai::DevBuf;
+using gpuai::ctxutil::ensure;
+using gpuai::ctxutil::fail;
+using gpuai::ctxutil::hip_fail;
+
+namespace {
 
 int check_tuning(mi_ctx *ctx, const mi_tuning &t) {
   if (t.block_size < 64 || t.block_size > 1024 || t.block_size % 64)

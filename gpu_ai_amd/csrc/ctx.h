@@ -1,0 +1,49 @@
+// ctx.h -- the context object behind mi_ctx (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gpuai/mi_playout.h"
+
+namespace gpuai {
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+}  // namespace gpuai
+
+struct mi_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  mi_tuning tuning = {512u, 2u, MI_REFILL_QUEUE, 8u};
+  uint32_t cus = 0, wave = 64, clock_khz = 0;
+  gpuai::DevBuf pos, offs, winners, plies, hashes, work, tallies, moves, counts;
+  // staged batch
+  uint32_t staged_npos = 0, staged_per_pos = 0, staged_total = 0;
+  bool staged = false, staged_offs = false;
+  std::vector<uint32_t> host_offs;
+  std::string last_error;
+  std::mutex mu;
+};
+
+namespace gpuai {
+namespace ctxutil {
+// Records `msg` as the context's last error and returns `code`.
+int fail(mi_ctx *ctx, int code, const std::string &msg);
+// Records a HIP failure and returns MI_ERR_DEVICE or MI_ERR_OUT_OF_MEMORY.
+int hip_fail(mi_ctx *ctx, hipError_t e, const char *what);
+// Grows a device buffer to at least `bytes`.
+int ensure(mi_ctx *ctx, DevBuf &b, size_t bytes);
+}  // namespace ctxutil
+}  // namespace gpuai
+
+#define MI_HIP(ctx, call)                                                 \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) return gpuai::ctxutil::hip_fail(ctx, e_, #call); \
+  } while (0)

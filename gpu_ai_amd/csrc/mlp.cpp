@@ -1,0 +1,427 @@
+// mlp.cpp -- the value network's C ABI (include/gpuai/mi_mlp.h): buffers, the sequence of
+// kernels of a forward pass, of a backward pass and of an Adam step, device-resident epochs and
+// checkpoints.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "checkers.h"
+#include "ctx.h"
+#include "gpuai/mi_mlp.h"
+#include "mlp_kernels.h"
+
+using gpuai::DevBuf;
+using gpuai::ctxutil::ensure;
+using gpuai::ctxutil::fail;
+using gpuai::ctxutil::hip_fail;
+namespace K = gpuai::mlp;
+
+struct mi_mlp {
+  mi_ctx *ctx = nullptr;
+  uint32_t max_rows = 0;
+  uint32_t step = 0;
+  float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+  // parameters, Adam moments, gradients (FP32) and BF16 operand copies
+  float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
+  uint16_t *W1t = nullptr, *W2t = nullptr, *W2r = nullptr;
+  // activations of one batch
+  uint16_t *X = nullptr, *H1 = nullptr, *H2 = nullptr, *dH2 = nullptr, *dH1 = nullptr;
+  float *y = nullptr, *dz = nullptr, *targets = nullptr, *loss = nullptr;
+  float *pW1 = nullptr, *pW2 = nullptr;  // split-K partials of dW1, dW2
+  mi_pos *pos = nullptr;
+  // device-resident training set
+  DevBuf ds_pos, ds_t, ds_perm, ds_loss;
+  uint32_t ds_n = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+uint32_t round64(uint32_t n) { return (n + 63u) & ~63u; }
+
+// Largest split length in {512, 256, 128, 64} that divides rows.
+uint32_t split_len(uint32_t rows) {
+  for (uint32_t k = 512; k >= 64; k >>= 1)
+    if (rows % k == 0) return k;
+  return 64;
+}
+
+template <class T>
+hipError_t dev_alloc(mi_mlp *n, T **p, size_t count) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T));
+  if (e == hipSuccess) {
+    n->allocs.push_back(q);
+    *p = (T *)q;
+  }
+  return e;
+}
+
+// He-uniform initialisation: U(-a, a), a = sqrt(6 / fan_in), from Philox key (seed, tensor),
+// counter (i >> 2, 0, 3, 0), word i & 3, mapped to [0, 1) by its top 24 bits.
+void init_params(uint32_t seed, std::vector<float> &w) {
+  w.assign(MI_MLP_PARAMS, 0.f);
+  struct T { uint32_t off, count, fan_in; } tensors[] = {
+      {MI_MLP_OFF_W1, 128 * 256, 128}, {MI_MLP_OFF_W2, 256 * 256, 256}, {MI_MLP_OFF_W3, 256, 256}};
+  for (uint32_t t = 0; t < 3; t++) {
+    const float a = std::sqrt(6.f / (float)tensors[t].fan_in);
+    uint32_t words[4];
+    for (uint32_t i = 0; i < tensors[t].count; i++) {
+      if ((i & 3) == 0) ck::philox(i >> 2, 0u, ck::TAG_MLP, 0u, seed, t, words);
+      const float u = (float)(words[i & 3] >> 8) * (1.f / 16777216.f);
+      w[tensors[t].off + i] = (2.f * u - 1.f) * a;
+    }
+  }
+}
+
+int upload_params(mi_mlp *n, const float *w) {
+  mi_ctx *ctx = n->ctx;
+  MI_HIP(ctx, hipMemcpyAsync(n->w, w, sizeof(float) * MI_MLP_PARAMS, hipMemcpyHostToDevice, ctx->stream));
+  MI_HIP(ctx, K::zero(n->m, MI_MLP_PARAMS, ctx->stream));
+  MI_HIP(ctx, K::zero(n->v, MI_MLP_PARAMS, ctx->stream));
+  MI_HIP(ctx, K::zero(n->g, MI_MLP_PARAMS, ctx->stream));
+  // refresh the BF16 copies without updating (splits = 0: W gradients read as 0)
+  K::AdamArgs a = {n->w, n->m, n->v, n->g, n->pW1, n->pW2, 0u, n->W1t, n->W2t, n->W2r,
+                   0.f, n->beta1, n->beta2, n->eps, 1.f, 1.f, 0};
+  MI_HIP(ctx, K::adam(a, ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  n->step = 0;
+  return MI_OK;
+}
+
+// Forward pass of `rows` encoded rows (X already filled); H1, H2 and y.
+int forward_rows(mi_mlp *n, uint32_t valid, uint32_t rows) {
+  mi_ctx *ctx = n->ctx;
+  K::GemmArgs l1 = {};
+  l1.A = n->X; l1.sAm = 128; l1.sAk = 1;
+  l1.B = n->W1t; l1.sBk = 1; l1.sBn = 128;
+  l1.M = (int)rows; l1.N = 256; l1.k_per_split = 128;
+  l1.ldc = 256; l1.Cbf = n->H1; l1.bias = n->w + MI_MLP_OFF_B1; l1.relu = 1;
+  MI_HIP(ctx, K::gemm(l1, 1, true, true, ctx->stream));
+  K::GemmArgs l2 = {};
+  l2.A = n->H1; l2.sAm = 256; l2.sAk = 1;
+  l2.B = n->W2t; l2.sBk = 1; l2.sBn = 256;
+  l2.M = (int)rows; l2.N = 256; l2.k_per_split = 256;
+  l2.ldc = 256; l2.Cbf = n->H2; l2.bias = n->w + MI_MLP_OFF_B2; l2.relu = 1;
+  MI_HIP(ctx, K::gemm(l2, 1, true, true, ctx->stream));
+  (void)valid;
+  return MI_OK;
+}
+
+// Backward pass after a forward pass whose head received targets: every gradient, then Adam
+// (or only the gradient sums when update == 0).
+int backward_rows(mi_mlp *n, uint32_t rows, float lr, int update) {
+  mi_ctx *ctx = n->ctx;
+  hipStream_t s = ctx->stream;
+  MI_HIP(ctx, K::colsum_bf16(n->H2, rows, 256, n->dz, n->g + MI_MLP_OFF_W3, s));
+  MI_HIP(ctx, K::colsum_f32(n->dz, rows, 1, nullptr, n->g + MI_MLP_OFF_B3, s));
+  MI_HIP(ctx, K::colsum_bf16(n->dH2, rows, 256, nullptr, n->g + MI_MLP_OFF_B2, s));
+  const uint32_t kps = split_len(rows), splits = rows / kps;
+  K::GemmArgs w2 = {};  // dW2[i][o] = sum_b H1[b][i] dH2[b][o]
+  w2.A = n->H1; w2.sAm = 1; w2.sAk = 256;
+  w2.B = n->dH2; w2.sBk = 256; w2.sBn = 1;
+  w2.M = 256; w2.N = 256; w2.k_per_split = (int)kps;
+  w2.C = n->pW2; w2.split_stride = 65536; w2.ldc = 256;
+  MI_HIP(ctx, K::gemm(w2, (int)splits, true, true, s));
+  K::GemmArgs h1 = {};  // dH1[b][i] = sum_o dH2[b][o] W2[i][o], masked by H1 > 0
+  h1.A = n->dH2; h1.sAm = 256; h1.sAk = 1;
+  h1.B = n->W2r; h1.sBk = 1; h1.sBn = 256;
+  h1.M = (int)rows; h1.N = 256; h1.k_per_split = 256;
+  h1.ldc = 256; h1.Cbf = n->dH1; h1.mask = n->H1; h1.ldm = 256;
+  MI_HIP(ctx, K::gemm(h1, 1, true, true, s));
+  MI_HIP(ctx, K::colsum_bf16(n->dH1, rows, 256, nullptr, n->g + MI_MLP_OFF_B1, s));
+  K::GemmArgs w1 = {};  // dW1[j][i] = sum_b X[b][j] dH1[b][i]
+  w1.A = n->X; w1.sAm = 1; w1.sAk = 128;
+  w1.B = n->dH1; w1.sBk = 256; w1.sBn = 1;
+  w1.M = 128; w1.N = 256; w1.k_per_split = (int)kps;
+  w1.C = n->pW1; w1.split_stride = 32768; w1.ldc = 256;
+  MI_HIP(ctx, K::gemm(w1, (int)splits, true, true, s));
+  float c1 = 1.f, c2 = 1.f;
+  if (update) {
+    n->step++;
+    c1 = 1.f / (1.f - std::pow(n->beta1, (float)n->step));
+    c2 = 1.f / (1.f - std::pow(n->beta2, (float)n->step));
+  }
+  K::AdamArgs a = {n->w, n->m, n->v, n->g, n->pW1, n->pW2, splits, n->W1t, n->W2t, n->W2r,
+                   lr, n->beta1, n->beta2, n->eps, c1, c2, update};
+  MI_HIP(ctx, K::adam(a, s));
+  return MI_OK;
+}
+
+int check_batch(mi_mlp *n, uint32_t count, const void *pos) {
+  if (count == 0) return MI_OK;
+  if (!pos) return fail(n->ctx, MI_ERR_INVALID_ARGUMENT, "null positions");
+  if (round64(count) > n->max_rows)
+    return fail(n->ctx, MI_ERR_INVALID_ARGUMENT, "batch larger than the network's max_batch");
+  return MI_OK;
+}
+
+// Uploads a batch (positions, optional targets) and encodes it; returns the padded row count.
+int stage_batch(mi_mlp *n, const mi_pos *pos, const float *targets, uint32_t count,
+                uint32_t &rows) {
+  mi_ctx *ctx = n->ctx;
+  rows = round64(count);
+  MI_HIP(ctx, hipMemcpyAsync(n->pos, pos, sizeof(mi_pos) * count, hipMemcpyHostToDevice, ctx->stream));
+  if (targets)
+    MI_HIP(ctx, hipMemcpyAsync(n->targets, targets, sizeof(float) * count, hipMemcpyHostToDevice,
+                               ctx->stream));
+  MI_HIP(ctx, K::encode(n->pos, nullptr, count, rows, n->X, ctx->stream));
+  return MI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mi_mlp_create(mi_ctx *ctx, uint32_t max_batch, uint32_t seed, mi_mlp **out) {
+  if (!ctx || !out || max_batch == 0) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  *out = nullptr;
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  mi_mlp *n = new mi_mlp();
+  n->ctx = ctx;
+  n->max_rows = round64(max_batch);
+  const size_t R = n->max_rows, P = MI_MLP_PARAMS;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = dev_alloc(n, &n->w, P);
+  if (e == hipSuccess) e = dev_alloc(n, &n->m, P);
+  if (e == hipSuccess) e = dev_alloc(n, &n->v, P);
+  if (e == hipSuccess) e = dev_alloc(n, &n->g, P);
+  if (e == hipSuccess) e = dev_alloc(n, &n->W1t, (size_t)128 * 256);
+  if (e == hipSuccess) e = dev_alloc(n, &n->W2t, (size_t)256 * 256);
+  if (e == hipSuccess) e = dev_alloc(n, &n->W2r, (size_t)256 * 256);
+  if (e == hipSuccess) e = dev_alloc(n, &n->X, R * 128);
+  if (e == hipSuccess) e = dev_alloc(n, &n->H1, R * 256);
+  if (e == hipSuccess) e = dev_alloc(n, &n->H2, R * 256);
+  if (e == hipSuccess) e = dev_alloc(n, &n->dH2, R * 256);
+  if (e == hipSuccess) e = dev_alloc(n, &n->dH1, R * 256);
+  if (e == hipSuccess) e = dev_alloc(n, &n->y, R);
+  if (e == hipSuccess) e = dev_alloc(n, &n->dz, R);
+  if (e == hipSuccess) e = dev_alloc(n, &n->targets, R);
+  if (e == hipSuccess) e = dev_alloc(n, &n->loss, 1);
+  if (e == hipSuccess) e = dev_alloc(n, &n->pW1, (R / 64) * 32768);
+  if (e == hipSuccess) e = dev_alloc(n, &n->pW2, (R / 64) * 65536);
+  if (e == hipSuccess) e = dev_alloc(n, &n->pos, R);
+  if (e == hipSuccess) e = hipEventCreate(&n->e0);
+  if (e == hipSuccess) e = hipEventCreate(&n->e1);
+  int rc = MI_OK;
+  if (e != hipSuccess) rc = hip_fail(ctx, e, "mi_mlp_create");
+  std::vector<float> w;
+  init_params(seed, w);
+  if (!rc) rc = upload_params(n, w.data());
+  if (rc) {
+    for (void *p : n->allocs) (void)hipFree(p);
+    delete n;
+    return rc;
+  }
+  *out = n;
+  return MI_OK;
+}
+
+int mi_mlp_destroy(mi_mlp *n) {
+  if (!n) return MI_OK;
+  std::lock_guard<std::mutex> lock(n->ctx->mu);
+  (void)hipSetDevice(n->ctx->device);
+  (void)hipStreamSynchronize(n->ctx->stream);
+  for (void *p : n->allocs) (void)hipFree(p);
+  gpuai::DevBuf *bufs[] = {&n->ds_pos, &n->ds_t, &n->ds_perm, &n->ds_loss};
+  for (auto *b : bufs)
+    if (b->ptr) (void)hipFree(b->ptr);
+  if (n->e0) (void)hipEventDestroy(n->e0);
+  if (n->e1) (void)hipEventDestroy(n->e1);
+  delete n;
+  return MI_OK;
+}
+
+int mi_mlp_get_params(mi_mlp *n, float *params) {
+  if (!n || !params) return MI_ERR_INVALID_ARGUMENT;
+  mi_ctx *ctx = n->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  MI_HIP(ctx, hipMemcpyAsync(params, n->w, sizeof(float) * MI_MLP_PARAMS, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MI_OK;
+}
+
+int mi_mlp_set_params(mi_mlp *n, const float *params) {
+  if (!n || !params) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(n->ctx->mu);
+  n->ctx->last_error.clear();
+  return upload_params(n, params);
+}
+
+int mi_mlp_forward(mi_mlp *n, const mi_pos *pos, uint32_t count, float *values) {
+  if (!n) return MI_ERR_INVALID_ARGUMENT;
+  mi_ctx *ctx = n->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  int rc;
+  if ((rc = check_batch(n, count, pos))) return rc;
+  if (count == 0) return MI_OK;
+  if (!values) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "null values");
+  uint32_t rows;
+  if ((rc = stage_batch(n, pos, nullptr, count, rows))) return rc;
+  if ((rc = forward_rows(n, count, rows))) return rc;
+  MI_HIP(ctx, K::head(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, nullptr, nullptr, count,
+                      rows, n->y, nullptr, nullptr, nullptr, ctx->stream));
+  MI_HIP(ctx, hipMemcpyAsync(values, n->y, sizeof(float) * count, hipMemcpyDeviceToHost, ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MI_OK;
+}
+
+static int grad_common(mi_mlp *n, const mi_pos *pos, const float *targets, uint32_t count,
+                       float lr, int update, float *grads, float *loss) {
+  mi_ctx *ctx = n->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  int rc;
+  if ((rc = check_batch(n, count, pos))) return rc;
+  if (count == 0 || !targets) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "empty batch or null targets");
+  uint32_t rows;
+  if ((rc = stage_batch(n, pos, targets, count, rows))) return rc;
+  if ((rc = forward_rows(n, count, rows))) return rc;
+  MI_HIP(ctx, K::zero(n->loss, 1, ctx->stream));
+  MI_HIP(ctx, K::head(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, n->targets, nullptr,
+                      count, rows, n->y, n->dz, n->dH2, n->loss, ctx->stream));
+  if ((rc = backward_rows(n, rows, lr, update))) return rc;
+  if (grads)
+    MI_HIP(ctx, hipMemcpyAsync(grads, n->g, sizeof(float) * MI_MLP_PARAMS, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  if (loss) MI_HIP(ctx, hipMemcpyAsync(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MI_OK;
+}
+
+int mi_mlp_gradients(mi_mlp *n, const mi_pos *pos, const float *targets, uint32_t count,
+                     float *grads, float *loss) {
+  if (!n || !grads) return MI_ERR_INVALID_ARGUMENT;
+  return grad_common(n, pos, targets, count, 0.f, 0, grads, loss);
+}
+
+int mi_mlp_train_step(mi_mlp *n, const mi_pos *pos, const float *targets, uint32_t count,
+                      float lr, float *loss) {
+  if (!n) return MI_ERR_INVALID_ARGUMENT;
+  return grad_common(n, pos, targets, count, lr, 1, nullptr, loss);
+}
+
+int mi_mlp_adam_state(mi_mlp *n, float *m, float *v, uint32_t *step) {
+  if (!n) return MI_ERR_INVALID_ARGUMENT;
+  mi_ctx *ctx = n->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (m) MI_HIP(ctx, hipMemcpyAsync(m, n->m, sizeof(float) * MI_MLP_PARAMS, hipMemcpyDeviceToHost, ctx->stream));
+  if (v) MI_HIP(ctx, hipMemcpyAsync(v, n->v, sizeof(float) * MI_MLP_PARAMS, hipMemcpyDeviceToHost, ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (step) *step = n->step;
+  return MI_OK;
+}
+
+int mi_mlp_set_dataset(mi_mlp *n, const mi_pos *pos, const float *targets, uint32_t count) {
+  if (!n || (count && (!pos || !targets))) return MI_ERR_INVALID_ARGUMENT;
+  mi_ctx *ctx = n->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  int rc;
+  if ((rc = ensure(ctx, n->ds_pos, sizeof(mi_pos) * (size_t)std::max<uint32_t>(count, 1)))) return rc;
+  if ((rc = ensure(ctx, n->ds_t, sizeof(float) * (size_t)std::max<uint32_t>(count, 1)))) return rc;
+  if ((rc = ensure(ctx, n->ds_perm, sizeof(uint32_t) * (size_t)std::max<uint32_t>(count, 1)))) return rc;
+  MI_HIP(ctx, hipMemcpyAsync(n->ds_pos.ptr, pos, sizeof(mi_pos) * count, hipMemcpyHostToDevice, ctx->stream));
+  MI_HIP(ctx, hipMemcpyAsync(n->ds_t.ptr, targets, sizeof(float) * count, hipMemcpyHostToDevice, ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  n->ds_n = count;
+  return MI_OK;
+}
+
+int mi_mlp_train_epoch(mi_mlp *n, uint32_t batch, float lr, uint32_t shuffle_seed, uint32_t epoch,
+                       float *mean_loss, float *ms) {
+  if (!n) return MI_ERR_INVALID_ARGUMENT;
+  mi_ctx *ctx = n->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  if (batch == 0 || batch % 64 || batch > n->max_rows)
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "batch must be a multiple of 64 within max_batch");
+  const uint32_t nb = n->ds_n / batch;
+  if (nb == 0) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "the dataset holds less than one batch");
+  // Fisher-Yates with Philox key (shuffle_seed, epoch), counter (i >> 2, 0, 3, 1)
+  std::vector<uint32_t> perm(n->ds_n);
+  std::iota(perm.begin(), perm.end(), 0u);
+  uint32_t words[4];
+  for (uint32_t i = n->ds_n - 1, c = 0; i > 0; i--, c++) {
+    if ((c & 3) == 0) ck::philox(c >> 2, 0u, ck::TAG_MLP, 1u, shuffle_seed, epoch, words);
+    std::swap(perm[i], perm[words[c & 3] % (i + 1)]);
+  }
+  int rc;
+  if ((rc = ensure(ctx, n->ds_loss, sizeof(float) * nb))) return rc;
+  MI_HIP(ctx, hipMemcpyAsync(n->ds_perm.ptr, perm.data(), sizeof(uint32_t) * n->ds_n,
+                             hipMemcpyHostToDevice, ctx->stream));
+  float *losses = (float *)n->ds_loss.ptr;
+  MI_HIP(ctx, K::zero(losses, nb, ctx->stream));
+  MI_HIP(ctx, hipEventRecord(n->e0, ctx->stream));
+  for (uint32_t b = 0; b < nb; b++) {
+    const uint32_t *idx = (const uint32_t *)n->ds_perm.ptr + (size_t)b * batch;
+    MI_HIP(ctx, K::encode((const mi_pos *)n->ds_pos.ptr, idx, batch, batch, n->X, ctx->stream));
+    if ((rc = forward_rows(n, batch, batch))) return rc;
+    MI_HIP(ctx, K::head(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, (const float *)n->ds_t.ptr,
+                        idx, batch, batch, n->y, n->dz, n->dH2, losses + b, ctx->stream));
+    if ((rc = backward_rows(n, batch, lr, 1))) return rc;
+  }
+  MI_HIP(ctx, hipEventRecord(n->e1, ctx->stream));
+  std::vector<float> l(nb);
+  MI_HIP(ctx, hipMemcpyAsync(l.data(), losses, sizeof(float) * nb, hipMemcpyDeviceToHost, ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (mean_loss) *mean_loss = std::accumulate(l.begin(), l.end(), 0.f) / (float)nb;
+  if (ms) MI_HIP(ctx, hipEventElapsedTime(ms, n->e0, n->e1));
+  return MI_OK;
+}
+
+int mi_mlp_save(mi_mlp *n, const char *path) {
+  if (!n || !path) return MI_ERR_INVALID_ARGUMENT;
+  std::vector<float> w(MI_MLP_PARAMS), m(MI_MLP_PARAMS), v(MI_MLP_PARAMS);
+  uint32_t step = 0;
+  int rc;
+  if ((rc = mi_mlp_get_params(n, w.data()))) return rc;
+  if ((rc = mi_mlp_adam_state(n, m.data(), v.data(), &step))) return rc;
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return fail(n->ctx, MI_ERR_INVALID_ARGUMENT, std::string("cannot write ") + path);
+  const char magic[8] = {'G', 'P', 'U', 'A', 'I', 'M', 'L', 'P'};
+  const uint32_t head[3] = {1u, step, (uint32_t)MI_MLP_PARAMS};
+  bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(head, 4, 3, f) == 3 &&
+            std::fwrite(w.data(), 4, MI_MLP_PARAMS, f) == MI_MLP_PARAMS &&
+            std::fwrite(m.data(), 4, MI_MLP_PARAMS, f) == MI_MLP_PARAMS &&
+            std::fwrite(v.data(), 4, MI_MLP_PARAMS, f) == MI_MLP_PARAMS;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? MI_OK : fail(n->ctx, MI_ERR_INVALID_ARGUMENT, std::string("short write to ") + path);
+}
+
+int mi_mlp_load(mi_mlp *n, const char *path) {
+  if (!n || !path) return MI_ERR_INVALID_ARGUMENT;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return fail(n->ctx, MI_ERR_INVALID_ARGUMENT, std::string("cannot read ") + path);
+  char magic[8];
+  uint32_t head[3];
+  std::vector<float> w(MI_MLP_PARAMS), m(MI_MLP_PARAMS), v(MI_MLP_PARAMS);
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "GPUAIMLP", 8) == 0 &&
+            std::fread(head, 4, 3, f) == 3 && head[0] == 1u && head[2] == MI_MLP_PARAMS &&
+            std::fread(w.data(), 4, MI_MLP_PARAMS, f) == MI_MLP_PARAMS &&
+            std::fread(m.data(), 4, MI_MLP_PARAMS, f) == MI_MLP_PARAMS &&
+            std::fread(v.data(), 4, MI_MLP_PARAMS, f) == MI_MLP_PARAMS;
+  std::fclose(f);
+  if (!ok) return fail(n->ctx, MI_ERR_INVALID_ARGUMENT, std::string("not a checkpoint: ") + path);
+  mi_ctx *ctx = n->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  int rc = upload_params(n, w.data());
+  if (rc) return rc;
+  MI_HIP(ctx, hipMemcpyAsync(n->m, m.data(), sizeof(float) * MI_MLP_PARAMS, hipMemcpyHostToDevice, ctx->stream));
+  MI_HIP(ctx, hipMemcpyAsync(n->v, v.data(), sizeof(float) * MI_MLP_PARAMS, hipMemcpyHostToDevice, ctx->stream));
+  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  n->step = head[1];
+  return MI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// mlp_kernels.h -- launch interface of the value-network kernels (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gpuai/mi_mlp.h"
+#include "gpuai/mi_playout.h"
+
+namespace gpuai {
+namespace mlp {
+
+// C[M][N] (+ epilogue) = A[M][K] B[K][N]; A(m,k) = A[m*sAm + k*sAk], B(k,n) = B[k*sBk + n*sBn]
+// with one of each pair of strides equal to 1.  M and N multiples of 64, k_per_split of 32;
+// split z covers K range [z*k_per_split, (z+1)*k_per_split) and writes C + z*split_stride.
+struct GemmArgs {
+  const void *A;
+  long sAm, sAk;
+  const void *B;
+  long sBk, sBn;
+  int M, N, k_per_split;
+  float *C;               // FP32 output (may be null)
+  long split_stride;
+  int ldc;                // row stride of C and Cbf
+  uint16_t *Cbf;          // BF16 output (may be null)
+  const float *bias;      // + bias[n] (may be null)
+  int relu;               // max(x, 0)
+  const uint16_t *mask;   // keep x where mask[m*ldm + n] > 0 (may be null)
+  int ldm;
+  int row_valid;          // rows m >= row_valid are written as 0 (0: all rows valid)
+};
+
+struct AdamArgs {
+  float *w, *m, *v, *grads;
+  const float *pW1, *pW2;  // split-K partial gradients of W1 ([splits][128*256]) and W2
+  uint32_t splits;
+  uint16_t *W1t, *W2t, *W2r;
+  float lr, beta1, beta2, eps, c1, c2;  // c1 = 1/(1-beta1^t), c2 = 1/(1-beta2^t)
+  int update;              // 0: only sum the gradients and refresh the BF16 copies
+};
+
+hipError_t gemm(const GemmArgs &g, int splits, bool a_bf16, bool b_bf16, hipStream_t stream);
+hipError_t encode(const mi_pos *pos, const uint32_t *idx, uint32_t n, uint32_t rows, uint16_t *X,
+                  hipStream_t stream);
+// targets[tidx ? tidx[row] : row] is the target of a row (targets may be null: forward only)
+hipError_t head(const uint16_t *H2, const float *w3, const float *b3, const float *targets,
+                const uint32_t *tidx, uint32_t n, uint32_t rows, float *y, float *dz,
+                uint16_t *dH2, float *loss, hipStream_t stream);
+hipError_t colsum_bf16(const uint16_t *src, uint32_t rows, uint32_t cols, const float *weight,
+                       float *out, hipStream_t stream);
+hipError_t colsum_f32(const float *src, uint32_t rows, uint32_t cols, const float *weight,
+                      float *out, hipStream_t stream);
+hipError_t adam(const AdamArgs &a, hipStream_t stream);
+hipError_t zero(float *p, uint32_t n, hipStream_t stream);
+
+}  // namespace mlp
+}  // namespace gpuai

@@ -1,0 +1,283 @@
+// mlp_kernels.hip -- gfx950 kernels of the 128-256-256-1 value network.
+//
+// k_gemm: C[M][N] = A[M][K] B[K][N] on v_mfma_f32_16x16x32_bf16.  A 256-thread workgroup owns a
+// 64x64 tile of C; its four waves own 32x32 quadrants (2x2 MFMA tiles of 16x16).  Each 32-deep
+// slice of K is staged through LDS as BF16 with K contiguous (A as [m][k], B as [n][k]), so every
+// lane feeds the MFMA with one 16-byte LDS read per operand: lane l supplies row l%16 of A and
+// column l%16 of B for eight K values of its lane group l/16 (the same eight for A and B, which
+// is all the dot product needs), and receives D[4*(l/16)+v][l%16], v = 0..3.  A and B are read
+// from global memory through arbitrary (row, column) strides, which gives the transposed
+// operands of the backward pass without transposing in memory.  Epilogues fuse bias, ReLU, the
+// ReLU mask of the backward pass, the BF16 store of activations and split-K partial sums.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mlp_kernels.h"
+
+namespace gpuai {
+namespace mlp {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 32, LDSK = 40;  // LDS rows padded to 40 elements
+
+__device__ __forceinline__ uint16_t to_bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t as_bf(uint16_t h) { return h; }
+__device__ __forceinline__ uint16_t as_bf(float f) { return to_bf(f); }
+
+// Stages a 64 x 32 slice of a strided operand X(r, k) = X[r*sr + k*sk] into S[r][k] (BF16).
+template <typename T>
+__device__ __forceinline__ void stage(const T *__restrict__ X, long sr, long sk, int r0, int k0,
+                                      uint16_t (*S)[LDSK]) {
+  const int t = threadIdx.x;
+  if (sk == 1) {  // K contiguous: 8 consecutive k per thread
+    const int r = t >> 2, kk = (t & 3) * 8;
+    const T *p = X + (long)(r0 + r) * sr + (k0 + kk);
+#pragma unroll
+    for (int j = 0; j < 8; j++) S[r][kk + j] = as_bf(p[j]);
+  } else {        // rows contiguous: 8 consecutive rows per thread
+    const int kk = t >> 3, rr = (t & 7) * 8;
+    const T *p = X + (long)(r0 + rr) * sr + (long)(k0 + kk) * sk;
+#pragma unroll
+    for (int j = 0; j < 8; j++) S[rr + j][kk] = as_bf(p[(long)j * sr]);
+  }
+}
+
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM][LDSK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][LDSK];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int kbeg = blockIdx.z * g.k_per_split;
+  const int kend = kbeg + g.k_per_split;
+  const TA *A = (const TA *)g.A;
+  const TB *B = (const TB *)g.B;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    stage(A, g.sAm, g.sAk, m0, k0, As);
+    stage(B, g.sBn, g.sBk, n0, k0, Bs);
+    __syncthreads();
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+      a[r] = *(const bf16x8 *)&As[wr * 32 + r * 16 + (lane & 15)][(lane >> 4) * 8];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+      b[c] = *(const bf16x8 *)&Bs[wc * 32 + c * 16 + (lane & 15)][(lane >> 4) * 8];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b[c], acc[r][c], 0, 0, 0);
+    __syncthreads();
+  }
+  float *C = g.C ? g.C + (long)blockIdx.z * g.split_stride : nullptr;
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int n = n0 + wc * 32 + c * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int m = m0 + wr * 32 + r * 16 + 4 * (lane >> 4) + v;
+        float x = acc[r][c][v];
+        if (g.bias) x += g.bias[n];
+        if (g.relu) x = x > 0.f ? x : 0.f;
+        if (g.mask) x = from_bf(g.mask[(long)m * g.ldm + n]) > 0.f ? x : 0.f;
+        if (g.row_valid && m >= g.row_valid) x = 0.f;
+        if (C) C[(long)m * g.ldc + n] = x;
+        if (g.Cbf) g.Cbf[(long)m * g.ldc + n] = to_bf(x);
+      }
+    }
+  }
+}
+
+// Positions -> BF16 input planes in the mover's frame; rows at or past n (padding) are zero.
+__global__ void __launch_bounds__(256) k_encode(const mi_pos *__restrict__ pos,
+                                               const uint32_t *__restrict__ idx, uint32_t n,
+                                               uint32_t rows, uint16_t *__restrict__ X) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (row, plane)
+  const uint32_t row = t >> 2, plane = t & 3;
+  if (row >= rows) return;
+  uint32_t bits = 0;
+  if (row < n) {
+    const mi_pos p = pos[idx ? idx[row] : row];
+    uint32_t own = p.turn ? __builtin_bitreverse32(p.p2) : p.p1;
+    uint32_t opp = p.turn ? __builtin_bitreverse32(p.p1) : p.p2;
+    uint32_t kings = p.turn ? __builtin_bitreverse32(p.kings) : p.kings;
+    kings &= own | opp;
+    bits = plane == 0 ? (own & ~kings) : plane == 1 ? (own & kings)
+         : plane == 2 ? (opp & ~kings) : (opp & kings);
+  }
+  uint16_t *dst = X + (size_t)row * 128 + plane * 32;
+#pragma unroll
+  for (int s = 0; s < 32; s++) dst[s] = ((bits >> s) & 1u) ? (uint16_t)0x3F80u : (uint16_t)0u;
+}
+
+// Output layer and loss: y = tanh(H2 . w3 + b3) per row (one wave per row, four columns per
+// lane).  With targets: dz = 2 (y - t)(1 - y^2) / n, the squared error into loss (atomic), and
+// dH2 = dz w3 masked by H2 > 0 (BF16).  Rows at or past n get dz = 0.
+__global__ void __launch_bounds__(256) k_head(const uint16_t *__restrict__ H2,
+                                             const float *__restrict__ w3, const float *b3,
+                                             const float *__restrict__ targets,
+                                             const uint32_t *__restrict__ tidx, uint32_t n,
+                                             uint32_t rows, float *__restrict__ y,
+                                             float *__restrict__ dz, uint16_t *__restrict__ dH2,
+                                             float *loss) {
+  const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= rows) return;
+  const uint16_t *h = H2 + (size_t)row * 256 + lane * 4;
+  float hv[4], part = 0.f;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    hv[c] = from_bf(h[c]);
+    part += hv[c] * w3[lane * 4 + c];
+  }
+  for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
+  const float yy = tanhf(part + b3[0]);
+  if (lane == 0 && y && row < n) y[row] = yy;
+  if (!targets) return;
+  float d = 0.f;
+  if (row < n) {
+    const float e = yy - targets[tidx ? tidx[row] : row];
+    d = 2.f * e * (1.f - yy * yy) / (float)n;
+    if (lane == 0) atomicAdd(loss, e * e / (float)n);
+  }
+  if (lane == 0) dz[row] = d;
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+    dH2[(size_t)row * 256 + lane * 4 + c] = to_bf(hv[c] > 0.f ? d * w3[lane * 4 + c] : 0.f);
+}
+
+// out[col] = sum over rows of weight[row] * src[row][col] (weight 1 when null), BF16 or FP32
+// source with `cols` columns; 64 columns per block, four row groups reduced through LDS.
+template <typename T>
+__global__ void __launch_bounds__(256) k_colsum(const T *__restrict__ src, uint32_t rows,
+                                               uint32_t cols, const float *__restrict__ weight,
+                                               float *__restrict__ out) {
+  __shared__ float part[4][64];
+  const int c = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const uint32_t col = blockIdx.x * 64 + c;
+  float s = 0.f;
+  if (col < cols) {
+    for (uint32_t r = grp; r < rows; r += 4) {
+      float v;
+      if constexpr (sizeof(T) == 2) v = from_bf((uint16_t)src[(size_t)r * cols + col]);
+      else v = (float)src[(size_t)r * cols + col];
+      s += weight ? weight[r] * v : v;
+    }
+  }
+  part[grp][c] = s;
+  __syncthreads();
+  if (grp == 0 && col < cols) out[col] = part[0][c] + part[1][c] + part[2][c] + part[3][c];
+}
+
+// Adam over every parameter.  Gradients of W1 and W2 are the sums of their split-K partials,
+// the rest come from `grads`; the summed gradient is written back to `grads` (for inspection).
+// Refreshes the BF16 copies: W1t[o][i], W2t[o][i] (forward B operands) and W2r[i][o] (backward).
+__global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= MI_MLP_PARAMS) return;
+  float g;
+  if (p < MI_MLP_OFF_B1) {
+    g = 0.f;
+    for (uint32_t s = 0; s < a.splits; s++) g += a.pW1[(size_t)s * 32768 + p];
+  } else if (p >= MI_MLP_OFF_W2 && p < MI_MLP_OFF_B2) {
+    g = 0.f;
+    for (uint32_t s = 0; s < a.splits; s++) g += a.pW2[(size_t)s * 65536 + (p - MI_MLP_OFF_W2)];
+  } else {
+    g = a.grads[p];
+  }
+  a.grads[p] = g;
+  float w = a.w[p];
+  if (a.update) {
+    const float m = a.beta1 * a.m[p] + (1.f - a.beta1) * g;
+    const float v = a.beta2 * a.v[p] + (1.f - a.beta2) * g * g;
+    a.m[p] = m;
+    a.v[p] = v;
+    w -= a.lr * (m * a.c1) / (sqrtf(v * a.c2) + a.eps);
+    a.w[p] = w;
+  }
+  if (p < MI_MLP_OFF_B1) {
+    const uint32_t i = p >> 8, o = p & 255;
+    a.W1t[o * 128 + i] = to_bf(w);
+  } else if (p >= MI_MLP_OFF_W2 && p < MI_MLP_OFF_B2) {
+    const uint32_t q = p - MI_MLP_OFF_W2, i = q >> 8, o = q & 255;
+    a.W2t[o * 256 + i] = to_bf(w);
+    a.W2r[q] = to_bf(w);
+  }
+}
+
+__global__ void k_zero(float *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+}  // namespace
+
+hipError_t gemm(const GemmArgs &g, int splits, bool a_bf16, bool b_bf16, hipStream_t stream) {
+  if (g.M % BM || g.N % BN || g.k_per_split % BK) return hipErrorInvalidValue;
+  dim3 grid(g.N / BN, g.M / BM, splits), block(256);
+  if (a_bf16 && b_bf16) hipLaunchKernelGGL((k_gemm<uint16_t, uint16_t>), grid, block, 0, stream, g);
+  else if (a_bf16) hipLaunchKernelGGL((k_gemm<uint16_t, float>), grid, block, 0, stream, g);
+  else if (b_bf16) hipLaunchKernelGGL((k_gemm<float, uint16_t>), grid, block, 0, stream, g);
+  else hipLaunchKernelGGL((k_gemm<float, float>), grid, block, 0, stream, g);
+  return hipGetLastError();
+}
+
+hipError_t encode(const mi_pos *pos, const uint32_t *idx, uint32_t n, uint32_t rows, uint16_t *X,
+                  hipStream_t stream) {
+  const uint32_t threads = rows * 4;
+  hipLaunchKernelGGL(k_encode, dim3((threads + 255) / 256), dim3(256), 0, stream, pos, idx, n, rows, X);
+  return hipGetLastError();
+}
+
+hipError_t head(const uint16_t *H2, const float *w3, const float *b3, const float *targets,
+                const uint32_t *tidx, uint32_t n, uint32_t rows, float *y, float *dz,
+                uint16_t *dH2, float *loss, hipStream_t stream) {
+  hipLaunchKernelGGL(k_head, dim3((rows * 64 + 255) / 256), dim3(256), 0, stream, H2, w3, b3,
+                     targets, tidx, n, rows, y, dz, dH2, loss);
+  return hipGetLastError();
+}
+
+hipError_t colsum_bf16(const uint16_t *src, uint32_t rows, uint32_t cols, const float *weight,
+                       float *out, hipStream_t stream) {
+  hipLaunchKernelGGL(k_colsum<uint16_t>, dim3((cols + 63) / 64), dim3(256), 0, stream, src, rows,
+                     cols, weight, out);
+  return hipGetLastError();
+}
+
+hipError_t colsum_f32(const float *src, uint32_t rows, uint32_t cols, const float *weight,
+                      float *out, hipStream_t stream) {
+  hipLaunchKernelGGL(k_colsum<float>, dim3((cols + 63) / 64), dim3(256), 0, stream, src, rows,
+                     cols, weight, out);
+  return hipGetLastError();
+}
+
+hipError_t adam(const AdamArgs &a, hipStream_t stream) {
+  hipLaunchKernelGGL(k_adam, dim3((MI_MLP_PARAMS + 255) / 256), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t zero(float *p, uint32_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(k_zero, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace mlp
+}  // namespace gpuai
