@@ -1,13 +1,15 @@
 """Throughput benchmark of the MI355X playout path (one JSON line on stdout).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c0] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
 
 A step is one launch of the persistent playout kernel over the staged batch of the workload:
-  c1 (default): 1024 seeded positions (k in [0, 30] random plies from the start) x 1024 random
-                playouts = 1,048,576 bit-exact playouts
-  c2          : 1024 seeded midgame positions (k in [20, 60], finished games redrawn) x 1024
-                heuristic playouts
-  c0          : 64 positions x 4096 random playouts
+  random (default): 1024 seeded positions (k in [0, 30] random plies from the start) x 1024
+                    random playouts = 1,048,576 bit-exact playouts
+  heuristic       : 1024 seeded midgame positions (k in [20, 60], finished games redrawn) x 1024
+                    heuristic playouts
+  small           : 64 positions x 4096 random playouts (the host path's own size)
+  multisearch     : 256 concurrent MCTS searches sharing the device (one line, its own fields)
+  value           : value-network training and evaluation (one line, its own fields)
 Positions are generated on the device (mi_gen_start_positions) and staged in HBM once; each step
 uses its own epoch (a fresh Philox stream), so no step repeats another.
 
@@ -35,22 +37,22 @@ sys.path.insert(0, ROOT)
 METRIC = "playouts/s on 1 MI355X; win rate of net-leaf vs playout MCTS at equal time"
 W_PLY = 192                      # int32 lane-ops per ply (frozen cost model)
 SPEC_CUS, SPEC_CLOCK_HZ = 256, 2.4e9
-CONFIGS = {
-    "c1": dict(n=1024, per_pos=1024, kmin=0, kmax=30, reject=False, policy=0,
-               desc="config 1: 1024 seeded positions (k in [0,30] random plies) x 1024 bit-exact "
-                    "random playouts"),
-    "c2": dict(n=1024, per_pos=1024, kmin=20, kmax=60, reject=True, policy=1,
-               desc="config 2: 1024 seeded midgame positions (k in [20,60], finished games "
-                    "redrawn) x 1024 heuristic playouts"),
-    "c0": dict(n=64, per_pos=4096, kmin=0, kmax=30, reject=False, policy=0,
-               desc="config 0: 64 seeded positions x 4096 random playouts"),
+WORKLOADS = {
+    "random": dict(n=1024, per_pos=1024, kmin=0, kmax=30, reject=False, policy=0,
+                   desc="random-1024x1024: 1024 seeded positions (k in [0,30] random plies) x "
+                        "1024 bit-exact random playouts"),
+    "heuristic": dict(n=1024, per_pos=1024, kmin=20, kmax=60, reject=True, policy=1,
+                      desc="heuristic-1024x1024: 1024 seeded midgame positions (k in [20,60], "
+                           "finished games redrawn) x 1024 heuristic playouts"),
+    "small": dict(n=64, per_pos=4096, kmin=0, kmax=30, reject=False, policy=0,
+                  desc="random-64x4096: 64 seeded positions x 4096 random playouts"),
 }
 POS_SEED, PLAY_SEED = 2016, 12345
 
 
-def read_traffic(config: str):
+def read_traffic(workload: str):
     """HBM bytes per launch recorded from rocprofv3 --pmc runs (profiles/*traffic*.md)."""
-    path = os.path.join(ROOT, "profiles", f"playout_{config}_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"k_playouts_{workload}_traffic.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
@@ -78,7 +80,7 @@ def cpu_baseline(cfg: dict, pos, threads: int) -> dict:
 
 
 def bench_multisearch() -> int:
-    """Configuration 3: 256 concurrent searches, budgets uniform in [1000, 64000] playouts,
+    """Batched multi-search: 256 concurrent searches, budgets uniform in [1000, 64000] playouts,
     UCT c = sqrt(2).  One graph replay (persistent refill kernel) per iteration versus one
     playout launch per search per iteration."""
     from gpu_ai_amd import host_api
@@ -88,7 +90,7 @@ def bench_multisearch() -> int:
         "metric": "aggregate playouts/s of 256 concurrent MCTS searches on 1 MI355X",
         "value": round(g["playouts_per_second"], 1), "unit": "playouts/s", "n_gpus": 1,
         "higher_is_better": True, "dtype": "int32", "data": "synthetic (seeded positions)",
-        "config": {"workload": "config 3: 256 searches from seeded distinct positions, budgets "
+        "config": {"workload": "multisearch-256: 256 searches from seeded distinct positions, budgets "
                                "uniform in [1000, 64000] playouts, 256 trials per search "
                                "per iteration, UCT c=sqrt(2)"},
         "graph": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in g.items()},
@@ -104,7 +106,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS) + ["c3"])
+    ap.add_argument("--workload", default="random",
+                    choices=sorted(WORKLOADS) + ["multisearch", "value"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shape instead of tuning on the batch")
@@ -117,12 +120,14 @@ def main() -> int:
               "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
         return 2
 
-    if args.config == "c3":
+    if args.workload == "multisearch":
         return bench_multisearch()
+    if args.workload == "value":
+        return bench_value(args)
 
     from gpu_ai_amd.engine import Engine
 
-    cfg = CONFIGS[args.config]
+    cfg = WORKLOADS[args.workload]
     eng = Engine(0)
     info = eng.device_info()
     pos = eng.start_positions(POS_SEED, cfg["n"], cfg["kmin"], cfg["kmax"], cfg["reject"])
@@ -152,7 +157,7 @@ def main() -> int:
     peak_dev = info["compute_units"] * 64 * clock_hz
     peak_spec = SPEC_CUS * 64 * SPEC_CLOCK_HZ
     achieved = plies_per_launch * W_PLY / avg_kernel_s
-    traffic = read_traffic(args.config)
+    traffic = read_traffic(args.workload)
     line = {
         "metric": METRIC,
         "value": round(value, 1),

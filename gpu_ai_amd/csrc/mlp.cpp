@@ -25,7 +25,7 @@ struct mi_mlp {
   mi_ctx *ctx = nullptr;
   uint32_t max_rows = 0;
   uint32_t step = 0;
-  float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+  double beta1 = 0.9, beta2 = 0.999, eps = 1e-8;  // rounded to float where a kernel takes them
   // parameters, Adam moments, gradients (FP32) and BF16 operand copies
   float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
   uint16_t *W1t = nullptr, *W2t = nullptr, *W2r = nullptr;
@@ -33,6 +33,7 @@ struct mi_mlp {
   uint16_t *X = nullptr, *H1 = nullptr, *H2 = nullptr, *dH2 = nullptr, *dH1 = nullptr;
   float *y = nullptr, *dz = nullptr, *targets = nullptr, *loss = nullptr;
   float *pW1 = nullptr, *pW2 = nullptr;  // split-K partials of dW1, dW2
+  float *pV = nullptr;                    // column-sum partials of b1, b2, w3, b3
   mi_pos *pos = nullptr;
   // device-resident training set
   DevBuf ds_pos, ds_t, ds_perm, ds_loss;
@@ -45,11 +46,35 @@ namespace {
 
 uint32_t round64(uint32_t n) { return (n + 63u) & ~63u; }
 
-// Largest split length in {512, 256, 128, 64} that divides rows.
-uint32_t split_len(uint32_t rows) {
-  for (uint32_t k = 512; k >= 64; k >>= 1)
-    if (rows % k == 0) return k;
-  return 64;
+// Rows per split of the weight-gradient GEMMs (K = rows): 128, or 64 when rows is an odd
+// multiple of 64, so a 4096-row batch gives 32 splits (512 and 256 workgroups for dW2, dW1).
+uint32_t split_len(uint32_t rows) { return rows % 128 == 0 ? 128 : 64; }
+
+K::AdamArgs adam_args(mi_mlp *n, uint32_t splits, uint32_t vsplits, float lr, float c1, float c2,
+                      int update) {
+  K::AdamArgs a = {};
+  a.w = n->w;
+  a.m = n->m;
+  a.v = n->v;
+  a.grads = n->g;
+  a.pW1 = n->pW1;
+  a.pW2 = n->pW2;
+  a.splits = splits;
+  a.pV = n->pV;
+  a.vsplits = vsplits;
+  a.W1t = n->W1t;
+  a.W2t = n->W2t;
+  a.W2r = n->W2r;
+  a.lr = lr;
+  a.beta1 = (float)n->beta1;
+  a.beta2 = (float)n->beta2;
+  a.eps = (float)n->eps;
+  a.c1 = c1;
+  a.c2 = c2;
+  a.omb1 = (float)(1.0 - n->beta1);  // formed in double precision, as Adam's usual
+  a.omb2 = (float)(1.0 - n->beta2);  // implementations do
+  a.update = update;
+  return a;
 }
 
 template <class T>
@@ -86,10 +111,8 @@ int upload_params(mi_mlp *n, const float *w) {
   MI_HIP(ctx, K::zero(n->m, MI_MLP_PARAMS, ctx->stream));
   MI_HIP(ctx, K::zero(n->v, MI_MLP_PARAMS, ctx->stream));
   MI_HIP(ctx, K::zero(n->g, MI_MLP_PARAMS, ctx->stream));
-  // refresh the BF16 copies without updating (splits = 0: W gradients read as 0)
-  K::AdamArgs a = {n->w, n->m, n->v, n->g, n->pW1, n->pW2, 0u, n->W1t, n->W2t, n->W2r,
-                   0.f, n->beta1, n->beta2, n->eps, 1.f, 1.f, 0};
-  MI_HIP(ctx, K::adam(a, ctx->stream));
+  // refresh the BF16 copies without updating (no partials: every gradient reads as 0)
+  MI_HIP(ctx, K::adam(adam_args(n, 0, 0, 0.f, 1.f, 1.f, 0), ctx->stream));
   MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
   n->step = 0;
   return MI_OK;
@@ -119,9 +142,9 @@ int forward_rows(mi_mlp *n, uint32_t valid, uint32_t rows) {
 int backward_rows(mi_mlp *n, uint32_t rows, float lr, int update) {
   mi_ctx *ctx = n->ctx;
   hipStream_t s = ctx->stream;
-  MI_HIP(ctx, K::colsum_bf16(n->H2, rows, 256, n->dz, n->g + MI_MLP_OFF_W3, s));
-  MI_HIP(ctx, K::colsum_f32(n->dz, rows, 1, nullptr, n->g + MI_MLP_OFF_B3, s));
-  MI_HIP(ctx, K::colsum_bf16(n->dH2, rows, 256, nullptr, n->g + MI_MLP_OFF_B2, s));
+  MI_HIP(ctx, K::colsum_bf16(n->H2, rows, 256, n->dz, n->pV + 512, s));  // w3
+  MI_HIP(ctx, K::colsum_f32(n->dz, rows, 1, nullptr, n->pV + 768, s));    // b3
+  MI_HIP(ctx, K::colsum_bf16(n->dH2, rows, 256, nullptr, n->pV + 256, s)); // b2
   const uint32_t kps = split_len(rows), splits = rows / kps;
   K::GemmArgs w2 = {};  // dW2[i][o] = sum_b H1[b][i] dH2[b][o]
   w2.A = n->H1; w2.sAm = 1; w2.sAk = 256;
@@ -135,22 +158,21 @@ int backward_rows(mi_mlp *n, uint32_t rows, float lr, int update) {
   h1.M = (int)rows; h1.N = 256; h1.k_per_split = 256;
   h1.ldc = 256; h1.Cbf = n->dH1; h1.mask = n->H1; h1.ldm = 256;
   MI_HIP(ctx, K::gemm(h1, 1, true, true, s));
-  MI_HIP(ctx, K::colsum_bf16(n->dH1, rows, 256, nullptr, n->g + MI_MLP_OFF_B1, s));
+  MI_HIP(ctx, K::colsum_bf16(n->dH1, rows, 256, nullptr, n->pV, s));       // b1
   K::GemmArgs w1 = {};  // dW1[j][i] = sum_b X[b][j] dH1[b][i]
   w1.A = n->X; w1.sAm = 1; w1.sAk = 128;
   w1.B = n->dH1; w1.sBk = 256; w1.sBn = 1;
   w1.M = 128; w1.N = 256; w1.k_per_split = (int)kps;
   w1.C = n->pW1; w1.split_stride = 32768; w1.ldc = 256;
   MI_HIP(ctx, K::gemm(w1, (int)splits, true, true, s));
+  // the bias corrections are formed in double precision, as Adam's usual implementations do
   float c1 = 1.f, c2 = 1.f;
   if (update) {
     n->step++;
-    c1 = 1.f / (1.f - std::pow(n->beta1, (float)n->step));
-    c2 = 1.f / (1.f - std::pow(n->beta2, (float)n->step));
+    c1 = (float)(1.0 / (1.0 - std::pow(n->beta1, (double)n->step)));
+    c2 = (float)(1.0 / (1.0 - std::pow(n->beta2, (double)n->step)));
   }
-  K::AdamArgs a = {n->w, n->m, n->v, n->g, n->pW1, n->pW2, splits, n->W1t, n->W2t, n->W2r,
-                   lr, n->beta1, n->beta2, n->eps, c1, c2, update};
-  MI_HIP(ctx, K::adam(a, s));
+  MI_HIP(ctx, K::adam(adam_args(n, splits, rows / 64, lr, c1, c2, update), s));
   return MI_OK;
 }
 
@@ -208,6 +230,7 @@ int mi_mlp_create(mi_ctx *ctx, uint32_t max_batch, uint32_t seed, mi_mlp **out) 
   if (e == hipSuccess) e = dev_alloc(n, &n->loss, 1);
   if (e == hipSuccess) e = dev_alloc(n, &n->pW1, (R / 64) * 32768);
   if (e == hipSuccess) e = dev_alloc(n, &n->pW2, (R / 64) * 65536);
+  if (e == hipSuccess) e = dev_alloc(n, &n->pV, (R / 64) * K::kVecSlots);
   if (e == hipSuccess) e = dev_alloc(n, &n->pos, R);
   if (e == hipSuccess) e = hipEventCreate(&n->e0);
   if (e == hipSuccess) e = hipEventCreate(&n->e1);

@@ -30,12 +30,18 @@ struct GemmArgs {
   int row_valid;          // rows m >= row_valid are written as 0 (0: all rows valid)
 };
 
+// Column-sum partial slots per 64-row block: b1 at 0-255, b2 256-511, w3 512-767, b3 768.
+constexpr uint32_t kVecSlots = 1024;
+
 struct AdamArgs {
   float *w, *m, *v, *grads;
   const float *pW1, *pW2;  // split-K partial gradients of W1 ([splits][128*256]) and W2
   uint32_t splits;
+  const float *pV;         // column-sum partials ([vsplits][kVecSlots])
+  uint32_t vsplits;
   uint16_t *W1t, *W2t, *W2r;
   float lr, beta1, beta2, eps, c1, c2;  // c1 = 1/(1-beta1^t), c2 = 1/(1-beta2^t)
+  float omb1, omb2;        // 1 - beta1, 1 - beta2 (formed in double precision on the host)
   int update;              // 0: only sum the gradients and refresh the BF16 copies
 };
 
@@ -46,10 +52,11 @@ hipError_t encode(const mi_pos *pos, const uint32_t *idx, uint32_t n, uint32_t r
 hipError_t head(const uint16_t *H2, const float *w3, const float *b3, const float *targets,
                 const uint32_t *tidx, uint32_t n, uint32_t rows, float *y, float *dz,
                 uint16_t *dH2, float *loss, hipStream_t stream);
+// Partial column sums of 64-row blocks into part[block * kVecSlots + col].
 hipError_t colsum_bf16(const uint16_t *src, uint32_t rows, uint32_t cols, const float *weight,
-                       float *out, hipStream_t stream);
+                       float *part, hipStream_t stream);
 hipError_t colsum_f32(const float *src, uint32_t rows, uint32_t cols, const float *weight,
-                      float *out, hipStream_t stream);
+                      float *part, hipStream_t stream);
 hipError_t adam(const AdamArgs &a, hipStream_t stream);
 hipError_t zero(float *p, uint32_t n, hipStream_t stream);
 

@@ -164,50 +164,58 @@ __global__ void __launch_bounds__(256) k_head(const uint16_t *__restrict__ H2,
     dH2[(size_t)row * 256 + lane * 4 + c] = to_bf(hv[c] > 0.f ? d * w3[lane * 4 + c] : 0.f);
 }
 
-// out[col] = sum over rows of weight[row] * src[row][col] (weight 1 when null), BF16 or FP32
-// source with `cols` columns; 64 columns per block, four row groups reduced through LDS.
+// Partial column sums: block (x, y) writes part[y * part_stride + col] = sum over rows
+// [64 y, 64 y + 64) of weight[row] * src[row][col] (weight 1 when null), for 64 columns of a BF16
+// or FP32 source with `cols` columns.  The partials are summed in a fixed order by k_adam, so
+// the reduction is deterministic.
 template <typename T>
 __global__ void __launch_bounds__(256) k_colsum(const T *__restrict__ src, uint32_t rows,
                                                uint32_t cols, const float *__restrict__ weight,
-                                               float *__restrict__ out) {
-  __shared__ float part[4][64];
+                                               float *__restrict__ part, uint32_t part_stride) {
+  __shared__ float red[4][64];
   const int c = threadIdx.x & 63, grp = threadIdx.x >> 6;
   const uint32_t col = blockIdx.x * 64 + c;
+  const uint32_t r0 = blockIdx.y * 64, r1 = min(r0 + 64u, rows);
   float s = 0.f;
   if (col < cols) {
-    for (uint32_t r = grp; r < rows; r += 4) {
+#pragma unroll 4
+    for (uint32_t r = r0 + grp; r < r1; r += 4) {
       float v;
       if constexpr (sizeof(T) == 2) v = from_bf((uint16_t)src[(size_t)r * cols + col]);
       else v = (float)src[(size_t)r * cols + col];
       s += weight ? weight[r] * v : v;
     }
   }
-  part[grp][c] = s;
+  red[grp][c] = s;
   __syncthreads();
-  if (grp == 0 && col < cols) out[col] = part[0][c] + part[1][c] + part[2][c] + part[3][c];
+  if (grp == 0 && col < cols)
+    part[(size_t)blockIdx.y * part_stride + col] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
 }
 
-// Adam over every parameter.  Gradients of W1 and W2 are the sums of their split-K partials,
-// the rest come from `grads`; the summed gradient is written back to `grads` (for inspection).
-// Refreshes the BF16 copies: W1t[o][i], W2t[o][i] (forward B operands) and W2r[i][o] (backward).
+// Adam over every parameter.  Gradients are sums of partials in a fixed order: W1 and W2 over
+// their split-K partials, b1, b2, w3 and b3 over the column-sum partials (slots 0-255 b1, then
+// b2, w3, b3 at 256-768, in parameter order).  The summed gradient is written to `grads` (for
+// inspection).  Refreshes the BF16 copies: W1t[o][i], W2t[o][i] (forward B operands) and
+// W2r[i][o] (backward).
 __global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= MI_MLP_PARAMS) return;
-  float g;
+  float g = 0.f;
   if (p < MI_MLP_OFF_B1) {
-    g = 0.f;
     for (uint32_t s = 0; s < a.splits; s++) g += a.pW1[(size_t)s * 32768 + p];
-  } else if (p >= MI_MLP_OFF_W2 && p < MI_MLP_OFF_B2) {
-    g = 0.f;
+  } else if (p < MI_MLP_OFF_W2) {
+    for (uint32_t s = 0; s < a.vsplits; s++) g += a.pV[(size_t)s * kVecSlots + (p - MI_MLP_OFF_B1)];
+  } else if (p < MI_MLP_OFF_B2) {
     for (uint32_t s = 0; s < a.splits; s++) g += a.pW2[(size_t)s * 65536 + (p - MI_MLP_OFF_W2)];
   } else {
-    g = a.grads[p];
+    for (uint32_t s = 0; s < a.vsplits; s++)
+      g += a.pV[(size_t)s * kVecSlots + 256 + (p - MI_MLP_OFF_B2)];
   }
   a.grads[p] = g;
   float w = a.w[p];
   if (a.update) {
-    const float m = a.beta1 * a.m[p] + (1.f - a.beta1) * g;
-    const float v = a.beta2 * a.v[p] + (1.f - a.beta2) * g * g;
+    const float m = a.beta1 * a.m[p] + a.omb1 * g;
+    const float v = a.beta2 * a.v[p] + a.omb2 * g * g;
     a.m[p] = m;
     a.v[p] = v;
     w -= a.lr * (m * a.c1) / (sqrtf(v * a.c2) + a.eps);
@@ -256,16 +264,16 @@ hipError_t head(const uint16_t *H2, const float *w3, const float *b3, const floa
 }
 
 hipError_t colsum_bf16(const uint16_t *src, uint32_t rows, uint32_t cols, const float *weight,
-                       float *out, hipStream_t stream) {
-  hipLaunchKernelGGL(k_colsum<uint16_t>, dim3((cols + 63) / 64), dim3(256), 0, stream, src, rows,
-                     cols, weight, out);
+                       float *part, hipStream_t stream) {
+  hipLaunchKernelGGL(k_colsum<uint16_t>, dim3((cols + 63) / 64, (rows + 63) / 64), dim3(256), 0,
+                     stream, src, rows, cols, weight, part, (uint32_t)kVecSlots);
   return hipGetLastError();
 }
 
 hipError_t colsum_f32(const float *src, uint32_t rows, uint32_t cols, const float *weight,
-                      float *out, hipStream_t stream) {
-  hipLaunchKernelGGL(k_colsum<float>, dim3((cols + 63) / 64), dim3(256), 0, stream, src, rows,
-                     cols, weight, out);
+                      float *part, hipStream_t stream) {
+  hipLaunchKernelGGL(k_colsum<float>, dim3((cols + 63) / 64, (rows + 63) / 64), dim3(256), 0,
+                     stream, src, rows, cols, weight, part, (uint32_t)kVecSlots);
   return hipGetLastError();
 }
 

@@ -1,4 +1,4 @@
-"""ctypes bindings of the C ABI in include/gpuai/mi_playout.h.
+"""ctypes bindings of the C ABI in include/gpuai/mi_playout.h and include/gpuai/mi_mlp.h.
 
 The library must have been built (python -m gpu_ai_amd.build); importing this module on a box
 without it raises at load time, so a missing extension never degrades into a silent fallback.
@@ -88,7 +88,21 @@ _SIGS = {
     "mi_graph_destroy": ([_P], ctypes.c_int),
     "mi_copy_calibration": ([_P, ctypes.c_uint64, _U32, ctypes.POINTER(ctypes.c_float)],
                             ctypes.c_int),
+    "mi_mlp_create": ([_P, _U32, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+    "mi_mlp_destroy": ([_P], ctypes.c_int),
+    "mi_mlp_get_params": ([_P, _P], ctypes.c_int),
+    "mi_mlp_set_params": ([_P, _P], ctypes.c_int),
+    "mi_mlp_forward": ([_P, _P, _U32, _P], ctypes.c_int),
+    "mi_mlp_gradients": ([_P, _P, _P, _U32, _P, _P], ctypes.c_int),
+    "mi_mlp_train_step": ([_P, _P, _P, _U32, ctypes.c_float, _P], ctypes.c_int),
+    "mi_mlp_adam_state": ([_P, _P, _P, _P], ctypes.c_int),
+    "mi_mlp_set_dataset": ([_P, _P, _P, _U32], ctypes.c_int),
+    "mi_mlp_train_epoch": ([_P, _U32, ctypes.c_float, _U32, _U32, _P, _P], ctypes.c_int),
+    "mi_mlp_save": ([_P, ctypes.c_char_p], ctypes.c_int),
+    "mi_mlp_load": ([_P, ctypes.c_char_p], ctypes.c_int),
 }
+
+MLP_PARAMS = 99073
 
 _lib = None
 
@@ -322,3 +336,96 @@ class GraphBatch:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class ValueNet:
+    """The 128-256-256-1 value network of one device context (include/gpuai/mi_mlp.h)."""
+
+    def __init__(self, engine: Engine, max_batch: int = 4096, seed: int = 1):
+        self.engine = engine
+        self.lib = engine.lib
+        self.handle = ctypes.c_void_p()
+        engine._check(self.lib.mi_mlp_create(engine.ctx, max_batch, seed,
+                                             ctypes.byref(self.handle)))
+        self.max_batch = max_batch
+
+    def _check(self, rc: int) -> None:
+        self.engine._check(rc)
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.mi_mlp_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        w = np.empty(MLP_PARAMS, dtype=np.float32)
+        self._check(self.lib.mi_mlp_get_params(self.handle, _ptr(w)))
+        return w
+
+    def set_params(self, w) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.shape != (MLP_PARAMS,):
+            raise ValueError(f"expected {MLP_PARAMS} parameters")
+        self._check(self.lib.mi_mlp_set_params(self.handle, _ptr(w)))
+
+    def forward(self, pos) -> np.ndarray:
+        pos = as_positions(pos)
+        out = np.empty(len(pos), dtype=np.float32)
+        self._check(self.lib.mi_mlp_forward(self.handle, _ptr(pos), len(pos), _ptr(out)))
+        return out
+
+    def gradients(self, pos, targets) -> tuple[np.ndarray, float]:
+        pos = as_positions(pos)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(self.lib.mi_mlp_gradients(self.handle, _ptr(pos), _ptr(t), len(pos), _ptr(g),
+                                              ctypes.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, pos, targets, lr: float = 1e-3) -> float:
+        pos = as_positions(pos)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(self.lib.mi_mlp_train_step(self.handle, _ptr(pos), _ptr(t), len(pos), lr,
+                                               ctypes.byref(loss)))
+        return loss.value
+
+    def adam_state(self) -> tuple[np.ndarray, np.ndarray, int]:
+        m = np.empty(MLP_PARAMS, dtype=np.float32)
+        v = np.empty(MLP_PARAMS, dtype=np.float32)
+        step = _U32()
+        self._check(self.lib.mi_mlp_adam_state(self.handle, _ptr(m), _ptr(v), ctypes.byref(step)))
+        return m, v, step.value
+
+    def set_dataset(self, pos, targets) -> None:
+        pos = as_positions(pos)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        if t.shape != (len(pos),):
+            raise ValueError("one target per position")
+        self._check(self.lib.mi_mlp_set_dataset(self.handle, _ptr(pos), _ptr(t), len(pos)))
+
+    def train_epoch(self, batch: int = 4096, lr: float = 1e-3, shuffle_seed: int = 7,
+                    epoch: int = 0) -> tuple[float, float]:
+        loss, ms = ctypes.c_float(), ctypes.c_float()
+        self._check(self.lib.mi_mlp_train_epoch(self.handle, batch, lr, shuffle_seed, epoch,
+                                                ctypes.byref(loss), ctypes.byref(ms)))
+        return loss.value, ms.value
+
+    def save(self, path: str) -> None:
+        self._check(self.lib.mi_mlp_save(self.handle, os.fsencode(path)))
+
+    def load(self, path: str) -> None:
+        self._check(self.lib.mi_mlp_load(self.handle, os.fsencode(path)))

@@ -1,4 +1,5 @@
 // capi_host.cpp -- C entry points of the host layer (include/gpuai/gpuai_host.h).
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <stdexcept>
@@ -6,6 +7,7 @@
 #include "gpuai/gpuai_host.h"
 #include "mcts.hpp"
 #include "multisearch.hpp"
+#include "netmatch.hpp"
 #include "state.hpp"
 
 namespace {
@@ -132,6 +134,46 @@ int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch
     out->max_ms = r.maxMs;
     out->playouts_per_second = r.playoutsPerSecond;
     out->root_digest = r.rootDigest;
+    return 0;
+  } catch (const std::exception &e) {
+    if (err && err_len) {
+      std::strncpy(err, e.what(), err_len - 1);
+      err[err_len - 1] = 0;
+    }
+    return -1;
+  }
+}
+
+int gpuai_net_match_run(const gpuai_net_match_config *cfg, const float *params,
+                        gpuai_net_match_result *out, int8_t *outcome, char *err, uint32_t err_len) {
+  try {
+    if (!cfg || !params || !out) throw std::runtime_error("null argument");
+    NetMatchConfig c;
+    c.games = cfg->games;
+    c.seed = cfg->seed;
+    c.openingMin = cfg->opening_min;
+    c.openingMax = cfg->opening_max;
+    c.moveMs = cfg->move_ms;
+    c.leafBatch = cfg->leaf_batch;
+    c.maxPlies = cfg->max_plies;
+    c.playoutSeed = cfg->playout_seed;
+    c.netBatch = cfg->net_batch;
+    if (c.openingMax < c.openingMin || c.leafBatch == 0 || c.netBatch == 0 || !(c.moveMs >= 0))
+      throw std::runtime_error("invalid match configuration");
+    NetMatchResult r = runNetMatch(c, params);
+    out->games = r.games;
+    out->net_wins = r.netWins;
+    out->playout_wins = r.playoutWins;
+    out->draws = r.draws;
+    out->score = r.score;
+    out->score_stderr = r.scoreStdErr;
+    out->net_leaves = r.netLeaves;
+    out->playout_leaves = r.playoutLeaves;
+    out->net_iterations = r.netIterations;
+    out->playout_iterations = r.playoutIterations;
+    out->plies = r.plies;
+    out->seconds = r.seconds;
+    if (outcome) std::copy(r.outcome.begin(), r.outcome.end(), outcome);
     return 0;
   } catch (const std::exception &e) {
     if (err && err_len) {

@@ -80,7 +80,7 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
   std::vector<uint32_t> counts;
   std::vector<int8_t> winners;
   const auto wall0 = Clock::now();
-  for (uint32_t it = 0;; it++) {
+  for (;;) {
     const auto t0 = Clock::now();
     bool any = false;
     for (uint32_t s = 0; s < S; s++) any |= remaining[s] > 0;

@@ -30,6 +30,20 @@ class MultiSearchResult(ctypes.Structure):
                 ("root_digest", ctypes.c_uint64)]
 
 
+class NetMatchConfig(ctypes.Structure):
+    _fields_ = [("games", _U32), ("seed", _U32), ("opening_min", _U32), ("opening_max", _U32),
+                ("move_ms", ctypes.c_double), ("leaf_batch", _U32), ("max_plies", _U32),
+                ("playout_seed", _U32), ("net_batch", _U32)]
+
+
+class NetMatchResult(ctypes.Structure):
+    _fields_ = [("games", _U32), ("net_wins", _U32), ("playout_wins", _U32), ("draws", _U32),
+                ("score", ctypes.c_double), ("score_stderr", ctypes.c_double),
+                ("net_leaves", ctypes.c_uint64), ("playout_leaves", ctypes.c_uint64),
+                ("net_iterations", ctypes.c_uint64), ("playout_iterations", ctypes.c_uint64),
+                ("plies", ctypes.c_uint64), ("seconds", ctypes.c_double)]
+
+
 def host_library() -> ctypes.CDLL:
     global _lib
     if _lib is None:
@@ -47,6 +61,10 @@ def host_library() -> ctypes.CDLL:
                                               ctypes.POINTER(MultiSearchResult), ctypes.c_char_p,
                                               _U32]
         lib.gpuai_multisearch_run.restype = ctypes.c_int
+        lib.gpuai_net_match_run.argtypes = [ctypes.POINTER(NetMatchConfig), _P,
+                                            ctypes.POINTER(NetMatchResult), _P, ctypes.c_char_p,
+                                            _U32]
+        lib.gpuai_net_match_run.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -88,7 +106,8 @@ def run_ai_path() -> str:
 def multisearch(searches: int = 256, seed: int = 2016, playout_seed: int = 12345,
                 min_budget: int = 1000, max_budget: int = 64000, initial: int = 256,
                 scale: float = 0.0, policy: int = 0, mode: str = "graph") -> dict:
-    """Configuration 3: concurrent searches sharing the device; returns the run's statistics."""
+    """Batched multi-search: concurrent searches sharing the device; returns the run's
+    statistics."""
     cfg = MultiSearchConfig(searches, seed, playout_seed, min_budget, max_budget, initial, scale,
                             policy, MODES[mode])
     res = MultiSearchResult()
@@ -96,3 +115,22 @@ def multisearch(searches: int = 256, seed: int = 2016, playout_seed: int = 12345
     if host_library().gpuai_multisearch_run(ctypes.byref(cfg), ctypes.byref(res), err, 512) != 0:
         raise RuntimeError(err.value.decode())
     return {k: getattr(res, k) for k, _ in MultiSearchResult._fields_ if k != "pad"}
+
+
+def net_match(params, games: int = 400, seed: int = 2016, opening_min: int = 0,
+              opening_max: int = 6, move_ms: float = 100.0, leaf_batch: int = 4096,
+              max_plies: int = 300, playout_seed: int = 12345, net_batch: int = 4096) -> dict:
+    """Value-network leaves against playout leaves at equal wall time per move (netmatch.hpp);
+    returns the tally, the network's score and the per-game outcomes (1 / -1 / 0)."""
+    w = np.ascontiguousarray(params, dtype=np.float32)
+    cfg = NetMatchConfig(games, seed, opening_min, opening_max, move_ms, leaf_batch, max_plies,
+                         playout_seed, net_batch)
+    res = NetMatchResult()
+    outcome = np.zeros(games, dtype=np.int8)
+    err = ctypes.create_string_buffer(512)
+    if host_library().gpuai_net_match_run(ctypes.byref(cfg), _ptr(w), ctypes.byref(res),
+                                          _ptr(outcome), err, 512) != 0:
+        raise RuntimeError(err.value.decode())
+    out = {k: getattr(res, k) for k, _ in NetMatchResult._fields_}
+    out["outcome"] = outcome
+    return out

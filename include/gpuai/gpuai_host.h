@@ -36,7 +36,7 @@ int gpuai_host_apply(mi_pos *pos, const mi_move *move);
 int gpuai_mcts_trace(const mi_pos *root, uint32_t iterations, uint32_t initial, double scale,
                      uint32_t *trace, uint32_t trace_cap);
 
-/* Configuration 3: concurrent MCTS searches sharing the device (multisearch.hpp). */
+/* Batched multi-search: concurrent MCTS searches sharing the device (multisearch.hpp). */
 typedef struct gpuai_multisearch_config {
   uint32_t searches;      /* 256 */
   uint32_t seed;          /* start positions and budgets (2016) */
@@ -57,9 +57,36 @@ typedef struct gpuai_multisearch_result {
   uint64_t root_digest;
 } gpuai_multisearch_result;
 
-/* Runs configuration 3 on device 0; returns 0, or -1 with a message in `err` (err_len bytes). */
+/* Runs the multi-search on device 0; returns 0, or -1 with a message in `err` (err_len bytes). */
 int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch_result *out,
                           char *err, uint32_t err_len);
+
+/* Match of value-network leaves against playout leaves (netmatch.hpp): two MCTS players that
+ * differ only in how a leaf is scored, in lockstep games at equal wall time per move.  No
+ * counterpart in the reference, whose players always play leaves out (src/player.cpp:135-150). */
+typedef struct gpuai_net_match_config {
+  uint32_t games;          /* even: each opening is played with both colour assignments (400) */
+  uint32_t seed;           /* openings (2016) */
+  uint32_t opening_min;    /* random plies before the players take over (0 .. 6) */
+  uint32_t opening_max;
+  double move_ms;          /* wall time of each side's search phase per ply (100) */
+  uint32_t leaf_batch;     /* leaves per iteration over the phase's games (4096) */
+  uint32_t max_plies;      /* games still running after this many plies are draws (300) */
+  uint32_t playout_seed;   /* (12345) */
+  uint32_t net_batch;      /* network batch (4096) */
+} gpuai_net_match_config;
+
+typedef struct gpuai_net_match_result {
+  uint32_t games, net_wins, playout_wins, draws;
+  double score, score_stderr;   /* network's (wins + draws / 2) / games */
+  uint64_t net_leaves, playout_leaves, net_iterations, playout_iterations, plies;
+  double seconds;
+} gpuai_net_match_result;
+
+/* params: MI_MLP_PARAMS network weights; outcome (may be NULL): per game 1 network won,
+ * -1 playout player won, 0 draw.  Returns 0, or -1 with a message in `err`. */
+int gpuai_net_match_run(const gpuai_net_match_config *cfg, const float *params,
+                        gpuai_net_match_result *out, int8_t *outcome, char *err, uint32_t err_len);
 
 #ifdef __cplusplus
 }
