@@ -158,6 +158,7 @@ int gpuai_net_match_run(const gpuai_net_match_config *cfg, const float *params,
     c.maxPlies = cfg->max_plies;
     c.playoutSeed = cfg->playout_seed;
     c.netBatch = cfg->net_batch;
+    c.progressEvery = cfg->progress_every;
     if (c.openingMax < c.openingMin || c.leafBatch == 0 || c.netBatch == 0 || !(c.moveMs >= 0))
       throw std::runtime_error("invalid match configuration");
     NetMatchResult r = runNetMatch(c, params);

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <memory>
 #include <stdexcept>
 
@@ -119,7 +120,7 @@ NetMatchResult runNetMatch(const NetMatchConfig &cfg, const float *params) {
   const auto wall0 = Clock::now();
   std::vector<uint32_t> ids[2];
   std::vector<Move> chosen(cfg.games);
-  for (;;) {
+  for (uint32_t round = 0;; round++) {
     ids[0].clear();
     ids[1].clear();
     for (uint32_t g = 0; g < cfg.games; g++) {
@@ -134,6 +135,15 @@ NetMatchResult runNetMatch(const NetMatchConfig &cfg, const float *params) {
       ids[G.state.turn == G.netSide ? 0 : 1].push_back(g);
     }
     if (ids[0].empty() && ids[1].empty()) break;
+    if (cfg.progressEvery && round % cfg.progressEvery == 0) {
+      uint32_t w = 0, l = 0, d = 0;
+      for (const Game &G : games)
+        if (G.done) (G.outcome > 0 ? w : G.outcome < 0 ? l : d)++;
+      std::fprintf(stderr, "[match] ply %u: %zu games running, network %u wins, playouts %u wins, "
+                   "%u draws, %.1f s\n", round, ids[0].size() + ids[1].size(), w, l, d,
+                   std::chrono::duration<double>(Clock::now() - wall0).count());
+      std::fflush(stderr);
+    }
     for (int side = 0; side < 2; side++) {
       if (side == 0) {
         searchPhase(games, ids[0], 0, cfg,

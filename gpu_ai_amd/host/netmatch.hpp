@@ -25,6 +25,7 @@ struct NetMatchConfig {
   uint32_t maxPlies = 300;       // games still running after this many plies are draws
   uint32_t playoutSeed = 12345;
   uint32_t netBatch = 4096;      // network batch (max_batch)
+  uint32_t progressEvery = 0;    // print a status line to stderr every this many plies (0: never)
   int device = 0;
 };
 

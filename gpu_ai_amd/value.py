@@ -129,7 +129,8 @@ def main(argv=None) -> int:
     if args.match_games:
         from . import host_api
         r = host_api.net_match(params, games=args.match_games, move_ms=args.move_ms,
-                               leaf_batch=args.leaf_batch, net_batch=args.batch)
+                               leaf_batch=args.leaf_batch, net_batch=args.batch,
+                               progress_every=25)
         r["outcome"] = "".join({1: "W", -1: "L", 0: "D"}[int(x)] for x in r["outcome"])
         print(json.dumps({"match": r}), flush=True)
     return 0

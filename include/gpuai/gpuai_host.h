@@ -74,6 +74,7 @@ typedef struct gpuai_net_match_config {
   uint32_t max_plies;      /* games still running after this many plies are draws (300) */
   uint32_t playout_seed;   /* (12345) */
   uint32_t net_batch;      /* network batch (4096) */
+  uint32_t progress_every; /* status line on stderr every this many plies (0: none) */
 } gpuai_net_match_config;
 
 typedef struct gpuai_net_match_result {

@@ -143,7 +143,7 @@ int mi_gen_moves(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, mi_move *moves, 
  * choosing move (r_t mod n_t) with r_t = word (t & 3) of Philox(key (seed, i),
  * counter (t >> 2, a, 1, 0)), stopping early when the game is over.  With reject_terminal, a
  * finished position is redrawn with attempt a + 1.  Follows genRandomStates
- * (src/driver.cpp:76-104), whose shared engine is not reproducible. */
+ * (src/driver.cpp:75-104), whose shared engine is not reproducible. */
 int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin, uint32_t kmax,
                            uint32_t reject_terminal, mi_pos *out);
 
