@@ -101,6 +101,68 @@ def bench_multisearch() -> int:
     return 0
 
 
+# FLOPs of one training step of the 128-256-256-1 network on a batch of B positions: forward
+# 2B(128*256 + 256*256 + 256), backward 2B(128*256 + 2*256*256) (dW1, dW2, dH1) + 2B*256 (dw3)
+def mlp_step_flops(batch: int) -> int:
+    fwd = 2 * batch * (128 * 256 + 256 * 256 + 256)
+    bwd = 2 * batch * (128 * 256 + 2 * 256 * 256 + 256)
+    return fwd + bwd
+
+
+def bench_value(args) -> int:
+    """Learned leaf value: label 2M positions with 256 random playouts each, train the network
+    (batch 4096, Adam lr 1e-3) and measure its error against 1024-playout estimates."""
+    from gpu_ai_amd import value
+    from gpu_ai_amd.engine import Engine, ValueNet
+    eng = Engine(0)
+    t0 = time.perf_counter()
+    pos, t = value.make_dataset(eng, 2_000_000, POS_SEED, playouts=256)
+    label_s = time.perf_counter() - t0
+    epochs = max(1, args.steps)
+    with ValueNet(eng, 4096, seed=POS_SEED) as net:
+        hist = value.train(net, pos, t, args.warmup, 4096, 1e-3) if args.warmup else []
+        t1 = time.perf_counter()
+        timed = value.train(net, pos, t, epochs, 4096, 1e-3, shuffle_seed=11)
+        train_s = time.perf_counter() - t1
+        ev = value.evaluate(eng, net, 16384)
+    eng.close()
+    steps = sum(h["steps"] for h in timed)
+    device_s = sum(h["device_ms"] for h in timed) / 1e3
+    flops = mlp_step_flops(4096)
+    achieved = flops * steps / device_s
+    match = None
+    path = os.path.join(ROOT, "profiles", "value_net_match_400x250ms.md")
+    if os.path.exists(path):
+        m = re.search(r"network score\*\* \(\(wins \+ draws/2\) / games\) \| \*\*([0-9.]+) \+- ([0-9.]+)",
+                      open(path).read())
+        if m:
+            match = {"score": float(m.group(1)), "stderr": float(m.group(2)), "games": 400,
+                     "move_ms": 250, "source": "profiles/value_net_match_400x250ms.md"}
+    line = {
+        "metric": "value-network training throughput (positions/s) on 1 MI355X",
+        "value": round(4096 * steps / train_s, 1), "unit": "positions/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(train_s / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (seeded positions labelled by device playouts)",
+        "config": {"workload": "value-2Mx256: 2,000,000 positions labelled by 256 random "
+                               "playouts, MLP 128-256-256-1, batch 4096, Adam lr 1e-3",
+                   "epochs_timed": epochs, "epochs_warmup": args.warmup},
+        "labelling": {"seconds": round(label_s, 3),
+                      "playouts_per_s": round(2_000_000 * 256 / label_s, 1)},
+        "value_error": {"mse_net": round(ev["mse_net"], 5),
+                        "mse_playouts": {k: round(v, 5) for k, v in ev["mse_playouts"].items()},
+                        "equivalent_playouts": round(ev["equivalent_playouts"], 1),
+                        "after_epochs": epochs + args.warmup},
+        "match_recorded": match,
+        "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 3), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(achieved / 2.5e15, 5), "traffic": None,
+                     "kernel": "training step (k_gemm x5, k_encode, k_head, k_colsum x4, k_adam)",
+                     "flops_per_step": flops, "device_us_per_step": round(device_s / steps * 1e6, 2)},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)

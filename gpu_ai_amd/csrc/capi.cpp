@@ -99,9 +99,51 @@ int plan_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *co
   return MI_OK;
 }
 
+// Longest-first hand-out.  At the end of a batch the chip idles while the last long games
+// finish, one ply at a time per wave; the more pieces a position has, the longer its games
+// tend to run, so positions are handed out by decreasing piece count (stable, counting sort)
+// when lanes will play several games each.  Only the schedule changes: every playout keeps its
+// stream and its result slot.
+void plan_order(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                uint64_t total) {
+  ctx->ordered = npos > 1 && total >= 2ull * ctx->cus * 1024u;
+  if (!ctx->ordered) return;
+  uint32_t start[34] = {0};  // bucket b holds positions with 32 - b pieces (b = 0: most)
+  for (uint32_t i = 0; i < npos; i++) start[1 + (32u - (uint32_t)__builtin_popcount(pos[i].p1 | pos[i].p2))]++;
+  for (uint32_t b = 1; b < 34; b++) start[b] += start[b - 1];
+  ctx->host_order.resize(npos);
+  ctx->host_ord_pos.resize(npos);
+  for (uint32_t i = 0; i < npos; i++) {
+    const uint32_t r = start[32u - (uint32_t)__builtin_popcount(pos[i].p1 | pos[i].p2)]++;
+    ctx->host_order[r] = i;
+    ctx->host_ord_pos[r] = pos[i];
+  }
+  if (counts) {
+    ctx->host_ord_offs.resize((size_t)npos + 1);
+    ctx->host_ord_offs[0] = 0;
+    for (uint32_t r = 0; r < npos; r++)
+      ctx->host_ord_offs[r + 1] = ctx->host_ord_offs[r] + counts[ctx->host_order[r]];
+  }
+}
+
 int upload_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
                  uint64_t total) {
   int rc;
+  plan_order(ctx, pos, npos, counts, total);
+  if (ctx->ordered) {
+    if ((rc = ensure(ctx, ctx->ord_pos, sizeof(mi_pos) * (size_t)npos))) return rc;
+    if ((rc = ensure(ctx, ctx->order, sizeof(uint32_t) * (size_t)npos))) return rc;
+    MI_HIP(ctx, hipMemcpyAsync(ctx->ord_pos.ptr, ctx->host_ord_pos.data(), sizeof(mi_pos) * npos,
+                               hipMemcpyHostToDevice, ctx->stream));
+    MI_HIP(ctx, hipMemcpyAsync(ctx->order.ptr, ctx->host_order.data(), sizeof(uint32_t) * npos,
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (counts) {
+      if ((rc = ensure(ctx, ctx->ord_offs, sizeof(uint32_t) * ((size_t)npos + 1)))) return rc;
+      MI_HIP(ctx, hipMemcpyAsync(ctx->ord_offs.ptr, ctx->host_ord_offs.data(),
+                                 sizeof(uint32_t) * ((size_t)npos + 1), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    }
+  }
   if ((rc = ensure(ctx, ctx->pos, sizeof(mi_pos) * (size_t)std::max<uint32_t>(npos, 1))))
     return rc;
   if ((rc = ensure(ctx, ctx->winners, (size_t)total))) return rc;
@@ -141,6 +183,9 @@ int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
   a.work = (gpuai::WorkWords *)ctx->work.ptr;
   a.defer_min = ctx->tuning.defer_min;
   a.desc = nullptr;
+  a.ord_pos = ctx->ordered ? (const mi_pos *)ctx->ord_pos.ptr : nullptr;
+  a.order = ctx->ordered ? (const uint32_t *)ctx->order.ptr : nullptr;
+  a.ord_offs = ctx->ordered && with_offs ? (const uint32_t *)ctx->ord_offs.ptr : nullptr;
   const uint32_t grid = grid_for(ctx, total);
   MI_HIP(ctx, gpuai::launch_reset(a.work, ctx->stream));
   MI_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));

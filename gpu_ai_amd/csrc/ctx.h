@@ -23,6 +23,12 @@ struct mi_ctx {
   mi_tuning tuning = {512u, 2u, MI_REFILL_QUEUE, 8u};
   uint32_t cus = 0, wave = 64, clock_khz = 0;
   gpuai::DevBuf pos, offs, winners, plies, hashes, work, tallies, moves, counts;
+  // longest-first order of the uploaded batch (see upload_batch): positions permuted into it,
+  // the original index of each, and (uneven counts) prefix offsets in that order
+  gpuai::DevBuf ord_pos, order, ord_offs;
+  bool ordered = false;
+  std::vector<uint32_t> host_order, host_ord_offs;
+  std::vector<mi_pos> host_ord_pos;
   // staged batch
   uint32_t staged_npos = 0, staged_per_pos = 0, staged_total = 0;
   bool staged = false, staged_offs = false;

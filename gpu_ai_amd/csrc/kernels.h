@@ -31,6 +31,13 @@ struct LaunchDesc {
 struct PlayoutArgs {
   const mi_pos *pos;       // npos positions
   const uint32_t *offs;    // npos+1 prefix offsets of playout counts, or null (uniform per_pos)
+  // Optional hand-out order.  When `order` is set, queue index g is taken in rank space:
+  // rank r = the position handed out r-th, order[r] its index in `pos`, ord_pos[r] its copy,
+  // ord_offs (uneven counts) the prefix offsets in rank order.  Results still go to the
+  // position's own slots, so the order changes nothing but the schedule.
+  const mi_pos *ord_pos;
+  const uint32_t *order;
+  const uint32_t *ord_offs;
   uint32_t npos;
   uint32_t per_pos;
   uint32_t total;          // total playouts (< 2^31)

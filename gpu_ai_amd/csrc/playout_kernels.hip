@@ -33,18 +33,31 @@ __global__ void k_reset(WorkWords *work) {
   }
 }
 
-// Position and playout index of global playout g.
-__device__ __forceinline__ void locate(const PlayoutArgs &a, uint32_t g, uint32_t &p, uint32_t &j) {
-  if (a.offs) {
+// Position p, playout j and result slot `out` of the playout with queue index g, and the
+// position itself.
+__device__ __forceinline__ void locate(const PlayoutArgs &a, uint32_t g, uint32_t &p, uint32_t &j,
+                                       uint32_t &out, mi_pos &q) {
+  const uint32_t *offs = a.order ? a.ord_offs : a.offs;
+  uint32_t r;
+  if (offs) {
     uint32_t lo = 0, hi = a.npos;
     while (hi - lo > 1u) {
       uint32_t mid = (lo + hi) >> 1;
-      if (a.offs[mid] <= g) lo = mid; else hi = mid;
+      if (offs[mid] <= g) lo = mid; else hi = mid;
     }
-    p = lo;
-    j = g - a.offs[lo];
+    r = lo;
+    j = g - offs[lo];
   } else {
-    ck::divmod_u32(g, a.per_pos, p, j);
+    ck::divmod_u32(g, a.per_pos, r, j);
+  }
+  if (a.order) {  // two independent loads
+    p = a.order[r];
+    q = a.ord_pos[r];
+    out = a.offs ? a.offs[p] + j : p * a.per_pos + j;
+  } else {
+    p = r;
+    q = a.pos[r];
+    out = g;
   }
 }
 
@@ -131,8 +144,8 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
       if (start) {
         active = true;
         deferred = false;
-        locate(a, g, p, j);
-        const mi_pos q = a.pos[p];
+        mi_pos q;
+        locate(a, g, p, j, g, q);  // from here on g is the playout's result slot
         L = ck::lane_from_abs(q.p1, q.p2, q.kings, q.turn, q.counter);
         ply = 0;
         h = 0;

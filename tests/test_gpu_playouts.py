@@ -158,6 +158,22 @@ def test_uneven_counts_empty_and_single(engine, oracle, golden):
     assert trace_mismatches(one, tr1) == 0
 
 
+@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
+def test_longest_first_order_uneven_counts(engine, oracle, golden, policy):
+    """Batches large enough for the longest-first hand-out (at least 2 x 1024 playouts per CU):
+    uneven counts, zeros and repeated positions; every result slot and trace bit-exact."""
+    pos = np.concatenate([golden["positions"]["c1"], golden["positions"]["c2"]])
+    rng = np.random.default_rng(17 + policy)
+    counts = rng.integers(0, 700, len(pos)).astype(np.uint32)
+    counts[::7] = 0
+    announce(f"longest-first hand-out, uneven counts, {int(counts.sum())} playouts, policy {policy}")
+    assert counts.sum() >= 2 * 256 * 1024
+    res = engine.run_traced(pos, counts=counts, seed=21, epoch=3, policy=policy)
+    _, tr = oracle.run_playouts(pos, counts=counts, seed=21, epoch=3, policy=policy)
+    assert trace_mismatches(res, tr) == 0
+    assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
+
+
 def test_terminal_and_odd_positions(engine, oracle, golden):
     ml = golden["movelists"]["positions"]
     # random placements and king-loop positions sit at the end of the golden set; they include
