@@ -39,12 +39,15 @@ W_PLY = 192                      # int32 lane-ops per ply (frozen cost model)
 SPEC_CUS, SPEC_CLOCK_HZ = 256, 2.4e9
 WORKLOADS = {
     "random": dict(n=1024, per_pos=1024, kmin=0, kmax=30, reject=False, policy=0,
+                   name="random-1024x1024",
                    desc="random-1024x1024: 1024 seeded positions (k in [0,30] random plies) x "
                         "1024 bit-exact random playouts"),
     "heuristic": dict(n=1024, per_pos=1024, kmin=20, kmax=60, reject=True, policy=1,
+                      name="heuristic-1024x1024",
                       desc="heuristic-1024x1024: 1024 seeded midgame positions (k in [20,60], "
                            "finished games redrawn) x 1024 heuristic playouts"),
     "small": dict(n=64, per_pos=4096, kmin=0, kmax=30, reject=False, policy=0,
+                  name="random-64x4096",
                   desc="random-64x4096: 64 seeded positions x 4096 random playouts"),
 }
 POS_SEED, PLAY_SEED = 2016, 12345
@@ -52,7 +55,7 @@ POS_SEED, PLAY_SEED = 2016, 12345
 
 def read_traffic(workload: str):
     """HBM bytes per launch recorded from rocprofv3 --pmc runs (profiles/*traffic*.md)."""
-    path = os.path.join(ROOT, "profiles", f"k_playouts_{workload}_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"k_playouts_{WORKLOADS[workload]['name']}_traffic.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
@@ -67,7 +70,8 @@ def cpu_baseline(cfg: dict, pos, threads: int) -> dict:
         lib = pyoracle.Reference(pyoracle.REF_SO)
     else:
         kind, lib = "port", pyoracle.Oracle()
-    npos = min(len(pos), 256 if cfg["policy"] == 0 else 128)
+    # about 10-30 s of CPU work: the whole random batch, a quarter of the heuristic one
+    npos = min(len(pos), 1024 if cfg["policy"] == 0 else 256)
     per = min(cfg["per_pos"], 1024)
     t0 = time.perf_counter()
     lib.run_playouts(pos[:npos], per_pos=per, seed=PLAY_SEED, epoch=0, policy=cfg["policy"],
@@ -101,6 +105,20 @@ def bench_multisearch() -> int:
     return 0
 
 
+def recorded_match():
+    """The network-leaf vs playout-leaf match result recorded by `python -m gpu_ai_amd.value`
+    (400 games, 250 ms per move; minutes of GPU time, so it is read, not rerun, here)."""
+    path = os.path.join(ROOT, "profiles", "value_net_match_400x250ms.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"network score\*\* \(\(wins \+ draws/2\) / games\) \| \*\*([0-9.]+) \+- ([0-9.]+)",
+                  open(path).read())
+    if not m:
+        return None
+    return {"score": float(m.group(1)), "stderr": float(m.group(2)), "games": 400,
+            "move_ms": 250, "source": "profiles/value_net_match_400x250ms.md"}
+
+
 # FLOPs of one training step of the 128-256-256-1 network on a batch of B positions: forward
 # 2B(128*256 + 256*256 + 256), backward 2B(128*256 + 2*256*256) (dW1, dW2, dH1) + 2B*256 (dw3)
 def mlp_step_flops(batch: int) -> int:
@@ -130,14 +148,7 @@ def bench_value(args) -> int:
     device_s = sum(h["device_ms"] for h in timed) / 1e3
     flops = mlp_step_flops(4096)
     achieved = flops * steps / device_s
-    match = None
-    path = os.path.join(ROOT, "profiles", "value_net_match_400x250ms.md")
-    if os.path.exists(path):
-        m = re.search(r"network score\*\* \(\(wins \+ draws/2\) / games\) \| \*\*([0-9.]+) \+- ([0-9.]+)",
-                      open(path).read())
-        if m:
-            match = {"score": float(m.group(1)), "stderr": float(m.group(2)), "games": 400,
-                     "move_ms": 250, "source": "profiles/value_net_match_400x250ms.md"}
+    match = recorded_match()
     line = {
         "metric": "value-network training throughput (positions/s) on 1 MI355X",
         "value": round(4096 * steps / train_s, 1), "unit": "positions/s", "n_gpus": 1,
@@ -156,7 +167,7 @@ def bench_value(args) -> int:
         "match_recorded": match,
         "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 3), "peak": 2500.0,
                      "unit": "TFLOP/s", "frac": round(achieved / 2.5e15, 5), "traffic": None,
-                     "kernel": "training step (k_gemm x5, k_encode, k_head, k_colsum x4, k_adam)",
+                     "kernel": "training step (k_encode, k_gemm x5, k_head_train, k_colsum, k_adam)",
                      "flops_per_step": flops, "device_us_per_step": round(device_s / steps * 1e6, 2)},
     }
     print(json.dumps(line), flush=True)
@@ -173,6 +184,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shape instead of tuning on the batch")
+    ap.add_argument("--tuning", default="", metavar="BLOCK,BLOCKS_PER_CU,REFILL,DEFER",
+                    help="use this launch shape instead of tuning (refill 0 static, 1 queue); "
+                         "profiled runs use it so that only timed launches are traced")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a plain copy kernel of BYTES (traffic-counter calibration)")
@@ -194,7 +208,10 @@ def main() -> int:
     info = eng.device_info()
     pos = eng.start_positions(POS_SEED, cfg["n"], cfg["kmin"], cfg["kmax"], cfg["reject"])
     tuned = None
-    if not args.no_autotune:
+    if args.tuning:
+        b, bpc, refill, defer = (int(x) for x in args.tuning.split(","))
+        eng.set_tuning(b, bpc, refill, defer)
+    elif not args.no_autotune:
         # the autotuner times every candidate launch shape on this batch (its own Philox epochs)
         tuned, _ = eng.autotune(pos, per_pos=cfg["per_pos"], policy=cfg["policy"], reps=3)
     total = eng.stage(pos, per_pos=cfg["per_pos"])
@@ -239,7 +256,8 @@ def main() -> int:
                    "position_seed": POS_SEED, "playout_seed": PLAY_SEED,
                    "block_size": tuning["block_size"], "blocks_per_cu": tuning["blocks_per_cu"],
                    "refill": "queue" if tuning["refill"] == 1 else "static",
-                   "defer_min": tuning["defer_min"], "autotuned": tuned is not None},
+                   "defer_min": tuning["defer_min"], "autotuned": tuned is not None,
+                   "longest_first": total >= 2 * info["compute_units"] * 1024},
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 4),
                      "peak": round(peak_spec / 1e12, 3), "unit": "Tops/s (int32 lane-ops)",
                      "frac": round(achieved / peak_spec, 4), "traffic": traffic,
@@ -249,6 +267,7 @@ def main() -> int:
                      "device": {"compute_units": info["compute_units"],
                                 "clock_mhz": info["clock_khz"] / 1e3}},
     }
+    line["net_leaf_win_rate_recorded"] = recorded_match()
     if args.copy_calibration:
         ms = eng.copy_calibration(args.copy_calibration, 5)
         line["copy_calibration"] = {"bytes": args.copy_calibration, "avg_ms": round(ms, 4),

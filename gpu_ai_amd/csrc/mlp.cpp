@@ -51,8 +51,9 @@ uint32_t round64(uint32_t n) { return (n + 63u) & ~63u; }
 uint32_t split_len(uint32_t rows) { return rows % 128 == 0 ? 128 : 64; }
 
 K::AdamArgs adam_args(mi_mlp *n, uint32_t splits, uint32_t vsplits, float lr, float c1, float c2,
-                      int update) {
+                      int update, float *loss_out = nullptr) {
   K::AdamArgs a = {};
+  a.loss_out = loss_out;
   a.w = n->w;
   a.m = n->m;
   a.v = n->v;
@@ -137,14 +138,12 @@ int forward_rows(mi_mlp *n, uint32_t valid, uint32_t rows) {
   return MI_OK;
 }
 
-// Backward pass after a forward pass whose head received targets: every gradient, then Adam
-// (or only the gradient sums when update == 0).
-int backward_rows(mi_mlp *n, uint32_t rows, float lr, int update) {
+// Backward pass after the forward pass and the training head (which left dz, dH2 and the b2,
+// w3, b3 and loss partials): every other gradient, then Adam (or only the gradient sums when
+// update == 0); loss_out (device, may be null) receives the batch loss.
+int backward_rows(mi_mlp *n, uint32_t rows, float lr, int update, float *loss_out) {
   mi_ctx *ctx = n->ctx;
   hipStream_t s = ctx->stream;
-  MI_HIP(ctx, K::colsum_bf16(n->H2, rows, 256, n->dz, n->pV + 512, s));  // w3
-  MI_HIP(ctx, K::colsum_f32(n->dz, rows, 1, nullptr, n->pV + 768, s));    // b3
-  MI_HIP(ctx, K::colsum_bf16(n->dH2, rows, 256, nullptr, n->pV + 256, s)); // b2
   const uint32_t kps = split_len(rows), splits = rows / kps;
   K::GemmArgs w2 = {};  // dW2[i][o] = sum_b H1[b][i] dH2[b][o]
   w2.A = n->H1; w2.sAm = 1; w2.sAk = 256;
@@ -172,7 +171,7 @@ int backward_rows(mi_mlp *n, uint32_t rows, float lr, int update) {
     c1 = (float)(1.0 / (1.0 - std::pow(n->beta1, (double)n->step)));
     c2 = (float)(1.0 / (1.0 - std::pow(n->beta2, (double)n->step)));
   }
-  MI_HIP(ctx, K::adam(adam_args(n, splits, rows / 64, lr, c1, c2, update), s));
+  MI_HIP(ctx, K::adam(adam_args(n, splits, rows / 64, lr, c1, c2, update, loss_out), s));
   return MI_OK;
 }
 
@@ -292,8 +291,8 @@ int mi_mlp_forward(mi_mlp *n, const mi_pos *pos, uint32_t count, float *values) 
   uint32_t rows;
   if ((rc = stage_batch(n, pos, nullptr, count, rows))) return rc;
   if ((rc = forward_rows(n, count, rows))) return rc;
-  MI_HIP(ctx, K::head(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, nullptr, nullptr, count,
-                      rows, n->y, nullptr, nullptr, nullptr, ctx->stream));
+  MI_HIP(ctx, K::head_forward(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, count, rows, n->y,
+                              ctx->stream));
   MI_HIP(ctx, hipMemcpyAsync(values, n->y, sizeof(float) * count, hipMemcpyDeviceToHost, ctx->stream));
   MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MI_OK;
@@ -310,10 +309,9 @@ static int grad_common(mi_mlp *n, const mi_pos *pos, const float *targets, uint3
   uint32_t rows;
   if ((rc = stage_batch(n, pos, targets, count, rows))) return rc;
   if ((rc = forward_rows(n, count, rows))) return rc;
-  MI_HIP(ctx, K::zero(n->loss, 1, ctx->stream));
-  MI_HIP(ctx, K::head(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, n->targets, nullptr,
-                      count, rows, n->y, n->dz, n->dH2, n->loss, ctx->stream));
-  if ((rc = backward_rows(n, rows, lr, update))) return rc;
+  MI_HIP(ctx, K::head_train(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, n->targets, nullptr,
+                            count, rows, n->dz, n->dH2, n->pV, ctx->stream));
+  if ((rc = backward_rows(n, rows, lr, update, n->loss))) return rc;
   if (grads)
     MI_HIP(ctx, hipMemcpyAsync(grads, n->g, sizeof(float) * MI_MLP_PARAMS, hipMemcpyDeviceToHost,
                                ctx->stream));
@@ -384,15 +382,15 @@ int mi_mlp_train_epoch(mi_mlp *n, uint32_t batch, float lr, uint32_t shuffle_see
   MI_HIP(ctx, hipMemcpyAsync(n->ds_perm.ptr, perm.data(), sizeof(uint32_t) * n->ds_n,
                              hipMemcpyHostToDevice, ctx->stream));
   float *losses = (float *)n->ds_loss.ptr;
-  MI_HIP(ctx, K::zero(losses, nb, ctx->stream));
   MI_HIP(ctx, hipEventRecord(n->e0, ctx->stream));
   for (uint32_t b = 0; b < nb; b++) {
     const uint32_t *idx = (const uint32_t *)n->ds_perm.ptr + (size_t)b * batch;
     MI_HIP(ctx, K::encode((const mi_pos *)n->ds_pos.ptr, idx, batch, batch, n->X, ctx->stream));
     if ((rc = forward_rows(n, batch, batch))) return rc;
-    MI_HIP(ctx, K::head(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, (const float *)n->ds_t.ptr,
-                        idx, batch, batch, n->y, n->dz, n->dH2, losses + b, ctx->stream));
-    if ((rc = backward_rows(n, batch, lr, 1))) return rc;
+    MI_HIP(ctx, K::head_train(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3,
+                              (const float *)n->ds_t.ptr, idx, batch, batch, n->dz, n->dH2, n->pV,
+                              ctx->stream));
+    if ((rc = backward_rows(n, batch, lr, 1, losses + b))) return rc;
   }
   MI_HIP(ctx, hipEventRecord(n->e1, ctx->stream));
   std::vector<float> l(nb);

@@ -30,7 +30,7 @@ struct GemmArgs {
   int row_valid;          // rows m >= row_valid are written as 0 (0: all rows valid)
 };
 
-// Column-sum partial slots per 64-row block: b1 at 0-255, b2 256-511, w3 512-767, b3 768.
+// Partial slots per 64-row block: b1 at 0-255, b2 256-511, w3 512-767, b3 768, loss 769.
 constexpr uint32_t kVecSlots = 1024;
 
 struct AdamArgs {
@@ -43,15 +43,21 @@ struct AdamArgs {
   float lr, beta1, beta2, eps, c1, c2;  // c1 = 1/(1-beta1^t), c2 = 1/(1-beta2^t)
   float omb1, omb2;        // 1 - beta1, 1 - beta2 (formed in double precision on the host)
   int update;              // 0: only sum the gradients and refresh the BF16 copies
+  float *loss_out;         // when set, receives the sum of the loss partials (slot 769)
 };
 
 hipError_t gemm(const GemmArgs &g, int splits, bool a_bf16, bool b_bf16, hipStream_t stream);
 hipError_t encode(const mi_pos *pos, const uint32_t *idx, uint32_t n, uint32_t rows, uint16_t *X,
                   hipStream_t stream);
-// targets[tidx ? tidx[row] : row] is the target of a row (targets may be null: forward only)
-hipError_t head(const uint16_t *H2, const float *w3, const float *b3, const float *targets,
-                const uint32_t *tidx, uint32_t n, uint32_t rows, float *y, float *dz,
-                uint16_t *dH2, float *loss, hipStream_t stream);
+// y = tanh(H2 . w3 + b3) for rows < n.
+hipError_t head_forward(const uint16_t *H2, const float *w3, const float *b3, uint32_t n,
+                        uint32_t rows, float *y, hipStream_t stream);
+// Loss and output-layer gradients over 64-row blocks (rows a multiple of 64); the target of
+// row r is targets[tidx ? tidx[r] : r].  Writes dz, dH2 and the partial slots of b2, w3, b3
+// and the loss.
+hipError_t head_train(const uint16_t *H2, const float *w3, const float *b3, const float *targets,
+                      const uint32_t *tidx, uint32_t n, uint32_t rows, float *dz, uint16_t *dH2,
+                      float *part, hipStream_t stream);
 // Partial column sums of 64-row blocks into part[block * kVecSlots + col].
 hipError_t colsum_bf16(const uint16_t *src, uint32_t rows, uint32_t cols, const float *weight,
                        float *part, hipStream_t stream);

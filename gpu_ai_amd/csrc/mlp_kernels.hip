@@ -128,40 +128,88 @@ __global__ void __launch_bounds__(256) k_encode(const mi_pos *__restrict__ pos,
   for (int s = 0; s < 32; s++) dst[s] = ((bits >> s) & 1u) ? (uint16_t)0x3F80u : (uint16_t)0u;
 }
 
-// Output layer and loss: y = tanh(H2 . w3 + b3) per row (one wave per row, four columns per
-// lane).  With targets: dz = 2 (y - t)(1 - y^2) / n, the squared error into loss (atomic), and
-// dH2 = dz w3 masked by H2 > 0 (BF16).  Rows at or past n get dz = 0.
-__global__ void __launch_bounds__(256) k_head(const uint16_t *__restrict__ H2,
-                                             const float *__restrict__ w3, const float *b3,
-                                             const float *__restrict__ targets,
-                                             const uint32_t *__restrict__ tidx, uint32_t n,
-                                             uint32_t rows, float *__restrict__ y,
-                                             float *__restrict__ dz, uint16_t *__restrict__ dH2,
-                                             float *loss) {
+// Output layer, forward only: y = tanh(H2 . w3 + b3) per row (one wave per row, four columns
+// per lane, a butterfly sum over the wave).
+__global__ void __launch_bounds__(256) k_head_fwd(const uint16_t *__restrict__ H2,
+                                                 const float *__restrict__ w3, const float *b3,
+                                                 uint32_t n, uint32_t rows, float *__restrict__ y) {
   const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (row >= rows) return;
   const uint16_t *h = H2 + (size_t)row * 256 + lane * 4;
-  float hv[4], part = 0.f;
+  float part = 0.f;
+#pragma unroll
+  for (int c = 0; c < 4; c++) part += from_bf(h[c]) * w3[lane * 4 + c];
+  for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
+  if (lane == 0 && row < n) y[row] = tanhf(part + b3[0]);
+}
+
+// Output layer with its loss and gradients, over 64-row blocks (four waves of 16 rows; lane l
+// holds columns 4l..4l+3).  Per row: y = tanh(H2 . w3 + b3), e = y - t, dz = 2 e (1 - y^2) / n
+// (0 on padding rows) and dH2 = dz w3 masked by H2 > 0 (BF16).  Per block, into the partial
+// slots of block b (part + b * kVecSlots): b2 (256 + c) = sum of dH2, w3 (512 + c) = sum of
+// dz H2, b3 (768) = sum of dz, and the loss (769) = sum of e^2 / n.  No atomics: the partials
+// are summed in a fixed order by k_adam.
+__global__ void __launch_bounds__(256) k_head_train(const uint16_t *__restrict__ H2,
+                                                   const float *__restrict__ w3, const float *b3,
+                                                   const float *__restrict__ targets,
+                                                   const uint32_t *__restrict__ tidx, uint32_t n,
+                                                   float *__restrict__ dz,
+                                                   uint16_t *__restrict__ dH2,
+                                                   float *__restrict__ part) {
+  __shared__ float red_b2[4][256], red_w3[4][256], red_s[4][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float wl[4], acc_b2[4] = {0.f, 0.f, 0.f, 0.f}, acc_w3[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < 4; c++) wl[c] = w3[lane * 4 + c];
+  const float bias = b3[0];
+  float acc_b3 = 0.f, acc_loss = 0.f;
+  const uint32_t row0 = blockIdx.x * 64u + wv * 16u;
+  for (uint32_t i = 0; i < 16u; i++) {
+    const uint32_t row = row0 + i;
+    const uint16_t *h = H2 + (size_t)row * 256 + lane * 4;
+    float hv[4], dot = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      hv[c] = from_bf(h[c]);
+      dot += hv[c] * wl[c];
+    }
+    for (int off = 32; off >= 1; off >>= 1) dot += __shfl_xor(dot, off);
+    const float yy = tanhf(dot + bias);
+    float d = 0.f;
+    if (row < n) {
+      const float e = yy - targets[tidx ? tidx[row] : row];
+      d = 2.f * e * (1.f - yy * yy) / (float)n;
+      acc_loss += e * e / (float)n;
+    }
+    if (lane == 0) dz[row] = d;
+    acc_b3 += d;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const uint16_t gb = to_bf(hv[c] > 0.f ? d * wl[c] : 0.f);
+      dH2[(size_t)row * 256 + lane * 4 + c] = gb;
+      acc_b2[c] += from_bf(gb);
+      acc_w3[c] += d * hv[c];
+    }
+  }
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    hv[c] = from_bf(h[c]);
-    part += hv[c] * w3[lane * 4 + c];
+    red_b2[wv][lane * 4 + c] = acc_b2[c];
+    red_w3[wv][lane * 4 + c] = acc_w3[c];
   }
-  for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
-  const float yy = tanhf(part + b3[0]);
-  if (lane == 0 && y && row < n) y[row] = yy;
-  if (!targets) return;
-  float d = 0.f;
-  if (row < n) {
-    const float e = yy - targets[tidx ? tidx[row] : row];
-    d = 2.f * e * (1.f - yy * yy) / (float)n;
-    if (lane == 0) atomicAdd(loss, e * e / (float)n);
+  if (lane == 0) {
+    red_s[wv][0] = acc_b3;
+    red_s[wv][1] = acc_loss;
   }
-  if (lane == 0) dz[row] = d;
-#pragma unroll
-  for (int c = 0; c < 4; c++)
-    dH2[(size_t)row * 256 + lane * 4 + c] = to_bf(hv[c] > 0.f ? d * w3[lane * 4 + c] : 0.f);
+  __syncthreads();
+  float *pb = part + (size_t)blockIdx.x * kVecSlots;
+  const int t = threadIdx.x;
+  pb[256 + t] = red_b2[0][t] + red_b2[1][t] + red_b2[2][t] + red_b2[3][t];
+  pb[512 + t] = red_w3[0][t] + red_w3[1][t] + red_w3[2][t] + red_w3[3][t];
+  if (t == 0) {
+    pb[768] = red_s[0][0] + red_s[1][0] + red_s[2][0] + red_s[3][0];
+    pb[769] = red_s[0][1] + red_s[1][1] + red_s[2][1] + red_s[3][1];
+  }
 }
 
 // Partial column sums: block (x, y) writes part[y * part_stride + col] = sum over rows
@@ -202,16 +250,25 @@ __global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
   if (p >= MI_MLP_PARAMS) return;
   float g = 0.f;
   if (p < MI_MLP_OFF_B1) {
+#pragma unroll 8
     for (uint32_t s = 0; s < a.splits; s++) g += a.pW1[(size_t)s * 32768 + p];
   } else if (p < MI_MLP_OFF_W2) {
+#pragma unroll 8
     for (uint32_t s = 0; s < a.vsplits; s++) g += a.pV[(size_t)s * kVecSlots + (p - MI_MLP_OFF_B1)];
   } else if (p < MI_MLP_OFF_B2) {
+#pragma unroll 8
     for (uint32_t s = 0; s < a.splits; s++) g += a.pW2[(size_t)s * 65536 + (p - MI_MLP_OFF_W2)];
   } else {
+#pragma unroll 8
     for (uint32_t s = 0; s < a.vsplits; s++)
       g += a.pV[(size_t)s * kVecSlots + 256 + (p - MI_MLP_OFF_B2)];
   }
   a.grads[p] = g;
+  if (p == 0u && a.loss_out) {  // the batch loss: sum of the head's per-block partials
+    float l = 0.f;
+    for (uint32_t s = 0; s < a.vsplits; s++) l += a.pV[(size_t)s * kVecSlots + 769];
+    *a.loss_out = l;
+  }
   float w = a.w[p];
   if (a.update) {
     const float m = a.beta1 * a.m[p] + a.omb1 * g;
@@ -255,11 +312,19 @@ hipError_t encode(const mi_pos *pos, const uint32_t *idx, uint32_t n, uint32_t r
   return hipGetLastError();
 }
 
-hipError_t head(const uint16_t *H2, const float *w3, const float *b3, const float *targets,
-                const uint32_t *tidx, uint32_t n, uint32_t rows, float *y, float *dz,
-                uint16_t *dH2, float *loss, hipStream_t stream) {
-  hipLaunchKernelGGL(k_head, dim3((rows * 64 + 255) / 256), dim3(256), 0, stream, H2, w3, b3,
-                     targets, tidx, n, rows, y, dz, dH2, loss);
+hipError_t head_forward(const uint16_t *H2, const float *w3, const float *b3, uint32_t n,
+                        uint32_t rows, float *y, hipStream_t stream) {
+  hipLaunchKernelGGL(k_head_fwd, dim3((rows * 64 + 255) / 256), dim3(256), 0, stream, H2, w3, b3,
+                     n, rows, y);
+  return hipGetLastError();
+}
+
+hipError_t head_train(const uint16_t *H2, const float *w3, const float *b3, const float *targets,
+                      const uint32_t *tidx, uint32_t n, uint32_t rows, float *dz, uint16_t *dH2,
+                      float *part, hipStream_t stream) {
+  if (rows % 64u) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_head_train, dim3(rows / 64), dim3(256), 0, stream, H2, w3, b3, targets,
+                     tidx, n, dz, dH2, part);
   return hipGetLastError();
 }
 
