@@ -267,6 +267,13 @@ def main() -> int:
                      "device": {"compute_units": info["compute_units"],
                                 "clock_mhz": info["clock_khz"] / 1e3}},
     }
+    # the same batch through mi_run_playouts with host buffers (positions in, winners out, over
+    # PCIe): reported beside the figure, never as it
+    hb = eng.run_playouts(pos, per_pos=cfg["per_pos"], seed=PLAY_SEED, epoch=5000,
+                          policy=cfg["policy"])
+    line["host_buffers"] = {"total_ms": round(hb.stats["total_ms"], 4),
+                            "kernel_ms": round(hb.stats["kernel_ms"], 4),
+                            "playouts_per_s": round(total / (hb.stats["total_ms"] / 1e3), 1)}
     line["net_leaf_win_rate_recorded"] = recorded_match()
     if args.copy_calibration:
         ms = eng.copy_calibration(args.copy_calibration, 5)
