@@ -37,15 +37,15 @@ def main() -> None:
     ref = pyoracle.Reference(pyoracle.REF_ASSERT_SO)
 
     # ---- start positions
-    c1 = ref.start_positions(SEED_POS, 1024, 0, 30, False)
-    c2 = ref.start_positions(SEED_POS, 1024, 20, 60, True)
-    np.savez_compressed(os.path.join(HERE, "positions.npz"), c1=c1, c2=c2)
+    open1024 = ref.start_positions(SEED_POS, 1024, 0, 30, False)
+    mid1024 = ref.start_positions(SEED_POS, 1024, 20, 60, True)
+    np.savez_compressed(os.path.join(HERE, "positions.npz"), open1024=open1024, mid1024=mid1024)
 
     # ---- move lists: configuration starts plus positions sampled along seeded random games
     rng = np.random.Generator(np.random.Philox(7))
-    pool = [c1, c2]
+    pool = [open1024, mid1024]
     walk = []
-    for start in np.concatenate([c1[:400], c2[:400]]):
+    for start in np.concatenate([open1024[:400], mid1024[:400]]):
         p = start.copy()
         for _ in range(int(rng.integers(1, 120))):
             moves = ref.gen_moves(p)
@@ -105,7 +105,7 @@ def main() -> None:
     # ---- perft
     perft = {"start": [ref.perft(pyoracle.start_position(), d) for d in range(1, 9)],
              "midgame": []}
-    for p in c2[:8]:
+    for p in mid1024[:8]:
         perft["midgame"].append({"position": [int(p["p1"]), int(p["p2"]), int(p["kings"]),
                                               int(p["turn"]), int(p["counter"])],
                                  "counts": [ref.perft(p, d) for d in range(1, 6)]})
@@ -114,16 +114,16 @@ def main() -> None:
 
     # ---- playouts
     out = {}
-    w, tr = ref.run_playouts(c1[:256], per_pos=16, seed=SEED_PLAY, epoch=0,
+    w, tr = ref.run_playouts(open1024[:256], per_pos=16, seed=SEED_PLAY, epoch=0,
                              policy=pyoracle.POLICY_RANDOM)
-    out["random_c1_256x16"] = tr
-    w, tr = ref.run_playouts(c2[:256], per_pos=16, seed=SEED_PLAY, epoch=0,
+    out["random_open256x16"] = tr
+    w, tr = ref.run_playouts(mid1024[:256], per_pos=16, seed=SEED_PLAY, epoch=0,
                              policy=pyoracle.POLICY_HEURISTIC)
-    out["heuristic_c2_256x16"] = tr
+    out["heuristic_mid256x16"] = tr
     counts_uneven = (np.arange(64, dtype=np.uint32) * 7) % 23
-    w, tr = ref.run_playouts(c2[:64], counts=counts_uneven, seed=99, epoch=7,
+    w, tr = ref.run_playouts(mid1024[:64], counts=counts_uneven, seed=99, epoch=7,
                              policy=pyoracle.POLICY_RANDOM)
-    out["random_c2_64_uneven_e7"] = tr
+    out["random_mid64_uneven_e7"] = tr
     out["uneven_counts"] = counts_uneven
     np.savez_compressed(os.path.join(HERE, "playouts.npz"), **out)
 

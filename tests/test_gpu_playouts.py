@@ -56,7 +56,7 @@ def test_gen_moves_golden(engine, golden):
 
 def test_gen_moves_capacity_error(engine, golden):
     from gpu_ai_amd.engine import MI_ERR_CAPACITY, MiError
-    pos = golden["positions"]["c1"][:64]
+    pos = golden["positions"]["open1024"][:64]
     with pytest.raises(MiError) as e:
         engine.gen_moves(pos, cap=1)
     assert e.value.code == MI_ERR_CAPACITY
@@ -65,12 +65,12 @@ def test_gen_moves_capacity_error(engine, golden):
 def golden_sets(golden):
     g, pl = golden["positions"], golden["playouts"]
     return [
-        ("random c1[:256]x16", g["c1"][:256], dict(per_pos=16, seed=12345, epoch=0, policy=0),
-         pl["random_c1_256x16"]),
-        ("heuristic c2[:256]x16", g["c2"][:256], dict(per_pos=16, seed=12345, epoch=0, policy=1),
-         pl["heuristic_c2_256x16"]),
-        ("random c2[:64] uneven e7", g["c2"][:64],
-         dict(counts=pl["uneven_counts"], seed=99, epoch=7, policy=0), pl["random_c2_64_uneven_e7"]),
+        ("random open1024[:256]x16", g["open1024"][:256], dict(per_pos=16, seed=12345, epoch=0, policy=0),
+         pl["random_open256x16"]),
+        ("heuristic mid1024[:256]x16", g["mid1024"][:256], dict(per_pos=16, seed=12345, epoch=0, policy=1),
+         pl["heuristic_mid256x16"]),
+        ("random mid1024[:64] uneven e7", g["mid1024"][:64],
+         dict(counts=pl["uneven_counts"], seed=99, epoch=7, policy=0), pl["random_mid64_uneven_e7"]),
     ]
 
 
@@ -84,7 +84,7 @@ def test_traced_playouts_match_reference_golden(engine, golden):
 
 @pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
 def test_traced_full_positions_vs_oracle(engine, oracle, golden, policy):
-    pos = golden["positions"]["c1" if policy == POLICY_RANDOM else "c2"]
+    pos = golden["positions"]["open1024" if policy == POLICY_RANDOM else "mid1024"]
     announce(f"traced 1024 positions x 64, policy {policy}")
     res = engine.run_traced(pos, per_pos=64, seed=2024, epoch=1, policy=policy)
     _, tr = oracle.run_playouts(pos, per_pos=64, seed=2024, epoch=1, policy=policy)
@@ -92,27 +92,27 @@ def test_traced_full_positions_vs_oracle(engine, oracle, golden, policy):
     assert res.stats["max_moves"] == int(tr["max_n"].max())
 
 
-def test_config1_random_full_bitexact(engine, oracle, golden):
-    pos = golden["positions"]["c1"]
-    announce("config 1: 1024 positions x 1024 random playouts")
+def test_random_1024x1024_full_bitexact(engine, oracle, golden):
+    pos = golden["positions"]["open1024"]
+    announce("random-1024x1024: 1024 positions x 1024 random playouts")
     res = engine.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=POLICY_RANDOM)
     w, tr = oracle.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=POLICY_RANDOM)
     assert np.array_equal(res.winners, w)
     assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
 
 
-def test_config2_heuristic_full_bitexact(engine, oracle, golden):
-    pos = golden["positions"]["c2"]
-    announce("config 2: 1024 positions x 1024 heuristic playouts")
+def test_heuristic_1024x1024_full_bitexact(engine, oracle, golden):
+    pos = golden["positions"]["mid1024"]
+    announce("heuristic-1024x1024: 1024 midgame positions x 1024 heuristic playouts")
     res = engine.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=POLICY_HEURISTIC)
     w, tr = oracle.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=POLICY_HEURISTIC)
     assert np.array_equal(res.winners, w)
     assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
 
 
-def test_config0_bitexact(engine, oracle, golden):
-    pos = golden["positions"]["c1"][:64]
-    announce("config 0: 64 positions x 4096 random playouts")
+def test_random_64x4096_bitexact(engine, oracle, golden):
+    pos = golden["positions"]["open1024"][:64]
+    announce("random-64x4096: 64 positions x 4096 random playouts")
     res = engine.run_playouts(pos, per_pos=4096, seed=12345, epoch=0, policy=POLICY_RANDOM)
     w, _ = oracle.run_playouts(pos, per_pos=4096, seed=12345, epoch=0, policy=POLICY_RANDOM,
                                traces=False)
@@ -123,7 +123,7 @@ def test_config0_bitexact(engine, oracle, golden):
                                                     (256, 8, REFILL_STATIC, 8), (1024, 2, REFILL_QUEUE, 1),
                                                     (512, 4, REFILL_QUEUE, 64)])
 def test_launch_shape_does_not_change_results(engine, golden, block, bpc, refill, defer):
-    pos = golden["positions"]["c2"][:256]
+    pos = golden["positions"]["mid1024"][:256]
     announce(f"tuning block {block} blocks/CU {bpc} refill {refill} defer {defer}")
     base = engine.run_traced(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
     fast = engine.run_playouts(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
@@ -141,7 +141,7 @@ def test_launch_shape_does_not_change_results(engine, golden, block, bpc, refill
 
 
 def test_uneven_counts_empty_and_single(engine, oracle, golden):
-    pos = golden["positions"]["c1"][:40]
+    pos = golden["positions"]["open1024"][:40]
     counts = np.array([0, 1, 0, 0, 5, 63, 64, 65, 0, 1000] + [3] * 30, dtype=np.uint32)
     announce("uneven counts with zeros")
     res = engine.run_traced(pos, counts=counts, seed=5, epoch=0, policy=POLICY_RANDOM)
@@ -162,7 +162,7 @@ def test_uneven_counts_empty_and_single(engine, oracle, golden):
 def test_longest_first_order_uneven_counts(engine, oracle, golden, policy):
     """Batches large enough for the longest-first hand-out (at least 2 x 1024 playouts per CU):
     uneven counts, zeros and repeated positions; every result slot and trace bit-exact."""
-    pos = np.concatenate([golden["positions"]["c1"], golden["positions"]["c2"]])
+    pos = np.concatenate([golden["positions"]["open1024"], golden["positions"]["mid1024"]])
     rng = np.random.default_rng(17 + policy)
     counts = rng.integers(0, 700, len(pos)).astype(np.uint32)
     counts[::7] = 0
@@ -191,7 +191,7 @@ def test_terminal_and_odd_positions(engine, oracle, golden):
 
 
 def test_tally_staged_and_repeat(engine, golden):
-    pos = golden["positions"]["c1"][:300]
+    pos = golden["positions"]["open1024"][:300]
     counts = (np.arange(300, dtype=np.uint32) * 37) % 200
     announce("tally / staged / repeat")
     res = engine.run_playouts(pos, counts=counts, seed=77, epoch=3, policy=POLICY_HEURISTIC)
@@ -211,17 +211,17 @@ def test_tally_staged_and_repeat(engine, golden):
 
 def test_device_start_positions_match_reference_golden(engine, golden):
     announce("mi_gen_start_positions for configs 0/1 and 2")
-    c1 = engine.start_positions(2016, 1024, 0, 30, False)
-    c2 = engine.start_positions(2016, 1024, 20, 60, True)
-    assert c1.tobytes() == golden["positions"]["c1"].tobytes()
-    assert c2.tobytes() == golden["positions"]["c2"].tobytes()
+    open1024 = engine.start_positions(2016, 1024, 0, 30, False)
+    mid1024 = engine.start_positions(2016, 1024, 20, 60, True)
+    assert open1024.tobytes() == golden["positions"]["open1024"].tobytes()
+    assert mid1024.tobytes() == golden["positions"]["mid1024"].tobytes()
 
 
 @pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
 def test_graph_replays_match_eager(engine, golden, policy):
     """Replays one captured graph on batches of different sizes, with eager launches and
     synchronisations in between, and checks every replay against mi_run_playouts."""
-    pos = golden["positions"]["c2" if policy else "c1"]
+    pos = golden["positions"]["mid1024" if policy else "open1024"]
     rng = np.random.default_rng(9 + policy)
     with engine.graph(max_positions=1024, max_playouts=300000, policy=policy) as g:
         for rep in range(6):

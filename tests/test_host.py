@@ -22,7 +22,7 @@ def test_host_gen_moves_golden(golden):
 def test_host_apply_matches_oracle(oracle, golden):
     from gpu_ai_amd import host_api
     rng = np.random.default_rng(5)
-    pos = golden["positions"]["c2"][:200]
+    pos = golden["positions"]["mid1024"][:200]
     for p in pos:
         for _ in range(30):
             ms = oracle.gen_moves(p)
@@ -37,7 +37,7 @@ def test_host_apply_matches_oracle(oracle, golden):
 
 def test_host_apply_rejects_illegal(golden):
     from gpu_ai_amd import host_api
-    p = golden["positions"]["c1"][0]
+    p = golden["positions"]["open1024"][0]
     m = np.zeros(1, dtype=pyoracle.MOVE_DTYPE)[0]
     m["from"], m["to"] = 31, 0
     with pytest.raises(ValueError):
@@ -45,8 +45,8 @@ def test_host_apply_rejects_illegal(golden):
 
 
 @pytest.mark.skipif(not pyoracle.reference_available(), reason="oracle/_ref not built here")
-@pytest.mark.parametrize("which,iters,initial,scale", [("start", 40, 50, 0.02), ("c2_0", 30, 200, 0.25),
-                                                      ("c2_5", 12, 64, 1.0), ("c1_9", 60, 7, 0.1)])
+@pytest.mark.parametrize("which,iters,initial,scale", [("start", 40, 50, 0.02), ("mid1024_0", 30, 200, 0.25),
+                                                      ("mid1024_5", 12, 64, 1.0), ("open1024_9", 60, 7, 0.1)])
 def test_mcts_matches_reference_gametree(golden, which, iters, initial, scale):
     """GameTree select/update/UCB1/getOptMove against the reference's own src/mcts.cpp."""
     from gpu_ai_amd import host_api

@@ -110,21 +110,21 @@ def test_movelists_golden(oracle, golden):
 
 
 def test_start_positions_golden(oracle, golden):
-    c1 = oracle.start_positions(2016, 1024, 0, 30, False)
-    c2 = oracle.start_positions(2016, 1024, 20, 60, True)
-    assert c1.tobytes() == golden["positions"]["c1"].tobytes()
-    assert c2.tobytes() == golden["positions"]["c2"].tobytes()
-    # config 2 rejects finished positions
-    assert not any(oracle.is_game_over(p) for p in c2)
+    open1024 = oracle.start_positions(2016, 1024, 0, 30, False)
+    mid1024 = oracle.start_positions(2016, 1024, 20, 60, True)
+    assert open1024.tobytes() == golden["positions"]["open1024"].tobytes()
+    assert mid1024.tobytes() == golden["positions"]["mid1024"].tobytes()
+    # the midgame set rejects finished positions
+    assert not any(oracle.is_game_over(p) for p in mid1024)
 
 
 def playout_sets(golden):
     g, pl = golden["positions"], golden["playouts"]
     return [
-        (g["c1"][:256], dict(per_pos=16, seed=12345, epoch=0, policy=0), pl["random_c1_256x16"]),
-        (g["c2"][:256], dict(per_pos=16, seed=12345, epoch=0, policy=1), pl["heuristic_c2_256x16"]),
-        (g["c2"][:64], dict(counts=pl["uneven_counts"], seed=99, epoch=7, policy=0),
-         pl["random_c2_64_uneven_e7"]),
+        (g["open1024"][:256], dict(per_pos=16, seed=12345, epoch=0, policy=0), pl["random_open256x16"]),
+        (g["mid1024"][:256], dict(per_pos=16, seed=12345, epoch=0, policy=1), pl["heuristic_mid256x16"]),
+        (g["mid1024"][:64], dict(counts=pl["uneven_counts"], seed=99, epoch=7, policy=0),
+         pl["random_mid64_uneven_e7"]),
     ]
 
 
@@ -271,7 +271,7 @@ def test_mutation_is_detected(mutants, golden, k):
 # ---------------------------------------------------------------------------------------------
 
 @pytest.mark.skipif(not pyoracle.reference_available(), reason="oracle/_ref not built here")
-def test_oracle_matches_reference_build_c0(oracle):
+def test_oracle_matches_reference_build_64_positions(oracle):
     ref = pyoracle.Reference()
     pos = oracle.start_positions(2016, 64, 0, 30, False)
     assert pos.tobytes() == ref.start_positions(2016, 64, 0, 30, False).tobytes()
