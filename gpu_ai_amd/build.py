@@ -36,7 +36,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I" + os.path
 LIB_SOURCES = ["playout_kernels.hip", "capi.cpp", "mlp_kernels.hip", "mlp.cpp"]
 # the C++ host layer (caller side: State, PlayoutDriver, GameTree, players, CLI)
 HOST_SOURCES = ["state.cpp", "playout.cpp", "mcts.cpp", "player.cpp", "multisearch.cpp",
-                "netmatch.cpp", "capi_host.cpp"]
+                "netleaf.cpp", "netmatch.cpp", "capi_host.cpp"]
 HOST_LIB_PATH = os.path.join(PKG, "libgpuai_host.so")
 RUN_AI_PATH = os.path.join(BIN, "run_ai")
 

@@ -8,8 +8,8 @@
 #include <memory>
 #include <stdexcept>
 
-#include "gpuai/mi_mlp.h"
 #include "mcts.hpp"
+#include "netleaf.hpp"
 #include "playout.hpp"
 
 namespace {
@@ -59,10 +59,7 @@ NetMatchResult runNetMatch(const NetMatchConfig &cfg, const float *params) {
   if (cfg.games == 0 || cfg.games % 2) throw std::invalid_argument("games must be even and > 0");
   DeviceMI355XPlayoutDriver driver(MI_POLICY_RANDOM, cfg.device, cfg.playoutSeed);
   mi_ctx *ctx = driver.context();
-  mi_mlp *net = nullptr;
-  checkMi(mi_mlp_create(ctx, cfg.netBatch, 1u, &net), ctx, "mi_mlp_create");
-  std::unique_ptr<mi_mlp, int (*)(mi_mlp *)> netGuard(net, mi_mlp_destroy);
-  checkMi(mi_mlp_set_params(net, params), ctx, "mi_mlp_set_params");
+  NetLeafEvaluator net(ctx, params, cfg.netBatch);
 
   // openings: one per pair of games
   const uint32_t pairs = cfg.games / 2;
@@ -81,34 +78,10 @@ NetMatchResult runNetMatch(const NetMatchConfig &cfg, const float *params) {
     }
   }
 
-  // leaf scorers: player 1's win probability per leaf
+  // leaf scorers: player 1's win probability per leaf, or playout winners
   std::vector<float> p1;
-  std::vector<mi_pos> packed;
-  std::vector<float> values;
-  std::vector<uint32_t> evalIdx;
   auto netEval = [&](const std::vector<State> &batch, const std::vector<size_t> &) {
-    p1.assign(batch.size(), 0.5f);
-    packed.clear();
-    evalIdx.clear();
-    for (size_t i = 0; i < batch.size(); i++) {
-      const State &s = batch[i];
-      if (s.isGameOver()) {
-        const PlayerId w = s.getWinner();
-        p1[i] = w == PLAYER_1 ? 1.f : w == PLAYER_2 ? 0.f : 0.5f;
-      } else {
-        packed.push_back(s.pack());
-        evalIdx.push_back((uint32_t)i);
-      }
-    }
-    values.resize(packed.size());
-    for (size_t a = 0; a < packed.size(); a += cfg.netBatch) {
-      const uint32_t n = (uint32_t)std::min<size_t>(cfg.netBatch, packed.size() - a);
-      checkMi(mi_mlp_forward(net, packed.data() + a, n, values.data() + a), ctx, "mi_mlp_forward");
-    }
-    for (size_t j = 0; j < evalIdx.size(); j++) {
-      const float v = values[j];  // expected result for the side to move
-      p1[evalIdx[j]] = batch[evalIdx[j]].turn == PLAYER_1 ? 0.5f * (1.f + v) : 0.5f * (1.f - v);
-    }
+    net.p1WinProb(batch, p1);
   };
   std::vector<PlayerId> winners;
   auto playoutEval = [&](const std::vector<State> &batch, const std::vector<size_t> &) {

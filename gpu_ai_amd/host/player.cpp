@@ -21,9 +21,17 @@ MCTSPlayer::MCTSPlayer(unsigned initialNumPlayouts, float numPlayoutsScale, doub
   tree->setParallel(true);
 }
 
-MCTSPlayer::~MCTSPlayer() {
-  if (running) stop();
+MCTSPlayer::MCTSPlayer(unsigned initialNumPlayouts, float numPlayoutsScale, double timeout,
+                       std::unique_ptr<NetLeafEvaluator> net)
+    : initialNumPlayouts(initialNumPlayouts),
+      numPlayoutsScale(numPlayoutsScale),
+      timeout(timeout),
+      net(std::move(net)),
+      tree(std::make_shared<GameTree>(getStartingState())) {
+  tree->setParallel(true);
 }
+
+MCTSPlayer::~MCTSPlayer() { stop(); }
 
 Move MCTSPlayer::getMove(const State &state, bool verbose) {
   {
@@ -36,14 +44,19 @@ Move MCTSPlayer::getMove(const State &state, bool verbose) {
   }
   std::this_thread::sleep_for(std::chrono::duration<double>(timeout));
   std::lock_guard<std::mutex> lock(treeMutex);
+  if (workerError) std::rethrow_exception(workerError);  // the search failed on the device
   if (verbose) {
     std::cout << "Tree size: " << tree->getTotalTrials() << std::endl;
     for (int i = 0; i < NUM_PLAYERS; i++)
       std::cout << (PlayerId)i << " score: " << tree->getScore((PlayerId)i) << std::endl;
   }
-  if (!tree->isExpanded()) {  // too little search time to expand the root: grow it now
+  if (!tree->isExpanded() || tree->getTotalTrials() == 0) {
+    // too little search time for a first update (the worker's batch may be in flight, its root
+    // expanded but scoreless): drop that batch and grow the root here
+    updateCancelled = true;
     std::vector<State> leaves = tree->select(2 * (unsigned)tree->getMoves().size() + 2);
-    tree->update(playoutDriver->runPlayouts(leaves));
+    if (net) tree->updateValues(net->p1WinProb(leaves));
+    else tree->update(playoutDriver->runPlayouts(leaves));
   }
   return tree->getOptMove(state.turn);
 }
@@ -55,9 +68,22 @@ void MCTSPlayer::move(const Move &m) {
   updateCancelled = true;
 }
 
+// A game starts from the initial position (playGame, src/driver.cpp:177-183).  The reference
+// keeps the previous game's tree here, so the opponent's first move of a second game is applied
+// to the last game's final position; the tree is reset instead.
 void MCTSPlayer::start() {
+  if (workerThread.joinable()) {
+    {
+      std::lock_guard<std::mutex> lock(treeMutex);
+      if (running) return;
+    }
+    workerThread.join();  // a worker that stopped on an error
+  }
   std::lock_guard<std::mutex> lock(treeMutex);
   if (running) return;
+  tree = std::make_shared<GameTree>(getStartingState());
+  tree->setParallel(true);
+  workerError = nullptr;
   running = true;
   workerThread = std::thread(&MCTSPlayer::worker, this);
 }
@@ -65,11 +91,10 @@ void MCTSPlayer::start() {
 void MCTSPlayer::stop() {
   {
     std::lock_guard<std::mutex> lock(treeMutex);
-    if (!running) return;
     updateCancelled = true;
     running = false;
   }
-  workerThread.join();
+  if (workerThread.joinable()) workerThread.join();  // also after the worker stopped on an error
 }
 
 SearchStats MCTSPlayer::stats() const {
@@ -80,6 +105,16 @@ SearchStats MCTSPlayer::stats() const {
 // src/player.cpp:134-150: select under the lock, play out without it, update under it unless a
 // move arrived meanwhile.
 void MCTSPlayer::worker() {
+  try {
+    search();
+  } catch (...) {  // handed to the next getMove instead of terminating the process
+    std::lock_guard<std::mutex> lock(treeMutex);
+    workerError = std::current_exception();
+    running = false;
+  }
+}
+
+void MCTSPlayer::search() {
   std::unique_lock<std::mutex> lock(treeMutex);
   while (running) {
     unsigned numPlayouts = (unsigned)(tree->getTotalTrials() * numPlayoutsScale);
@@ -88,12 +123,16 @@ void MCTSPlayer::worker() {
     std::vector<State> leaves = t->select(numPlayouts);
     updateCancelled = false;
     lock.unlock();
-    std::vector<PlayerId> results = playoutDriver->runPlayouts(leaves);
+    std::vector<PlayerId> results;
+    std::vector<float> values;
+    if (net) net->p1WinProb(leaves, values);
+    else results = playoutDriver->runPlayouts(leaves);
     lock.lock();
     if (!updateCancelled && t == tree) {
-      tree->update(results);
+      if (net) tree->updateValues(values);
+      else tree->update(results);
       stats_.iterations++;
-      stats_.playouts += results.size();
+      stats_.playouts += leaves.size();
     }
     updateCancelled = false;
   }
@@ -115,6 +154,9 @@ std::unique_ptr<Player> getPlayer(std::string name, double timeout) {
   if (name == "mcts_optimal") return mcts(50, 0.004f, 7, "device_mi355x");
   if (name == "mcts_heuristic") return mcts(50, 0.004f, 7, "device_mi355x_heuristic");
   if (name == "mcts_device_mi355x") return mcts(4096, 0.05f, 7, "device_mi355x");
+  if (name == "mcts_device_mi355x_net")
+    return std::make_unique<MCTSPlayer>(4096, 0.05f, timeout >= 0 ? timeout : 7.0,
+                                        std::make_unique<NetLeafEvaluator>(0, valueNetPath()));
   if (name == "mcts_device_mi355x_heuristic")
     return mcts(4096, 0.05f, 7, "device_mi355x_heuristic");
   throw std::runtime_error("Unknown player type");

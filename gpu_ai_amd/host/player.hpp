@@ -3,6 +3,7 @@
 // game waits for its move.
 #pragma once
 
+#include <exception>
 #include <memory>
 #include <mutex>
 #include <random>
@@ -10,6 +11,7 @@
 #include <thread>
 
 #include "mcts.hpp"
+#include "netleaf.hpp"
 #include "playout.hpp"
 
 #define MCTS_INITIAL_NUM_PLAYOUTS 50
@@ -50,9 +52,14 @@ class MCTSPlayer : public Player {
   // src/player.cpp:99)
   MCTSPlayer(unsigned initialNumPlayouts, float numPlayoutsScale, double timeout,
              std::unique_ptr<PlayoutDriver> playoutDriver);
+  // Leaves scored by a value network instead of played out (no reference counterpart).
+  MCTSPlayer(unsigned initialNumPlayouts, float numPlayoutsScale, double timeout,
+             std::unique_ptr<NetLeafEvaluator> net);
   ~MCTSPlayer() override;
 
-  std::string getName() const override { return "mcts_" + playoutDriver->getName(); }
+  std::string getName() const override {
+    return net ? std::string("mcts_device_mi355x_net") : "mcts_" + playoutDriver->getName();
+  }
   Move getMove(const State &state, bool verbose = true) override;
   void move(const Move &) override;
   void start() override;
@@ -61,20 +68,24 @@ class MCTSPlayer : public Player {
 
  private:
   void worker();
+  void search();
 
   const unsigned initialNumPlayouts;
   const float numPlayoutsScale;
   const double timeout;
   const std::unique_ptr<PlayoutDriver> playoutDriver;
+  const std::unique_ptr<NetLeafEvaluator> net;
   std::shared_ptr<GameTree> tree;
   mutable std::mutex treeMutex;
   std::thread workerThread;
   bool updateCancelled = false;
   bool running = false;
+  std::exception_ptr workerError;
   SearchStats stats_;
 };
 
 // Name -> player.  Reference names (src/player.cpp:152-186) keep their (initial playouts, scale,
 // timeout) and run on the MI355X drivers; new names: mcts_device_mi355x,
-// mcts_device_mi355x_heuristic.  `timeout` < 0 keeps each name's own timeout.
+// mcts_device_mi355x_heuristic, and mcts_device_mi355x_net (leaves scored by the value network
+// of the checkpoint valueNetPath() names).  `timeout` < 0 keeps each name's own timeout.
 std::unique_ptr<Player> getPlayer(std::string name, double timeout = -1.0);

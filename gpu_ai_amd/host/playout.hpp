@@ -2,6 +2,8 @@
 // drivers that implement it through the C ABI of include/gpuai/mi_playout.h.
 #pragma once
 
+#include <atomic>
+
 #include <memory>
 #include <string>
 #include <vector>
@@ -49,7 +51,7 @@ class DeviceMI355XPlayoutDriver : public PlayoutDriver {
   mi_ctx *ctx_ = nullptr;
   uint32_t policy_;
   uint32_t seed_;
-  uint32_t epoch_ = 0;
+  std::atomic<uint32_t> epoch_{0};  // a player's worker and getMove may both launch
 };
 
 // Name -> driver.  The reference's driver names (src/playout.cpp:358-392) all select the MI355X

@@ -40,7 +40,9 @@ void usage(const char *argv0) {
                " [--timeout SECONDS] [--seed S]\n"
                "  players: random mcts mcts_host mcts_host_heuristic mcts_device_coarse"
                " mcts_device_multiple mcts_hybrid mcts_optimal mcts_heuristic"
-               " mcts_device_mi355x mcts_device_mi355x_heuristic\n"
+               " mcts_device_mi355x mcts_device_mi355x_heuristic mcts_device_mi355x_net\n"
+               "  (mcts_device_mi355x_net scores leaves with the value network in\n"
+               "   $GPUAI_VALUE_NET, default ./value.ckpt)\n"
                "  playout drivers: host host_heuristic device_single device_heuristic"
                " device_multiple device_coarse hybrid hybrid_heuristic optimal"
                " optimal_heuristic device_mi355x device_mi355x_heuristic\n";

@@ -50,3 +50,35 @@ def test_run_ai_modes():
     out = subprocess.run([exe, "-m", "playout_test", "-n", "10", "-1", "nope"], capture_output=True,
                          text=True, timeout=60)
     assert out.returncode == 1 and "Unknown playout type" in out.stdout
+
+
+def test_run_ai_net_leaf_player(tmp_path):
+    """mcts_device_mi355x_net: the run_ai player whose leaves the value network scores."""
+    import os
+    from gpu_ai_amd import host_api
+    from gpu_ai_amd.engine import Engine, ValueNet
+    exe = host_api.run_ai_path()
+    ckpt = str(tmp_path / "value.ckpt")
+    announce("value network: a few training steps, checkpoint")
+    with Engine(0) as eng:
+        pos = eng.start_positions(3, 2048, 0, 60, reject_terminal=True)
+        tallies, _ = eng.run_tally(pos, per_pos=32, seed=9)
+        p1 = pos["turn"] == 0
+        t = (np.where(p1, tallies[:, 1], tallies[:, 2]).astype(np.float32) -
+             np.where(p1, tallies[:, 2], tallies[:, 1])) / 32.0
+        with ValueNet(eng, 1024, seed=5) as net:
+            for k in range(4):
+                net.train_step(pos[k * 512:(k + 1) * 512], t[k * 512:(k + 1) * 512])
+            net.save(ckpt)
+    env = dict(os.environ, GPUAI_VALUE_NET=ckpt)
+    announce("run_ai game_test mcts_device_mi355x_net vs random")
+    out = subprocess.run([exe, "-m", "game_test", "-n", "2", "-1", "mcts_device_mi355x_net", "-2",
+                          "random", "--timeout", "0.02"], capture_output=True, text=True,
+                         timeout=300, env=env)
+    assert out.returncode == 0 and "=== Results ===" in out.stdout, out.stdout + out.stderr
+    announce("run_ai with a missing checkpoint")
+    env["GPUAI_VALUE_NET"] = str(tmp_path / "missing.ckpt")
+    out = subprocess.run([exe, "-m", "game_test", "-n", "1", "-1", "mcts_device_mi355x_net", "-2",
+                          "random", "--timeout", "0.02"], capture_output=True, text=True,
+                         timeout=120, env=env)
+    assert out.returncode != 0 and "cannot load the value network" in out.stdout + out.stderr
