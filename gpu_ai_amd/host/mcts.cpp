@@ -92,25 +92,34 @@ void GameTree::selectInto(unsigned trials, std::vector<State> &out) {
     share[i] = 0;
     topped[i] = false;
   }
+  // The reference divides by the weight sum even when it is 0 (every child scoreless and the
+  // parent on its first trial: ln 1 = 0), which leaves 0/0 shares and more remainder than its
+  // one-extra-trial-per-child loop can place (src/mcts.cpp:106-138 asserts there).  Such a node
+  // splits evenly here; wherever the reference's shares are defined the result is the same.
+  const bool proportional = weightSum > 0.0 && std::isfinite(weightSum);
   assignedTrials = 0;
   for (size_t i = 0; i < n; i++) {
     if (untried > 0)  // untried children share the batch evenly
       share[i] = children[i]->totalTrials == 0 ? trials / untried : 0;
     else if (trials > 0)  // proportional to UCB1, rounded down
-      share[i] = (unsigned)(trials * (weight[i] / weightSum));
+      share[i] = proportional ? (unsigned)(trials * (weight[i] / weightSum)) : trials / (unsigned)n;
     assignedTrials += share[i];
   }
   // the remainder goes one trial each to the children of largest weight, first index first
+  // (a second round is only ever needed for the degenerate weights above)
   while (assignedTrials < trials) {
     double maxWeight = -INFINITY;
     int pick = -1;
     for (size_t i = 0; i < n; i++) {
-      if (!topped[i] && weight[i] > maxWeight) {
+      if (!topped[i] && !(weight[i] <= maxWeight)) {
         maxWeight = weight[i];
         pick = (int)i;
       }
     }
-    if (pick < 0) throw std::runtime_error("trial allocation failed");
+    if (pick < 0) {
+      for (size_t i = 0; i < n; i++) topped[i] = false;
+      continue;
+    }
     topped[pick] = true;
     share[pick]++;
     assignedTrials++;
