@@ -269,8 +269,9 @@ def main() -> int:
     }
     # the same batch through mi_run_playouts with host buffers (positions in, winners out, over
     # PCIe): reported beside the figure, never as it
-    hb = eng.run_playouts(pos, per_pos=cfg["per_pos"], seed=PLAY_SEED, epoch=5000,
-                          policy=cfg["policy"])
+    for e in range(3):  # the first call with host buffers also sets up HIP's staging copies
+        hb = eng.run_playouts(pos, per_pos=cfg["per_pos"], seed=PLAY_SEED, epoch=5000 + e,
+                              policy=cfg["policy"])
     line["host_buffers"] = {"total_ms": round(hb.stats["total_ms"], 4),
                             "kernel_ms": round(hb.stats["kernel_ms"], 4),
                             "playouts_per_s": round(total / (hb.stats["total_ms"] / 1e3), 1)}
