@@ -95,6 +95,8 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--positions", type=int, default=2_000_000)
     ap.add_argument("--playouts", type=int, default=256)
+    ap.add_argument("--kmax", type=int, default=90,
+                    help="training positions: k uniform in [0, kmax] random plies from the start")
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -113,9 +115,10 @@ def main(argv=None) -> int:
             net.load(args.load)
         else:
             t0 = time.perf_counter()
-            pos, t = make_dataset(eng, args.positions, args.seed, playouts=args.playouts)
+            pos, t = make_dataset(eng, args.positions, args.seed, kmax=args.kmax,
+                                  playouts=args.playouts)
             t_label = time.perf_counter() - t0
-            print(json.dumps({"labelled": len(pos), "playouts": args.playouts,
+            print(json.dumps({"labelled": len(pos), "playouts": args.playouts, "kmax": args.kmax,
                               "seconds": t_label}), flush=True)
             train(net, pos, t, args.epochs, args.batch, args.lr,
                   log=lambda h: print(json.dumps(h), flush=True))
