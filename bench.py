@@ -106,17 +106,19 @@ def bench_multisearch() -> int:
 
 
 def recorded_match():
-    """The network-leaf vs playout-leaf match result recorded by `python -m gpu_ai_amd.value`
-    (400 games, 250 ms per move; minutes of GPU time, so it is read, not rerun, here)."""
+    """The network-leaf vs playout-leaf match recorded by `python -m gpu_ai_amd.value` (400
+    games, 250 ms per move; minutes of GPU time, so it is read, not rerun, here): the first run
+    of the table in profiles/value_net_match_400x250ms.md."""
     path = os.path.join(ROOT, "profiles", "value_net_match_400x250ms.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"network score\*\* \(\(wins \+ draws/2\) / games\) \| \*\*([0-9.]+) \+- ([0-9.]+)",
-                  open(path).read())
+    m = re.search(r"^\| ([A-Z]) \| ([^|]*)\|.*\*\*([0-9.]+) \+- ([0-9.]+)\*\*", open(path).read(),
+                  flags=re.M)
     if not m:
         return None
-    return {"score": float(m.group(1)), "stderr": float(m.group(2)), "games": 400,
-            "move_ms": 250, "source": "profiles/value_net_match_400x250ms.md"}
+    return {"score": float(m.group(3)), "stderr": float(m.group(4)), "games": 400,
+            "move_ms": 250, "training": m.group(2).strip(),
+            "source": f"profiles/value_net_match_400x250ms.md, run {m.group(1)}"}
 
 
 # FLOPs of one training step of the 128-256-256-1 network on a batch of B positions: forward

@@ -8,7 +8,7 @@ Its error is measured against independent 1024-playout estimates of the same pos
 the error of k-playout estimates (k = 1 is what the reference's MCTS takes per leaf,
 src/mcts.cpp:57-61), which prices one network evaluation in playouts.
 
-    python -m gpu_ai_amd.value --positions 2000000 --epochs 8 --out gpurun_out/value.ckpt
+    python -m gpu_ai_amd.value --positions 2000000 --epochs 24 --out gpurun_out/value.ckpt
 """
 from __future__ import annotations
 
@@ -31,7 +31,7 @@ def mover_value(pos: np.ndarray, tallies: np.ndarray) -> np.ndarray:
     return ((wins - losses) / n).astype(np.float32)
 
 
-def make_dataset(engine: Engine, n: int, seed: int, kmin: int = 0, kmax: int = 90,
+def make_dataset(engine: Engine, n: int, seed: int, kmin: int = 0, kmax: int = 150,
                  playouts: int = 256, label_seed: int = 1, chunk: int = 1 << 18):
     """n positions (k uniform in [kmin, kmax] random plies, finished games rejected) and their
     playout labels."""
@@ -95,9 +95,9 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--positions", type=int, default=2_000_000)
     ap.add_argument("--playouts", type=int, default=256)
-    ap.add_argument("--kmax", type=int, default=90,
+    ap.add_argument("--kmax", type=int, default=150,
                     help="training positions: k uniform in [0, kmax] random plies from the start")
-    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=24)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=2016)
