@@ -530,18 +530,49 @@ CK_HD Choice select_in_walk(const Lane &L, const Gen &g, uint32_t s, uint32_t k)
   return c;
 }
 
+// The first four capture sequences of the multi-jump origins (in walk order): landing square
+// and removed pieces, held in registers (fixed slots, selected without dynamic indexing) so that
+// choosing one of them needs no second walk.
+struct LeafCache {
+  uint32_t to0, to1, to2, to3;
+  uint32_t rm0, rm1, rm2, rm3;
+};
+
+CK_HD void cache_put(LeafCache &c, uint32_t i, uint32_t to, uint32_t rm) {
+  c.to0 = i == 0u ? to : c.to0;
+  c.rm0 = i == 0u ? rm : c.rm0;
+  c.to1 = i == 1u ? to : c.to1;
+  c.rm1 = i == 1u ? rm : c.rm1;
+  c.to2 = i == 2u ? to : c.to2;
+  c.rm2 = i == 2u ? rm : c.rm2;
+  c.to3 = i == 3u ? to : c.to3;
+  c.rm3 = i == 3u ? rm : c.rm3;
+}
+
+CK_HD void cache_get(const LeafCache &c, uint32_t i, uint32_t &to, uint32_t &rm) {
+  to = i == 0u ? c.to0 : i == 1u ? c.to1 : i == 2u ? c.to2 : c.to3;
+  rm = i == 0u ? c.rm0 : i == 1u ? c.rm1 : i == 2u ? c.rm2 : c.rm3;
+}
+
 // Move count of a capture ply with multi-jump origins: single-jump origins contribute their slot
 // bits, each multi-jump origin its walked leaf count.  The first four such counts are returned
-// packed one per byte (in square order) for the selection pass.
-CK_HD uint32_t count_mixed(const Lane &L, const Gen &g, uint32_t &packed) {
+// packed one per byte (in square order) for the selection pass, the first four leaves in `lc`.
+CK_HD uint32_t count_mixed(const Lane &L, const Gen &g, uint32_t &packed, LeafCache &lc) {
   uint32_t n = popc(g.slot[0] & ~g.multi) + popc(g.slot[1] & ~g.multi) +
                popc(g.slot[2] & ~g.multi) + popc(g.slot[3] & ~g.multi);
-  uint32_t o = g.multi, i = 0;
+  uint32_t o = g.multi, i = 0, m = 0;
   packed = 0;
   while (o) {
     const uint32_t s = ctz(o);
     o &= o - 1u;
-    const uint32_t c = count_walk(L, g, s);
+    uint32_t c = 0;
+    auto leaf = [&](uint32_t to, uint32_t removed, uint32_t, uint32_t) {
+      if (m < 4u) cache_put(lc, m, to, removed);
+      m++;
+      c++;
+      return false;
+    };
+    walk(s, getbit(L.kings, s), g.cap, leaf);
     if (i < 4u) packed |= (c < 255u ? c : 255u) << (8u * i);
     i++;
     n += c;
@@ -550,7 +581,8 @@ CK_HD uint32_t count_mixed(const Lane &L, const Gen &g, uint32_t &packed) {
 }
 
 // k-th move (mover-frame order) of a capture ply with multi-jump origins.
-CK_HD Choice select_mixed(const Lane &L, const Gen &g, uint32_t k, uint32_t packed) {
+CK_HD Choice select_mixed(const Lane &L, const Gen &g, uint32_t k, uint32_t packed,
+                          const LeafCache &lc) {
   uint32_t sm[4] = {g.slot[0] & ~g.multi, g.slot[1] & ~g.multi, g.slot[2] & ~g.multi,
                     g.slot[3] & ~g.multi};
   uint32_t o = g.multi, acc = 0, i = 0;
@@ -562,7 +594,16 @@ CK_HD Choice select_mixed(const Lane &L, const Gen &g, uint32_t k, uint32_t pack
                            popc(sm[2] & below) + popc(sm[3] & below);
     if (k < start) break;
     const uint32_t c = i < 4u ? (packed >> (8u * i)) & 255u : count_walk(L, g, s);
-    if (k < start + c) return select_in_walk(L, g, s, k - start);
+    if (k < start + c) {
+      const uint32_t ci = acc + (k - start);  // index among all multi-jump leaves
+      if (ci < 4u) {
+        Choice ch;
+        ch.from = s;
+        cache_get(lc, ci, ch.to, ch.removed);
+        return ch;
+      }
+      return select_in_walk(L, g, s, k - start);
+    }
     acc += c;
     i++;
   }
@@ -593,7 +634,8 @@ CK_HD PlyResult ply_random(Lane &L, uint32_t r, uint32_t *list_dig) {
   analyse(L, g);
   PlyResult res;
   uint32_t packed = 0;
-  uint32_t n = g.multi ? count_mixed(L, g, packed) : slot_count(g.slot);
+  LeafCache lc = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  uint32_t n = g.multi ? count_mixed(L, g, packed, lc) : slot_count(g.slot);
   res.n = n;
   if (TRACE) *list_dig = list_digest(L, g, n);
   if (n == 0u || L.counter >= 50u) {  // src/state.cpp:16-23
@@ -609,7 +651,7 @@ CK_HD PlyResult ply_random(Lane &L, uint32_t r, uint32_t *list_dig) {
   uint32_t km = L.turn ? n - 1u - k : k;
   Choice c;
   if (g.multi) {
-    c = select_mixed(L, g, km, packed);
+    c = select_mixed(L, g, km, packed, lc);
   } else {
     uint32_t sq, sl;
     select_slot(g.slot, km, sq, sl);
