@@ -45,6 +45,7 @@ constexpr uint32_t EVEN_ROWS = 0x0F0F0F0Fu;  // rows 0, 2, 4, 6
 constexpr uint32_t ODD_ROWS = 0xF0F0F0F0u;   // rows 1, 3, 5, 7
 constexpr uint32_t I0 = 0x11111111u;         // i == 0
 constexpr uint32_t I3 = 0x88888888u;         // i == 3
+constexpr uint32_t ROW5 = 0x00F00000u;       // a man's jump from here lands on the far row
 constexpr uint32_t ROW6 = 0x0F000000u;
 constexpr uint32_t ROW7 = 0xF0000000u;       // the mover's far row (promotion)
 constexpr uint32_t FAR_ROW_FIRST = 28u;
@@ -708,7 +709,58 @@ CK_HD PlyResult ply_heuristic(Lane &L, uint32_t r, uint32_t *list_dig) {
   const uint32_t so = material(L.opp, L.kings);
   uint32_t n = 0, ties = 0, bn = 0, bd = 1;
   uint32_t tslot[4];
-  if (g.capture) {
+  const bool walked = g.capture && g.multi;  // capture sequences longer than one jump exist
+  if (g.capture && !g.multi) {
+    // Single jumps only: a move's weight depends only on whether it removes a king and whether
+    // a man lands on the far row, so there are four classes (sm + 3p) / (so - (k ? 4 : 1)),
+    // each a set of slot bits.  The tied best classes give the slots to choose from.
+    n = slot_count(g.slot);
+    const uint32_t men = L.own & ~L.kings, kng = L.own & L.kings;
+    const uint32_t okings = L.opp & L.kings;
+    // origins whose jump in direction d removes a king (the piece one step away in d)
+    const uint32_t kd0 = g.cap[0] & step3(okings), kd1 = g.cap[1] & step2(okings);
+    const uint32_t kd2 = g.cap[2] & step1(okings), kd3 = g.cap[3] & step0(okings);
+    // per slot (a man's slot 0 jumps in direction 1, its slot 1 in direction 0)
+    const uint32_t kc[4] = {g.slot[0] & ((men & kd1) | (kng & kd0)),
+                            g.slot[1] & ((men & kd0) | (kng & kd1)), g.slot[2] & kd2,
+                            g.slot[3] & kd3};
+    const uint32_t pr[4] = {g.slot[0] & men & ROW5, g.slot[1] & men & ROW5, 0u, 0u};
+    uint32_t any[4] = {0u, 0u, 0u, 0u};  // class c = 2 * promotion + king removed
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      any[0] |= g.slot[t] & ~kc[t] & ~pr[t];
+      any[1] |= kc[t] & ~pr[t];
+      any[2] |= pr[t] & ~kc[t];
+      any[3] |= kc[t] & pr[t];
+    }
+    uint32_t cn[4], cd[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      cn[c] = sm + ((c & 2) ? 3u : 0u);
+      cd[c] = so - ((c & 1) ? 4u : 1u);  // a class that occurs has that piece to remove
+    }
+    bool have = false;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (any[c] && (!have || cmp_weight(cn[c], cd[c], bn, bd) > 0)) {
+        bn = cn[c];
+        bd = cd[c];
+        have = true;
+      }
+    }
+    tslot[0] = tslot[1] = tslot[2] = tslot[3] = 0u;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (any[c] && cmp_weight(cn[c], cd[c], bn, bd) == 0) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const uint32_t m = ((c & 1) ? kc[t] : g.slot[t] & ~kc[t]) & ((c & 2) ? pr[t] : ~pr[t]);
+          tslot[t] |= m;
+        }
+      }
+    }
+    ties = slot_count(tslot);
+  } else if (g.capture) {
     uint32_t o = g.origins;
     uint32_t king = 0;
     auto leaf = [&](uint32_t to, uint32_t removed, uint32_t, uint32_t) {
@@ -756,7 +808,7 @@ CK_HD PlyResult ply_heuristic(Lane &L, uint32_t r, uint32_t *list_dig) {
   res.over = 0u;
   res.winner = 0;
   Choice c = {0u, 0u, 0u};
-  if (g.capture) {
+  if (walked) {
     uint32_t o = g.origins;
     uint32_t king = 0, left = km, idx = 0;
     bool found = false;
