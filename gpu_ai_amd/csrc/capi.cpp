@@ -479,7 +479,6 @@ int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
        for (uint32_t dm : defers) {
         const uint32_t waves_per_cu = 4u * w;           // four SIMDs per CU
         if ((b / 64u) > waves_per_cu) continue;          // a block larger than the residency
-        if (policy != MI_POLICY_RANDOM && dm != 8u) continue;  // deferral is a random-policy knob
         mi_tuning t = {b, waves_per_cu * 64u / b, refill, dm};
         ctx->tuning = t;
         std::vector<float> ms;

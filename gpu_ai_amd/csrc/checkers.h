@@ -698,19 +698,17 @@ CK_HD PlyResult ply_random_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool &d
   return res;
 }
 
-// Plays one ply of the heuristic policy with word r (src/heuristicPlayout.cpp:20-42 without
-// the noise; ties broken by T[r mod |T|]).
-template <bool TRACE>
-CK_HD PlyResult ply_heuristic(Lane &L, uint32_t r, uint32_t *list_dig) {
-  Gen g;
-  analyse(L, g);
+// The heuristic ply after analyse.  WALK = false promises that no capture sequence is longer
+// than one jump (the caller deferred such plies), which drops the tree walks from the code.
+template <bool TRACE, bool WALK>
+CK_HD PlyResult heuristic_body(Lane &L, const Gen &g, uint32_t r, uint32_t *list_dig) {
   PlyResult res;
   const uint32_t sm = material(L.own, L.kings);
   const uint32_t so = material(L.opp, L.kings);
   uint32_t n = 0, ties = 0, bn = 0, bd = 1;
   uint32_t tslot[4];
-  const bool walked = g.capture && g.multi;  // capture sequences longer than one jump exist
-  if (g.capture && !g.multi) {
+  const bool walked = WALK && g.capture && g.multi;  // sequences longer than one jump exist
+  if (g.capture && !walked) {
     // Single jumps only: a move's weight depends only on whether it removes a king and whether
     // a man lands on the far row, so there are four classes (sm + 3p) / (so - (k ? 4 : 1)),
     // each a set of slot bits.  The tied best classes give the slots to choose from.
@@ -853,6 +851,34 @@ CK_HD PlyResult ply_heuristic(Lane &L, uint32_t r, uint32_t *list_dig) {
   }
   apply(L, c.from, c.to, c.removed);
   return res;
+}
+
+// Plays one ply of the heuristic policy with word r (src/heuristicPlayout.cpp:20-42 without
+// the noise; ties broken by T[r mod |T|]).
+template <bool TRACE>
+CK_HD PlyResult ply_heuristic(Lane &L, uint32_t r, uint32_t *list_dig) {
+  Gen g;
+  analyse(L, g);
+  return heuristic_body<TRACE, true>(L, g, r, list_dig);
+}
+
+// The common part of a heuristic ply: plays it unless it is a capture ply with a sequence
+// longer than one jump, which is left untouched with `deferred` set (the caller plays it later
+// with ply_heuristic and the same word).
+template <bool TRACE>
+CK_HD PlyResult ply_heuristic_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool &deferred) {
+  Gen g;
+  analyse(L, g);
+  deferred = g.capture && g.multi;
+  if (deferred) {
+    PlyResult res;
+    res.n = 0u;
+    res.chosen = 0u;
+    res.over = 0u;
+    res.winner = 0;
+    return res;
+  }
+  return heuristic_body<TRACE, false>(L, g, r, list_dig);
 }
 
 // True when the position is finished (no legal move, or the draw counter reached 50).

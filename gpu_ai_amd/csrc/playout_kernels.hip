@@ -179,19 +179,15 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
     uint32_t ld = 0;
     if (active && !deferred) {
       const uint32_t r = take_word(w, ply);
-      if (POLICY == MI_POLICY_RANDOM) {
-        // a deferred ply takes the same word again in the walk phase (take_word is idempotent
-        // for a given ply)
-        bool d = false;
-        res = ck::ply_random_fast<TRACE>(L, r, &ld, d);
-        deferred = d;
-        play = !d;
-      } else {
-        res = ck::ply_heuristic<TRACE>(L, r, &ld);
-        play = true;
-      }
+      // a deferred ply takes the same word again in the walk phase (take_word is idempotent
+      // for a given ply)
+      bool d = false;
+      if (POLICY == MI_POLICY_RANDOM) res = ck::ply_random_fast<TRACE>(L, r, &ld, d);
+      else res = ck::ply_heuristic_fast<TRACE>(L, r, &ld, d);
+      deferred = d;
+      play = !d;
     }
-    if (POLICY == MI_POLICY_RANDOM) {
+    {
       const uint64_t dm = __ballot(deferred);
       if (dm) {
         const uint32_t nd = (uint32_t)__popcll(dm);
@@ -200,7 +196,8 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
           since_walk = 0;
           if (deferred) {
             const uint32_t r = take_word(w, ply);
-            res = ck::ply_random<TRACE>(L, r, &ld);
+            if (POLICY == MI_POLICY_RANDOM) res = ck::ply_random<TRACE>(L, r, &ld);
+            else res = ck::ply_heuristic<TRACE>(L, r, &ld);
             deferred = false;
             play = true;
           }

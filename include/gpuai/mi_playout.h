@@ -68,7 +68,7 @@ typedef struct mi_tuning {
   uint32_t block_size;      /* threads per workgroup: 64..1024, a multiple of 64 */
   uint32_t blocks_per_cu;   /* resident workgroups per compute unit for the persistent grid */
   uint32_t refill;          /* MI_REFILL_* */
-  uint32_t defer_min;       /* random policy: a wave plays its parked multi-jump plies once this
+  uint32_t defer_min;       /* a wave plays its parked multi-jump plies (both policies) once this
                                many lanes wait (or all active lanes wait, or 8 iterations pass) */
 } mi_tuning;
 

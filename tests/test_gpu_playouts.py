@@ -122,15 +122,16 @@ def test_random_64x4096_bitexact(engine, oracle, golden):
 @pytest.mark.parametrize("block,bpc,refill,defer", [(64, 1, REFILL_STATIC, 1), (64, 16, REFILL_QUEUE, 16),
                                                     (256, 8, REFILL_STATIC, 8), (1024, 2, REFILL_QUEUE, 1),
                                                     (512, 4, REFILL_QUEUE, 64)])
-def test_launch_shape_does_not_change_results(engine, golden, block, bpc, refill, defer):
+@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
+def test_launch_shape_does_not_change_results(engine, golden, block, bpc, refill, defer, policy):
     pos = golden["positions"]["mid1024"][:256]
-    announce(f"tuning block {block} blocks/CU {bpc} refill {refill} defer {defer}")
-    base = engine.run_traced(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
-    fast = engine.run_playouts(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
+    announce(f"tuning block {block} blocks/CU {bpc} refill {refill} defer {defer} policy {policy}")
+    base = engine.run_traced(pos, per_pos=256, seed=7, epoch=2, policy=policy)
+    fast = engine.run_playouts(pos, per_pos=256, seed=7, epoch=2, policy=policy)
     engine.set_tuning(block, bpc, refill, defer)
     try:
-        other = engine.run_traced(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
-        other_fast = engine.run_playouts(pos, per_pos=256, seed=7, epoch=2, policy=POLICY_RANDOM)
+        other = engine.run_traced(pos, per_pos=256, seed=7, epoch=2, policy=policy)
+        other_fast = engine.run_playouts(pos, per_pos=256, seed=7, epoch=2, policy=policy)
         assert other.stats["block_size"] == block and other.stats["refill"] == refill
     finally:
         engine.set_tuning()
