@@ -134,6 +134,9 @@ int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch
     out->max_ms = r.maxMs;
     out->playouts_per_second = r.playoutsPerSecond;
     out->root_digest = r.rootDigest;
+    out->select_ms = r.selectMs;
+    out->playout_ms = r.playoutMs;
+    out->update_ms = r.updateMs;
     return 0;
   } catch (const std::exception &e) {
     if (err && err_len) {

@@ -5,9 +5,16 @@
 #include <cmath>
 #include <stdexcept>
 
-GameTree::GameTree(State s, GameTree *parent) : state(s), parent(parent) {
-  moves = s.getMoves();
-  over = moves.empty() || s.movesSinceLastCapture >= NUM_DRAW_MOVES;  // src/state.cpp:16-18
+// The move list is generated when the node is expanded (most nodes never are); whether the game
+// is over needs only the bitboard test of State::isGameOver (src/state.cpp:16-18).
+GameTree::GameTree(State s, GameTree *parent) : state(s), parent(parent) { over = s.isGameOver(); }
+
+const std::vector<Move> &GameTree::getMoves() const {
+  if (!movesReady) {
+    moves = state.getMoves();
+    movesReady = true;
+  }
+  return moves;
 }
 
 GameTree::~GameTree() {
@@ -66,8 +73,9 @@ void GameTree::selectInto(unsigned trials, std::vector<State> &out) {
   }
   if (!expanded) {
     if (trials > 1) {
-      children.reserve(moves.size());
-      for (const Move &m : moves) {
+      const std::vector<Move> &mv = getMoves();
+      children.reserve(mv.size());
+      for (const Move &m : mv) {
         State next = state;
         next.move(m);
         children.push_back(std::make_shared<GameTree>(next, this));
@@ -85,8 +93,9 @@ void GameTree::selectInto(unsigned trials, std::vector<State> &out) {
   bool topped[MAX_MOVES];
   double weightSum = 0;
   unsigned untried = 0;
+  const double lnN = std::log((double)totalTrials);  // the parent's term, once per node
   for (size_t i = 0; i < n; i++) {
-    weight[i] = children[i]->ucb1();
+    weight[i] = children[i]->ucb1(lnN);
     weightSum += weight[i];
     if (children[i]->totalTrials == 0) untried++;
     share[i] = 0;
@@ -124,55 +133,80 @@ void GameTree::selectInto(unsigned trials, std::vector<State> &out) {
     share[pick]++;
     assignedTrials++;
   }
+  // The reference recurses into every child, also with 0 trials, which only zeroes the assigned
+  // counts of that child's whole subtree; zeroing the child's own count is enough here, since
+  // update never descends below a child with none.
   if (!parallel) {
-    for (size_t i = 0; i < n; i++) children[i]->selectInto(share[i], out);
+    for (size_t i = 0; i < n; i++) {
+      if (share[i]) children[i]->selectInto(share[i], out);
+      else children[i]->assignedTrials = 0;
+    }
     return;
   }
   std::vector<std::vector<State>> parts(n);
 #pragma omp parallel for
   for (long i = 0; i < (long)n; i++) {
+    if (!share[i]) {
+      children[i]->assignedTrials = 0;
+      continue;
+    }
     parts[i].reserve(share[i]);
     children[i]->selectInto(share[i], parts[i]);
   }
   for (auto &p : parts) out.insert(out.end(), p.begin(), p.end());
 }
 
+// Each node gains the results of the trials that passed through it (src/mcts.cpp:159-180).  The
+// sums are formed once at the leaves and handed up, instead of every node re-adding all of its
+// results; for whole-game results they are integers, so the totals are exactly the reference's.
 template <class R, class Add>
-void GameTree::updateImpl(const R *results, size_t count, Add add) {
+void GameTree::updateImpl(const R *results, size_t count, Add add, double sums[NUM_PLAYERS]) {
   if (count != assignedTrials)
     throw std::runtime_error("update: result count differs from the trials handed out");
   totalTrials += assignedTrials;
-  if (expanded) {
+  double local[NUM_PLAYERS] = {0.0, 0.0};
+  if (expanded) {  // the trials were split among the children, in child order
     size_t at = 0;
     for (auto &c : children) {
-      c->updateImpl(results + at, c->assignedTrials, add);
+      if (c->assignedTrials == 0) continue;
+      c->updateImpl(results + at, c->assignedTrials, add, local);
       at += c->assignedTrials;
     }
+  } else {
+    for (size_t i = 0; i < count; i++) add(local, results[i]);
   }
-  for (size_t i = 0; i < count; i++) add(wins, results[i]);
+  for (int p = 0; p < NUM_PLAYERS; p++) {
+    wins[p] += local[p];
+    sums[p] += local[p];
+  }
 }
 
 void GameTree::update(const std::vector<PlayerId> &results) { update(results.data(), results.size()); }
 
 void GameTree::update(const PlayerId *results, size_t n) {
+  double sums[NUM_PLAYERS] = {0.0, 0.0};
   updateImpl(results, n, [](double *w, PlayerId r) {
     if (r != PLAYER_NONE) w[r] += 1.0;
-  });
+  }, sums);
 }
 
 void GameTree::updateValues(const std::vector<float> &p) { updateValues(p.data(), p.size()); }
 
 void GameTree::updateValues(const float *p1WinProb, size_t n) {
+  double sums[NUM_PLAYERS] = {0.0, 0.0};
   updateImpl(p1WinProb, n, [](double *w, float p) {
     w[PLAYER_1] += p;
     w[PLAYER_2] += 1.0 - p;
-  });
+  }, sums);
 }
 
 // The reference's arithmetic, including its long-double exploration term: the win rate of the
 // side to move in this child (src/mcts.cpp:189) plus sqrt(2 ln N / n).
-double GameTree::ucb1() const {
+double GameTree::ucb1() const { return ucb1(std::log((double)parent->totalTrials)); }
+
+// lnParent = log(parent's trials) in double precision, as log(unsigned) evaluates in the
+// reference; the rest in long double as there.
+double GameTree::ucb1(double lnParent) const {
   if (totalTrials == 0) return INFINITY;
-  return (double)wins[state.turn] / totalTrials +
-         std::sqrt(2.0L * std::log(parent->totalTrials) / totalTrials);
+  return (double)wins[state.turn] / totalTrials + std::sqrt(2.0L * lnParent / totalTrials);
 }

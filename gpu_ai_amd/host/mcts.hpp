@@ -45,21 +45,23 @@ class GameTree {
 
   // introspection
   const std::vector<std::shared_ptr<GameTree>> &getChildren() const { return children; }
-  const std::vector<Move> &getMoves() const { return moves; }
+  const std::vector<Move> &getMoves() const;  // generated on first use
   double getWins(PlayerId p) const { return wins[p]; }
   unsigned getAssignedTrials() const { return assignedTrials; }
   bool isExpanded() const { return expanded; }
 
  private:
   double ucb1() const;  // src/mcts.cpp:182-191
+  double ucb1(double lnParent) const;
   template <class R, class Add>
-  void updateImpl(const R *results, size_t n, Add add);
+  void updateImpl(const R *results, size_t n, Add add, double sums[NUM_PLAYERS]);
 
   const GameTree *parent;
   bool expanded = false;
   bool over = false;       // isGameOver() of the node's state, fixed at construction
   bool parallel = false;
-  std::vector<Move> moves;
+  mutable std::vector<Move> moves;
+  mutable bool movesReady = false;
   std::vector<std::shared_ptr<GameTree>> children;
   unsigned assignedTrials = 0;
   unsigned totalTrials = 0;

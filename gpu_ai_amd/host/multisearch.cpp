@@ -85,6 +85,9 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
     bool any = false;
     for (uint32_t s = 0; s < S; s++) any |= remaining[s] > 0;
     if (!any) break;
+    auto ms_since = [](Clock::time_point a) {
+      return std::chrono::duration<double, std::milli>(Clock::now() - a).count();
+    };
 #pragma omp parallel for schedule(dynamic, 4)
     for (long s = 0; s < (long)S; s++) {
       leaves[s].clear();
@@ -94,6 +97,8 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
       n = std::min<unsigned>(n, remaining[s]);
       leaves[s] = trees[s]->select(n);
     }
+    res.selectMs += ms_since(t0);
+    const auto t1 = Clock::now();
     uint64_t iterPlayouts = 0;
     if (cfg.mode == MultiSearchMode::PER_SEARCH) {
       for (uint32_t s = 0; s < S; s++) {
@@ -132,12 +137,15 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
         for (size_t i = 0; i < leaves[s].size(); i++) results[s][i] = (PlayerId)winners[at++];
       }
     }
+    res.playoutMs += ms_since(t1);
+    const auto t2 = Clock::now();
 #pragma omp parallel for schedule(dynamic, 4)
     for (long s = 0; s < (long)S; s++) {
       if (leaves[s].empty()) continue;
       trees[s]->update(results[s]);
       remaining[s] -= (uint32_t)leaves[s].size();
     }
+    res.updateMs += ms_since(t2);
     res.playouts += iterPlayouts;
     res.iterations++;
     res.iterationMs.push_back(std::chrono::duration<double, std::milli>(Clock::now() - t0).count());

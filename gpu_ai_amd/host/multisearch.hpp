@@ -38,6 +38,9 @@ struct MultiSearchResult {
   double wallSeconds = 0;
   double deviceMs = 0;           // summed device time of the playout launches
   double p50Ms = 0, p99Ms = 0, maxMs = 0;  // per-iteration wall latency
+  // wall time by phase, summed over iterations: tree selection, playouts (packing, launch,
+  // results), tree update
+  double selectMs = 0, playoutMs = 0, updateMs = 0;
   double playoutsPerSecond = 0;
   uint64_t rootDigest = 0;       // digest of every root's statistics (determinism checks)
   std::vector<double> iterationMs;

@@ -27,7 +27,8 @@ class MultiSearchResult(ctypes.Structure):
                 ("wall_seconds", ctypes.c_double), ("device_ms", ctypes.c_double),
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("max_ms", ctypes.c_double), ("playouts_per_second", ctypes.c_double),
-                ("root_digest", ctypes.c_uint64)]
+                ("root_digest", ctypes.c_uint64), ("select_ms", ctypes.c_double),
+                ("playout_ms", ctypes.c_double), ("update_ms", ctypes.c_double)]
 
 
 class NetMatchConfig(ctypes.Structure):

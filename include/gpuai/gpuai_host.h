@@ -55,6 +55,7 @@ typedef struct gpuai_multisearch_result {
   uint32_t pad;
   double wall_seconds, device_ms, p50_ms, p99_ms, max_ms, playouts_per_second;
   uint64_t root_digest;
+  double select_ms, playout_ms, update_ms;  /* wall time by phase, summed over iterations */
 } gpuai_multisearch_result;
 
 /* Runs the multi-search on device 0; returns 0, or -1 with a message in `err` (err_len bytes). */
