@@ -633,6 +633,9 @@ struct mi_graph {
   hipGraphExec_t exec = nullptr;
   hipGraphNode_t n_h2d = nullptr, n_reset = nullptr, n_play = nullptr, n_d2h = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // a replay launched by mi_graph_launch and not yet finished
+  bool pending = false;
+  uint64_t pending_total = 0;
 };
 
 namespace {
@@ -726,43 +729,56 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
   return MI_OK;
 }
 
-int mi_graph_run(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
-                 uint32_t seed, uint32_t epoch, int8_t *winners, mi_stats *stats) {
+int mi_graph_launch(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                    uint32_t seed, uint32_t epoch) {
   if (!g) return MI_ERR_INVALID_ARGUMENT;
   mi_ctx *ctx = g->ctx;
   std::lock_guard<std::mutex> lock(ctx->mu);
-  const auto t0 = Clock::now();
   ctx->last_error.clear();
-  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (g->pending) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "the previous replay was not finished");
   if (npos > 0 && (!pos || !counts)) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "null batch");
   if (npos > g->max_pos) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "more positions than the graph holds");
   uint64_t total = 0;
   int rc;
   if ((rc = plan_batch(ctx, pos, npos, counts, 0, total))) return rc;
   if (total > g->max_play) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "more playouts than the graph holds");
-  if (total == 0) {
-    finish_stats(stats, t0);
-    return MI_OK;
+  if (total > 0) {
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+    gpuai::LaunchDesc *d = (gpuai::LaunchDesc *)g->h_blob;
+    d->npos = npos;
+    d->total = (uint32_t)total;
+    d->seed = seed;
+    d->epoch = epoch;
+    std::memcpy((char *)g->h_blob + g->offs_off, ctx->host_offs.data(),
+                sizeof(uint32_t) * ((size_t)npos + 1));
+    std::memcpy((char *)g->h_blob + g->pos_off, pos, sizeof(mi_pos) * (size_t)npos);
+    const size_t in_bytes = g->pos_off + sizeof(mi_pos) * (size_t)npos;
+    const size_t out_bytes = sizeof(gpuai::WorkWords) + (size_t)total;
+    MI_HIP(ctx, hipGraphExecMemcpyNodeSetParams1D(g->exec, g->n_h2d, g->d_blob, g->h_blob, in_bytes,
+                                                  hipMemcpyHostToDevice));
+    MI_HIP(ctx, hipGraphExecMemcpyNodeSetParams1D(g->exec, g->n_d2h, g->h_out, g->d_out, out_bytes,
+                                                  hipMemcpyDeviceToHost));
+    MI_HIP(ctx, hipEventRecord(g->e0, ctx->stream));
+    MI_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    MI_HIP(ctx, hipEventRecord(g->e1, ctx->stream));
   }
-  if (!winners) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "winners pointer is null");
-  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
-  gpuai::LaunchDesc *d = (gpuai::LaunchDesc *)g->h_blob;
-  d->npos = npos;
-  d->total = (uint32_t)total;
-  d->seed = seed;
-  d->epoch = epoch;
-  std::memcpy((char *)g->h_blob + g->offs_off, ctx->host_offs.data(), sizeof(uint32_t) * ((size_t)npos + 1));
-  std::memcpy((char *)g->h_blob + g->pos_off, pos, sizeof(mi_pos) * (size_t)npos);
-  const size_t in_bytes = g->pos_off + sizeof(mi_pos) * (size_t)npos;
-  const size_t out_bytes = sizeof(gpuai::WorkWords) + (size_t)total;
-  MI_HIP(ctx, hipGraphExecMemcpyNodeSetParams1D(g->exec, g->n_h2d, g->d_blob, g->h_blob, in_bytes,
-                                                hipMemcpyHostToDevice));
-  MI_HIP(ctx, hipGraphExecMemcpyNodeSetParams1D(g->exec, g->n_d2h, g->h_out, g->d_out, out_bytes,
-                                                hipMemcpyDeviceToHost));
-  MI_HIP(ctx, hipEventRecord(g->e0, ctx->stream));
-  MI_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
-  MI_HIP(ctx, hipEventRecord(g->e1, ctx->stream));
-  MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  g->pending = true;
+  g->pending_total = total;
+  return MI_OK;
+}
+
+int mi_graph_finish(mi_graph *g, int8_t *winners, mi_stats *stats) {
+  if (!g) return MI_ERR_INVALID_ARGUMENT;
+  mi_ctx *ctx = g->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (!g->pending) return fail(ctx, MI_ERR_NOT_STAGED, "no replay was launched");
+  const uint64_t total = g->pending_total;
+  if (total > 0 && !winners) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "winners pointer is null");
+  g->pending = false;  // the replay is consumed whatever happens below
+  if (total == 0) return MI_OK;
+  MI_HIP(ctx, hipEventSynchronize(g->e1));
   std::memcpy(winners, (const char *)g->h_out + sizeof(gpuai::WorkWords), (size_t)total);
   if (stats) {
     const gpuai::WorkWords *w = (const gpuai::WorkWords *)g->h_out;
@@ -776,8 +792,24 @@ int mi_graph_run(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_t *
     stats->block_size = ctx->tuning.block_size;
     stats->refill = ctx->tuning.refill;
   }
-  finish_stats(stats, t0);
   return MI_OK;
+}
+
+int mi_graph_run(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                 uint32_t seed, uint32_t epoch, int8_t *winners, mi_stats *stats) {
+  const auto t0 = Clock::now();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (npos > 0 && counts && pos) {  // a null winners buffer is refused before anything runs
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < npos; i++) total += counts[i];
+    if (total > 0 && !winners) return g ? fail(g->ctx, MI_ERR_INVALID_ARGUMENT, "winners pointer is null")
+                                        : MI_ERR_INVALID_ARGUMENT;
+  }
+  int rc = mi_graph_launch(g, pos, npos, counts, seed, epoch);
+  if (rc) return rc;
+  rc = mi_graph_finish(g, winners, stats);
+  finish_stats(stats, t0);
+  return rc;
 }
 
 int mi_graph_destroy(mi_graph *g) {

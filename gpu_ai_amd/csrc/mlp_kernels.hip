@@ -30,28 +30,48 @@ __device__ __forceinline__ uint16_t to_bf(float f) {
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float from_bf(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ __forceinline__ uint16_t as_bf(uint16_t h) { return h; }
-__device__ __forceinline__ uint16_t as_bf(float f) { return to_bf(f); }
 
-// Stages a 64 x 32 slice of a strided operand X(r, k) = X[r*sr + k*sk] into S[r][k] (BF16).
-template <typename T>
-__device__ __forceinline__ void stage(const T *__restrict__ X, long sr, long sk, int r0, int k0,
-                                      uint16_t (*S)[LDSK]) {
+// One thread's share of a 64 x 32 slice of a strided BF16 operand X(r, k) = X[r*sr + k*sk]: eight
+// values, with K contiguous (sk == 1: row t/4, k 8(t%4)..+7) or rows contiguous (sr == 1: k t/8,
+// rows 8(t%8)..+7), both one 16-byte load; anything else element by element.
+__device__ __forceinline__ uint4 load_slice(const uint16_t *__restrict__ X, long sr, long sk,
+                                            int r0, int k0) {
   const int t = threadIdx.x;
-  if (sk == 1) {  // K contiguous: 8 consecutive k per thread
-    const int r = t >> 2, kk = (t & 3) * 8;
-    const T *p = X + (long)(r0 + r) * sr + (k0 + kk);
+  if (sk == 1) {
+    return *(const uint4 *)(X + (long)(r0 + (t >> 2)) * sr + (k0 + (t & 3) * 8));
+  } else if (sr == 1) {
+    return *(const uint4 *)(X + (long)(r0 + (t & 7) * 8) + (long)(k0 + (t >> 3)) * sk);
+  }
+  uint16_t e[8];
+  const int kk = t >> 3, rr = (t & 7) * 8;
 #pragma unroll
-    for (int j = 0; j < 8; j++) S[r][kk + j] = as_bf(p[j]);
-  } else {        // rows contiguous: 8 consecutive rows per thread
-    const int kk = t >> 3, rr = (t & 7) * 8;
-    const T *p = X + (long)(r0 + rr) * sr + (long)(k0 + kk) * sk;
+  for (int j = 0; j < 8; j++) e[j] = X[(long)(r0 + rr + j) * sr + (long)(k0 + kk) * sk];
+  uint4 v;
+  v.x = e[0] | ((uint32_t)e[1] << 16);
+  v.y = e[2] | ((uint32_t)e[3] << 16);
+  v.z = e[4] | ((uint32_t)e[5] << 16);
+  v.w = e[6] | ((uint32_t)e[7] << 16);
+  return v;
+}
+
+// Stores a thread's share into the LDS tile S[row][k] (K contiguous, rows padded to LDSK).
+__device__ __forceinline__ void store_slice(uint16_t (*S)[LDSK], long sk, const uint4 &v) {
+  const int t = threadIdx.x;
+  if (sk == 1) {
+    *(uint4 *)&S[t >> 2][(t & 3) * 8] = v;
+    return;
+  }
+  const int kk = t >> 3, rr = (t & 7) * 8;
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int j = 0; j < 8; j++) S[rr + j][kk] = as_bf(p[(long)j * sr]);
+  for (int j = 0; j < 4; j++) {
+    S[rr + 2 * j][kk] = (uint16_t)(w[j] & 0xFFFFu);
+    S[rr + 2 * j + 1][kk] = (uint16_t)(w[j] >> 16);
   }
 }
 
-template <typename TA, typename TB>
+// C = A B over one 64x64 tile and one K split; the next 32-deep slice is loaded into registers
+// while the current one is multiplied.
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) uint16_t As[BM][LDSK];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][LDSK];
@@ -60,17 +80,23 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int kbeg = blockIdx.z * g.k_per_split;
   const int kend = kbeg + g.k_per_split;
-  const TA *A = (const TA *)g.A;
-  const TB *B = (const TB *)g.B;
+  const uint16_t *A = (const uint16_t *)g.A;
+  const uint16_t *B = (const uint16_t *)g.B;
   f32x4 acc[2][2];
 #pragma unroll
   for (int r = 0; r < 2; r++)
 #pragma unroll
     for (int c = 0; c < 2; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  uint4 ra = load_slice(A, g.sAm, g.sAk, m0, kbeg);
+  uint4 rb = load_slice(B, g.sBn, g.sBk, n0, kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    stage(A, g.sAm, g.sAk, m0, k0, As);
-    stage(B, g.sBn, g.sBk, n0, k0, Bs);
+    store_slice(As, g.sAk, ra);
+    store_slice(Bs, g.sBk, rb);
     __syncthreads();
+    if (k0 + BK < kend) {  // in flight during the MFMAs below
+      ra = load_slice(A, g.sAm, g.sAk, m0, k0 + BK);
+      rb = load_slice(B, g.sBn, g.sBk, n0, k0 + BK);
+    }
     bf16x8 a[2], b[2];
 #pragma unroll
     for (int r = 0; r < 2; r++)
@@ -297,11 +323,9 @@ __global__ void k_zero(float *p, uint32_t n) {
 
 hipError_t gemm(const GemmArgs &g, int splits, bool a_bf16, bool b_bf16, hipStream_t stream) {
   if (g.M % BM || g.N % BN || g.k_per_split % BK) return hipErrorInvalidValue;
+  if (!a_bf16 || !b_bf16) return hipErrorInvalidValue;  // BF16 operands only
   dim3 grid(g.N / BN, g.M / BM, splits), block(256);
-  if (a_bf16 && b_bf16) hipLaunchKernelGGL((k_gemm<uint16_t, uint16_t>), grid, block, 0, stream, g);
-  else if (a_bf16) hipLaunchKernelGGL((k_gemm<uint16_t, float>), grid, block, 0, stream, g);
-  else if (b_bf16) hipLaunchKernelGGL((k_gemm<float, uint16_t>), grid, block, 0, stream, g);
-  else hipLaunchKernelGGL((k_gemm<float, float>), grid, block, 0, stream, g);
+  hipLaunchKernelGGL(k_gemm, grid, block, 0, stream, g);
   return hipGetLastError();
 }
 

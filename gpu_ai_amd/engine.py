@@ -86,6 +86,8 @@ _SIGS = {
     "mi_graph_create": ([_P, _U32, _U32, _U32, ctypes.POINTER(_P)], ctypes.c_int),
     "mi_graph_run": ([_P, _P, _U32, _P, _U32, _U32, _P, ctypes.POINTER(MiStats)], ctypes.c_int),
     "mi_graph_destroy": ([_P], ctypes.c_int),
+    "mi_graph_launch": ([_P, _P, _U32, _P, _U32, _U32], ctypes.c_int),
+    "mi_graph_finish": ([_P, _P, ctypes.POINTER(MiStats)], ctypes.c_int),
     "mi_copy_calibration": ([_P, ctypes.c_uint64, _U32, ctypes.POINTER(ctypes.c_float)],
                             ctypes.c_int),
     "mi_mlp_create": ([_P, _U32, _U32, ctypes.POINTER(_P)], ctypes.c_int),
@@ -324,6 +326,22 @@ class GraphBatch:
         self.engine._check(self.engine.lib.mi_graph_run(self.handle, _ptr(pos), len(pos), _ptr(c),
                                                         seed, epoch, _ptr(winners),
                                                         ctypes.byref(st)))
+        return PlayoutResult(winners, st.as_dict())
+
+    def launch(self, pos, counts, seed: int = 12345, epoch: int = 0) -> None:
+        """Enqueues one replay and returns at once (the batch is copied before returning)."""
+        pos = as_positions(pos)
+        c = np.ascontiguousarray(counts, dtype=np.uint32)
+        self._total = int(c.sum(dtype=np.uint64))
+        self.engine._check(self.engine.lib.mi_graph_launch(self.handle, _ptr(pos), len(pos),
+                                                           _ptr(c), seed, epoch))
+
+    def finish(self) -> PlayoutResult:
+        """Waits for the replay enqueued by launch and returns its winners."""
+        winners = np.empty(getattr(self, "_total", 0), dtype=np.int8)
+        st = MiStats()
+        self.engine._check(self.engine.lib.mi_graph_finish(self.handle, _ptr(winners),
+                                                           ctypes.byref(st)))
         return PlayoutResult(winners, st.as_dict())
 
     def close(self) -> None:

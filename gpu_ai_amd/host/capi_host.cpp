@@ -121,7 +121,8 @@ int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch
     c.scale = cfg->scale;
     c.policy = cfg->policy;
     c.mode = (MultiSearchMode)cfg->mode;
-    if (c.maxBudget < c.minBudget || c.searches == 0 || cfg->mode > 2)
+    c.groups = cfg->groups ? cfg->groups : 2;
+    if (c.maxBudget < c.minBudget || c.searches == 0 || cfg->mode > 3)
       throw std::runtime_error("invalid multi-search configuration");
     MultiSearchResult r = runMultiSearch(c);
     out->playouts = r.playouts;

@@ -50,6 +50,165 @@ void multiSearchSetup(const MultiSearchConfig &cfg, std::vector<mi_pos> &roots,
   }
 }
 
+namespace {
+
+// Latency percentiles, throughput and the digest of every root's statistics.
+void finishResult(MultiSearchResult &res, const std::vector<std::unique_ptr<GameTree>> &trees) {
+  std::vector<double> lat = res.iterationMs;
+  std::sort(lat.begin(), lat.end());
+  if (!lat.empty()) {
+    res.p50Ms = lat[lat.size() / 2];
+    res.p99Ms = lat[std::min(lat.size() - 1, (size_t)std::ceil(0.99 * lat.size()) - 1)];
+    res.maxMs = lat.back();
+  }
+  res.playoutsPerSecond = res.wallSeconds > 0 ? res.playouts / res.wallSeconds : 0;
+  uint64_t h = 0;
+  for (auto &t : trees) {
+    h = mix64(h, t->getTotalTrials());
+    h = mix64(h, (uint64_t)t->getWins(PLAYER_1));
+    h = mix64(h, (uint64_t)t->getWins(PLAYER_2));
+  }
+  res.rootDigest = h;
+}
+
+// Half of the searches with a device context and graph of its own (pipelined mode).
+struct SearchGroup {
+  std::vector<uint32_t> ids;
+  std::unique_ptr<DeviceMI355XPlayoutDriver> driver;
+  mi_graph *graph = nullptr;
+  std::vector<mi_pos> pos;
+  std::vector<uint32_t> counts;
+  std::vector<int8_t> winners;
+  bool launched = false;
+  ~SearchGroup() {
+    if (graph) mi_graph_destroy(graph);
+  }
+};
+
+double msSince(Clock::time_point a) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - a).count();
+}
+
+// Pipelined mode: the searches are dealt round-robin into `groups` groups, each with a device
+// context (stream) and graph of its own; groups take turns, so that while one group's tree
+// selection and update run on the host the other groups' playouts run on the device.  Each search
+// still alternates select -> playouts -> update.
+MultiSearchResult runPipelined(const MultiSearchConfig &cfg,
+                               std::vector<std::unique_ptr<GameTree>> &trees,
+                               std::vector<uint32_t> &remaining) {
+  const uint32_t S = (uint32_t)trees.size();
+  const uint32_t ngroups = std::max<uint32_t>(1u, std::min<uint32_t>(cfg.groups, S));
+  std::vector<std::unique_ptr<SearchGroup>> groups;
+  for (uint32_t k = 0; k < ngroups; k++) {
+    groups.emplace_back(new SearchGroup());
+    SearchGroup &Gk = *groups.back();
+    uint64_t cap = 0;
+    for (uint32_t s = k; s < S; s += ngroups) {
+      Gk.ids.push_back(s);
+      cap += remaining[s];
+    }
+    // independent streams: group k draws under seed + k
+    Gk.driver = std::make_unique<DeviceMI355XPlayoutDriver>(cfg.policy, cfg.device,
+                                                            cfg.playoutSeed + k);
+    cap = std::max<uint64_t>(cap, 1);
+    checkMi(mi_graph_create(Gk.driver->context(), (uint32_t)std::min<uint64_t>(cap, 1u << 22),
+                            (uint32_t)std::min<uint64_t>(cap, (1u << 31) - 1), cfg.policy,
+                            &Gk.graph),
+            Gk.driver->context(), "mi_graph_create");
+  }
+  MultiSearchResult res;
+  std::vector<std::vector<State>> leaves(S);
+  std::vector<std::vector<PlayerId>> results(S);
+
+  auto selectAndLaunch = [&](SearchGroup &G) {
+    const auto t0 = Clock::now();
+    const long m = (long)G.ids.size();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long i = 0; i < m; i++) {
+      const uint32_t s = G.ids[i];
+      leaves[s].clear();
+      if (remaining[s] == 0) continue;
+      unsigned n = (unsigned)(trees[s]->getTotalTrials() * cfg.scale);
+      n = std::max<unsigned>(n, cfg.initial);
+      n = std::min<unsigned>(n, remaining[s]);
+      trees[s]->selectInto(n, leaves[s]);
+    }
+    res.selectMs += msSince(t0);
+    const auto t1 = Clock::now();
+    G.pos.clear();
+    G.counts.clear();
+    for (uint32_t s : G.ids) {
+      const std::vector<State> &L = leaves[s];
+      for (size_t i = 0; i < L.size(); i++) {
+        if (i > 0 && L[i] == L[i - 1]) {
+          G.counts.back()++;
+        } else {
+          G.pos.push_back(L[i].pack());
+          G.counts.push_back(1);
+        }
+      }
+    }
+    G.launched = !G.pos.empty();
+    if (G.launched)
+      checkMi(mi_graph_launch(G.graph, G.pos.data(), (uint32_t)G.pos.size(), G.counts.data(),
+                              G.driver->seed(), G.driver->nextEpoch()),
+              G.driver->context(), "mi_graph_launch");
+    res.playoutMs += msSince(t1);
+  };
+  auto finishAndUpdate = [&](SearchGroup &G) {
+    const auto t0 = Clock::now();
+    uint64_t total = 0;
+    for (uint32_t c : G.counts) total += c;
+    G.winners.resize(total);
+    mi_stats st;
+    checkMi(mi_graph_finish(G.graph, G.winners.data(), &st), G.driver->context(),
+            "mi_graph_finish");
+    G.launched = false;
+    res.deviceMs += st.kernel_ms;
+    size_t at = 0;
+    for (uint32_t s : G.ids) {
+      results[s].resize(leaves[s].size());
+      for (size_t i = 0; i < leaves[s].size(); i++) results[s][i] = (PlayerId)G.winners[at++];
+    }
+    res.playoutMs += msSince(t0);
+    const auto t1 = Clock::now();
+    const long m = (long)G.ids.size();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long i = 0; i < m; i++) {
+      const uint32_t s = G.ids[i];
+      if (leaves[s].empty()) continue;
+      trees[s]->update(results[s]);
+      remaining[s] -= (uint32_t)leaves[s].size();
+    }
+    res.updateMs += msSince(t1);
+    res.playouts += total;
+  };
+  auto pendingWork = [&](const SearchGroup &G) {
+    for (uint32_t s : G.ids)
+      if (remaining[s]) return true;
+    return false;
+  };
+
+  const auto wall0 = Clock::now();
+  for (auto &g : groups) selectAndLaunch(*g);
+  for (;;) {
+    const auto t0 = Clock::now();
+    bool busy = false;
+    for (auto &g : groups) {
+      if (g->launched) finishAndUpdate(*g);
+      if (pendingWork(*g)) selectAndLaunch(*g);  // runs while the other groups' replays run
+      busy |= g->launched;
+    }
+    if (!busy) break;
+    res.iterations++;
+    res.iterationMs.push_back(msSince(t0));
+  }
+  res.wallSeconds = std::chrono::duration<double>(Clock::now() - wall0).count();
+  return res;
+}
+
+}  // namespace
+
 MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
   std::vector<mi_pos> roots;
   std::vector<uint32_t> budgets;
@@ -61,6 +220,11 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
   uint64_t maxIter = 0;
   for (uint32_t b : budgets) maxIter += b;
 
+  if (cfg.mode == MultiSearchMode::PIPELINED) {
+    MultiSearchResult res = runPipelined(cfg, trees, remaining);
+    finishResult(res, trees);
+    return res;
+  }
   DeviceMI355XPlayoutDriver driver(cfg.policy, cfg.device, cfg.playoutSeed);
   mi_graph *graph = nullptr;
   if (cfg.mode == MultiSearchMode::GRAPH) {
@@ -153,20 +317,6 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
   }
   res.wallSeconds = std::chrono::duration<double>(Clock::now() - wall0).count();
   if (graph) mi_graph_destroy(graph);
-  std::vector<double> lat = res.iterationMs;
-  std::sort(lat.begin(), lat.end());
-  if (!lat.empty()) {
-    res.p50Ms = lat[lat.size() / 2];
-    res.p99Ms = lat[std::min(lat.size() - 1, (size_t)std::ceil(0.99 * lat.size()) - 1)];
-    res.maxMs = lat.back();
-  }
-  res.playoutsPerSecond = res.wallSeconds > 0 ? res.playouts / res.wallSeconds : 0;
-  uint64_t h = 0;
-  for (auto &t : trees) {
-    h = mix64(h, t->getTotalTrials());
-    h = mix64(h, (uint64_t)t->getWins(PLAYER_1));
-    h = mix64(h, (uint64_t)t->getWins(PLAYER_2));
-  }
-  res.rootDigest = h;
+  finishResult(res, trees);
   return res;
 }

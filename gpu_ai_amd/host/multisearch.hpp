@@ -8,6 +8,9 @@
 //   BATCHED    : in one ordinary mi_run_playouts call, or
 //   PER_SEARCH : one mi_run_playouts call per search (the reference's one-driver-call-per-search
 //                pattern, the baseline of the comparison),
+//   PIPELINED  : the searches in `groups` groups, each on a context (stream) and graph of its
+//                own, taking turns so that one group's selection and update on the host overlap
+//                the other groups' replays,
 // and every search updates its tree with its own results.
 #pragma once
 
@@ -17,7 +20,7 @@
 
 #include "gpuai/mi_playout.h"
 
-enum class MultiSearchMode { GRAPH = 0, BATCHED = 1, PER_SEARCH = 2 };
+enum class MultiSearchMode { GRAPH = 0, BATCHED = 1, PER_SEARCH = 2, PIPELINED = 3 };
 
 struct MultiSearchConfig {
   uint32_t searches = 256;
@@ -29,6 +32,7 @@ struct MultiSearchConfig {
   double scale = 0.0;            //   capped by the remaining budget
   uint32_t policy = MI_POLICY_RANDOM;
   MultiSearchMode mode = MultiSearchMode::GRAPH;
+  uint32_t groups = 2;           // pipelined mode: groups of searches (device streams)
   int device = 0;
 };
 

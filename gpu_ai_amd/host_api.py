@@ -13,13 +13,13 @@ _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
 _lib = None
 
-MODES = {"graph": 0, "batched": 1, "per_search": 2}
+MODES = {"graph": 0, "batched": 1, "per_search": 2, "pipelined": 3}
 
 
 class MultiSearchConfig(ctypes.Structure):
     _fields_ = [("searches", _U32), ("seed", _U32), ("playout_seed", _U32), ("min_budget", _U32),
                 ("max_budget", _U32), ("initial", _U32), ("scale", ctypes.c_double),
-                ("policy", _U32), ("mode", _U32)]
+                ("policy", _U32), ("mode", _U32), ("groups", _U32)]
 
 
 class MultiSearchResult(ctypes.Structure):
@@ -106,11 +106,12 @@ def run_ai_path() -> str:
 
 def multisearch(searches: int = 256, seed: int = 2016, playout_seed: int = 12345,
                 min_budget: int = 1000, max_budget: int = 64000, initial: int = 256,
-                scale: float = 0.0, policy: int = 0, mode: str = "graph") -> dict:
+                scale: float = 0.0, policy: int = 0, mode: str = "graph",
+                groups: int = 2) -> dict:
     """Batched multi-search: concurrent searches sharing the device; returns the run's
     statistics."""
     cfg = MultiSearchConfig(searches, seed, playout_seed, min_budget, max_budget, initial, scale,
-                            policy, MODES[mode])
+                            policy, MODES[mode], groups)
     res = MultiSearchResult()
     err = ctypes.create_string_buffer(512)
     if host_library().gpuai_multisearch_run(ctypes.byref(cfg), ctypes.byref(res), err, 512) != 0:

@@ -46,7 +46,9 @@ typedef struct gpuai_multisearch_config {
   uint32_t initial;       /* trials per iteration: max(initial, totalTrials * scale) */
   double scale;
   uint32_t policy;        /* MI_POLICY_* */
-  uint32_t mode;          /* 0 graph replay, 1 one plain launch, 2 one launch per search */
+  uint32_t mode;          /* 0 graph replay, 1 one plain launch, 2 one launch per search,
+                             3 graph replays of `groups` groups overlapped with the host's work */
+  uint32_t groups;        /* pipelined mode: groups of searches, one stream each (2) */
 } gpuai_multisearch_config;
 
 typedef struct gpuai_multisearch_result {

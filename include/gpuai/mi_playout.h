@@ -170,6 +170,13 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
                     mi_graph **graph);
 int mi_graph_run(mi_graph *graph, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
                  uint32_t seed, uint32_t epoch, int8_t *winners, mi_stats *stats);
+/* The two halves of mi_graph_run, so that the caller can work while the replay runs: launch
+ * copies the batch into the graph's pinned buffer and enqueues the replay on the context's
+ * stream; finish waits for it and returns its winners.  A graph holds one replay in flight;
+ * graphs of different contexts run concurrently on their own streams. */
+int mi_graph_launch(mi_graph *graph, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                    uint32_t seed, uint32_t epoch);
+int mi_graph_finish(mi_graph *graph, int8_t *winners, mi_stats *stats);
 int mi_graph_destroy(mi_graph *graph);
 
 /* Device-resident batches (inputs stay in HBM between launches).

@@ -28,6 +28,40 @@ def test_multisearch_graph_equals_plain_batched():
     assert p["playouts"] == g["playouts"]
 
 
+def test_multisearch_pipelined_is_deterministic():
+    from gpu_ai_amd import host_api
+    announce("multi-search pipelined, twice, and graph mode")
+    kw = dict(searches=24, min_budget=200, max_budget=3000, initial=64, scale=0.5)
+    a = host_api.multisearch(mode="pipelined", **kw)
+    b = host_api.multisearch(mode="pipelined", **kw)
+    g = host_api.multisearch(mode="graph", **kw)
+    assert a["root_digest"] == b["root_digest"]
+    assert a["playouts"] == b["playouts"] == g["playouts"] > 0  # every budget is spent
+
+
+def test_graph_launch_finish_on_two_contexts(golden):
+    """Two contexts, two graphs in flight at once; each replay equals the plain call."""
+    from gpu_ai_amd.engine import Engine, MiError
+    pos = golden["positions"]["mid1024"][:300]
+    counts = np.random.default_rng(3).integers(0, 40, len(pos)).astype(np.uint32)
+    announce("two graphs in flight")
+    with Engine(0) as e1, Engine(0) as e2:
+        ref1 = e1.run_playouts(pos, counts=counts, seed=5, epoch=1).winners
+        ref2 = e2.run_playouts(pos, counts=counts, seed=6, epoch=2).winners
+        with e1.graph(len(pos), int(counts.sum())) as g1, e2.graph(len(pos), int(counts.sum())) as g2:
+            with pytest.raises(MiError) as e:
+                g1.finish()
+            assert e.value.code == 6  # nothing launched
+            g1.launch(pos, counts, seed=5, epoch=1)
+            with pytest.raises(MiError):
+                g1.launch(pos, counts, seed=5, epoch=1)  # one replay in flight per graph
+            g2.launch(pos, counts, seed=6, epoch=2)
+            r1 = g1.finish()
+            r2 = g2.finish()
+            assert np.array_equal(r1.winners, ref1) and np.array_equal(r2.winners, ref2)
+            assert np.array_equal(g1.run(pos, counts, seed=5, epoch=1).winners, ref1)
+
+
 def test_run_ai_modes():
     from gpu_ai_amd import host_api
     exe = host_api.run_ai_path()
