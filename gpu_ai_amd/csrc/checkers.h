@@ -550,9 +550,14 @@ CK_HD void cache_put(LeafCache &c, uint32_t i, uint32_t to, uint32_t rm) {
   c.rm3 = i == 3u ? rm : c.rm3;
 }
 
+// Reads entry i (< 4) with masks rather than a select chain: a chain of selects over the fields
+// is folded by the compiler into one load through a selected address, which puts the whole
+// cache in scratch memory.
 CK_HD void cache_get(const LeafCache &c, uint32_t i, uint32_t &to, uint32_t &rm) {
-  to = i == 0u ? c.to0 : i == 1u ? c.to1 : i == 2u ? c.to2 : c.to3;
-  rm = i == 0u ? c.rm0 : i == 1u ? c.rm1 : i == 2u ? c.rm2 : c.rm3;
+  const uint32_t m0 = 0u - (uint32_t)(i == 0u), m1 = 0u - (uint32_t)(i == 1u);
+  const uint32_t m2 = 0u - (uint32_t)(i == 2u), m3 = 0u - (uint32_t)(i == 3u);
+  to = (c.to0 & m0) | (c.to1 & m1) | (c.to2 & m2) | (c.to3 & m3);
+  rm = (c.rm0 & m0) | (c.rm1 & m1) | (c.rm2 & m2) | (c.rm3 & m3);
 }
 
 // Move count of a capture ply with multi-jump origins: single-jump origins contribute their slot
