@@ -10,13 +10,15 @@ A step is one launch of the persistent playout kernel over the staged batch of t
   small           : 64 positions x 4096 random playouts (the host path's own size)
   multisearch     : 256 concurrent MCTS searches sharing the device (one line, its own fields)
   value           : value-network training and evaluation (one line, its own fields)
+  cpu             : CPU rates only (reference host code and restatement, 1 and all threads)
 Positions are generated on the device (mi_gen_start_positions) and staged in HBM once; each step
 uses its own epoch (a fresh Philox stream), so no step repeats another.
 
 roofline: the path is integer- and branch-bound.  Its work unit is one ply (generate, choose,
 apply, terminal test) priced at a frozen W = 192 int32 lane-ops; achieved = plies x W / average
 kernel time (HIP events on the library's stream), against the chip's int32 VALU issue rate
-CUs x 64 lanes x clock.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+CUs x 64 lanes x clock; peak_measured is that rate as a probe kernel of plain v_add_u32 /
+v_xor_b32 chains reaches it on this device (mi_valu_rate).  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
 recorded in profiles/ (null when absent).
 cpu_baseline: the reference's own host code (oracle/_ref/libref.so: State::genMoves twice per ply
 as src/playout.cpp:17-32 does, OpenMP over playouts, Philox draws) on a bounded sample of the same
@@ -81,6 +83,52 @@ def cpu_baseline(cfg: dict, pos, threads: int) -> dict:
             "kind": kind,
             "sample": f"first {npos} positions x {per} playouts of the workload "
                       f"({npos * per} playouts, {dt:.2f} s wall on {threads} threads)"}
+
+
+def bench_cpu(args) -> int:
+    """CPU rates beside the device figure, on the random-64x4096 positions (the host path's own
+    size).  Codes: the reference's own host code (oracle/_ref/libref.so: State::genMoves twice
+    per ply as src/playout.cpp:17-32 does) with the Philox choice, and with glibc rand() % n as
+    RandomPlayer::getMove draws it (src/player.cpp:13-16; rand() takes a global lock, so that
+    variant shows the reference's contention); and the C restatement (oracle/liboracle.so).  Each
+    on one thread and on all threads.  No GPU is used: the positions come from the restatement's
+    start-position generator (the positions the device generates for the same seed)."""
+    from oracle import pyoracle
+    cfg = WORKLOADS["small"]
+    orc = pyoracle.Oracle()
+    pos = orc.start_positions(POS_SEED, cfg["n"], cfg["kmin"], cfg["kmax"])
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    codes = [("restatement (oracle/liboracle.so)", orc, [("philox", 0)])]
+    if os.path.exists(pyoracle.REF_SO):
+        codes.insert(0, ("reference host code (oracle/_ref/libref.so)",
+                         pyoracle.Reference(pyoracle.REF_SO),
+                         [("philox", 0), ("glibc rand()", 2)]))
+    rows = []
+    for code, lib, choices in codes:
+        for choice, policy in choices:
+            for th in sorted({1, threads}):
+                # whole workload where it takes a second or two, else 512 playouts per position
+                per = cfg["per_pos"] if th > 1 and policy == 0 else 512
+                t0 = time.perf_counter()
+                lib.run_playouts(pos, per_pos=per, seed=PLAY_SEED, epoch=0, policy=policy,
+                                 threads=th, traces=False)
+                dt = time.perf_counter() - t0
+                rows.append({"code": code, "choice": choice, "threads": th,
+                             "playouts": len(pos) * per, "seconds": round(dt, 3),
+                             "playouts_per_s": round(len(pos) * per / dt, 1)})
+                print(f"[cpu] {code}, {choice}, {th} threads: {rows[-1]['playouts_per_s']:.0f} "
+                      f"playouts/s", file=sys.stderr, flush=True)
+    model = ""
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                     if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    print(json.dumps({"metric": "CPU playouts/s (random-64x4096 positions)", "unit": "playouts/s",
+                      "higher_is_better": True, "data": "synthetic (seeded positions)",
+                      "host": {"cpu_count": os.cpu_count(), "model": model},
+                      "rows": rows}), flush=True)
+    return 0
 
 
 def bench_multisearch() -> int:
@@ -184,7 +232,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="random",
-                    choices=sorted(WORKLOADS) + ["multisearch", "value"])
+                    choices=sorted(WORKLOADS) + ["multisearch", "value", "cpu"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shape instead of tuning on the batch")
@@ -200,6 +248,8 @@ def main() -> int:
               "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
         return 2
 
+    if args.workload == "cpu":
+        return bench_cpu(args)
     if args.workload == "multisearch":
         return bench_multisearch()
     if args.workload == "value":
@@ -279,6 +329,10 @@ def main() -> int:
     line["host_buffers"] = {"total_ms": round(hb.stats["total_ms"], 4),
                             "kernel_ms": round(hb.stats["kernel_ms"], 4),
                             "playouts_per_s": round(total / (hb.stats["total_ms"] / 1e3), 1)}
+    # the roofline's peak, measured: a probe kernel of independent int32 add/xor instructions
+    valu_rate, valu_ms = eng.valu_rate()
+    line["roofline"]["peak_measured"] = round(valu_rate / 1e12, 3)
+    line["roofline"]["frac_of_measured"] = round(achieved / valu_rate, 4) if valu_rate else None
     line["net_leaf_win_rate_recorded"] = recorded_match()
     if args.copy_calibration:
         ms = eng.copy_calibration(args.copy_calibration, 5)

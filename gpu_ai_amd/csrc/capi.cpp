@@ -617,6 +617,36 @@ int mi_copy_calibration(mi_ctx *ctx, uint64_t bytes, uint32_t repeats, float *av
   return rc;
 }
 
+int mi_valu_rate(mi_ctx *ctx, uint32_t iters, uint32_t repeats, double *lane_ops_per_s,
+                 float *avg_ms) {
+  if (!ctx || !lane_ops_per_s || !avg_ms || iters == 0 || iters > (1u << 20) || repeats == 0)
+    return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  // 32 waves per CU (8 per SIMD), enough to issue a VALU instruction every cycle on each SIMD
+  const uint32_t threads = ctx->cus * 32u * 64u;
+  void *buf = nullptr;
+  MI_HIP(ctx, hipMalloc(&buf, (size_t)threads * sizeof(uint32_t)));
+  hipError_t e = hipSuccess;
+  float total = 0.f, best = 0.f;
+  for (uint32_t i = 0; i < repeats && e == hipSuccess; i++) {
+    e = hipEventRecord(ctx->ev_start, ctx->stream);
+    if (e == hipSuccess) e = gpuai::launch_valu_rate(iters, 0x5bd1e995u + i, (uint32_t *)buf, threads, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_stop);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    total += ms;
+    if (e == hipSuccess && (best == 0.f || ms < best)) best = ms;
+  }
+  int rc = e == hipSuccess ? MI_OK : hip_fail(ctx, e, "VALU rate probe");
+  (void)hipFree(buf);
+  *avg_ms = total / (float)repeats;
+  *lane_ops_per_s = best > 0.f ? (double)threads * iters * 32.0 / (best * 1e-3) : 0.0;
+  return rc;
+}
+
 // ---------------------------------------------------------------------------------------------
 // graph-replayed batches
 // ---------------------------------------------------------------------------------------------

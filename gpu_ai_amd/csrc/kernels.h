@@ -66,6 +66,10 @@ hipError_t launch_start_positions(uint32_t seed, uint32_t n, uint32_t kmin, uint
 hipError_t launch_divmod(const uint32_t *r, const uint32_t *n, uint32_t count, uint32_t *q,
                          uint32_t *rem, hipStream_t stream);
 hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n16, hipStream_t stream);
+// `threads` must be a multiple of 256; `out` holds `threads` words.  32 VALU instructions per
+// lane per iteration.
+hipError_t launch_valu_rate(uint32_t iters, uint32_t salt, uint32_t *out, uint32_t threads,
+                            hipStream_t stream);
 hipError_t launch_tally(const int8_t *winners, const uint32_t *offs, uint32_t npos,
                         uint32_t per_pos, uint32_t *tallies, hipStream_t stream);
 

@@ -360,6 +360,28 @@ __global__ void __launch_bounds__(256) k_copy(const uint4 *__restrict__ src, uin
     dst[i] = src[i];
 }
 
+// Integer VALU issue-rate probe (the roofline's peak, measured).  Every lane runs eight
+// independent chains of v_add_u32 / v_xor_b32 with uniform (SGPR) operands, written as asm so
+// that the compiler can neither fuse them (v_xad_u32, v_bitop3_b32) nor fold them: one loop trip
+// is exactly 32 VALU instructions, so lane-ops per launch = threads x iters x 32.
+#define VALU_STEP(a)                                                                     \
+  asm volatile("v_add_u32 %0, %1, %0\n\tv_xor_b32 %0, %2, %0" : "+v"(a) : "s"(k0), "s"(k1))
+
+__global__ void __launch_bounds__(256) k_valu_rate(uint32_t iters, uint32_t salt, uint32_t *out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k0 = salt | 1u, k1 = salt * 0x9E3779B9u;
+  uint32_t a0 = t ^ salt, a1 = a0 + 1u, a2 = a0 + 2u, a3 = a0 + 3u;
+  uint32_t a4 = a0 + 4u, a5 = a0 + 5u, a6 = a0 + 6u, a7 = a0 + 7u;
+  for (uint32_t i = 0; i < iters; i++) {
+    VALU_STEP(a0); VALU_STEP(a1); VALU_STEP(a2); VALU_STEP(a3);
+    VALU_STEP(a4); VALU_STEP(a5); VALU_STEP(a6); VALU_STEP(a7);
+    VALU_STEP(a0); VALU_STEP(a1); VALU_STEP(a2); VALU_STEP(a3);
+    VALU_STEP(a4); VALU_STEP(a5); VALU_STEP(a6); VALU_STEP(a7);
+  }
+  out[t] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
+}
+#undef VALU_STEP
+
 template <uint32_t POLICY, bool TRACE>
 hipError_t launch_refill(const PlayoutArgs &args, uint32_t refill, uint32_t grid, uint32_t block,
                          hipStream_t stream) {
@@ -432,6 +454,13 @@ hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n16, hipStream_t s
   const uint64_t want = (n16 + block - 1) / block;
   const uint32_t grid = (uint32_t)(want < 16384u ? want : 16384u);
   hipLaunchKernelGGL(k_copy, dim3(grid ? grid : 1u), dim3(block), 0, stream, src, dst, n16);
+  return hipGetLastError();
+}
+
+hipError_t launch_valu_rate(uint32_t iters, uint32_t salt, uint32_t *out, uint32_t threads,
+                            hipStream_t stream) {
+  const uint32_t block = 256;
+  hipLaunchKernelGGL(k_valu_rate, dim3(threads / block), dim3(block), 0, stream, iters, salt, out);
   return hipGetLastError();
 }
 

@@ -90,6 +90,8 @@ _SIGS = {
     "mi_graph_finish": ([_P, _P, ctypes.POINTER(MiStats)], ctypes.c_int),
     "mi_copy_calibration": ([_P, ctypes.c_uint64, _U32, ctypes.POINTER(ctypes.c_float)],
                             ctypes.c_int),
+    "mi_valu_rate": ([_P, _U32, _U32, ctypes.POINTER(ctypes.c_double),
+                      ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "mi_mlp_create": ([_P, _U32, _U32, ctypes.POINTER(_P)], ctypes.c_int),
     "mi_mlp_destroy": ([_P], ctypes.c_int),
     "mi_mlp_get_params": ([_P, _P], ctypes.c_int),
@@ -298,6 +300,13 @@ class Engine:
         ms = ctypes.c_float()
         self._check(self.lib.mi_copy_calibration(self.ctx, nbytes, repeats, ctypes.byref(ms)))
         return ms.value
+
+    def valu_rate(self, iters: int = 16384, repeats: int = 5) -> tuple[float, float]:
+        """Measured int32 VALU issue rate (lane-ops/s, best of `repeats`) and average ms."""
+        rate, ms = ctypes.c_double(), ctypes.c_float()
+        self._check(self.lib.mi_valu_rate(self.ctx, iters, repeats, ctypes.byref(rate),
+                                          ctypes.byref(ms)))
+        return rate.value, ms.value
 
     def check_divmod(self, r, n) -> tuple[np.ndarray, np.ndarray]:
         r = np.ascontiguousarray(r, dtype=np.uint32)

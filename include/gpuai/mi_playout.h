@@ -199,6 +199,15 @@ int mi_check_divmod(mi_ctx *ctx, const uint32_t *r, const uint32_t *n, uint32_t 
  * calibrate the HBM traffic counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
 int mi_copy_calibration(mi_ctx *ctx, uint64_t bytes, uint32_t repeats, float *avg_ms);
 
+/* Diagnostic (no counterpart in the reference): the device's measured int32 VALU issue rate, the
+ * peak of the playout kernels' roofline.  Runs a probe kernel (k_valu_rate: 32 waves per CU, 8
+ * per SIMD, eight independent chains of v_add_u32 / v_xor_b32 per lane, exactly 32 VALU
+ * instructions per lane per iteration) `repeats` times over `iters` iterations and returns
+ * the best rate in lane-operations per second and the average kernel time.  iters in
+ * [1, 1<<20], repeats >= 1. */
+int mi_valu_rate(mi_ctx *ctx, uint32_t iters, uint32_t repeats, double *lane_ops_per_s,
+                 float *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

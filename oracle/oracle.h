@@ -34,6 +34,9 @@ extern "C" {
 
 #define OR_POLICY_RANDOM 0
 #define OR_POLICY_HEURISTIC 1
+/* Reference harness only (timing): the random choice drawn with glibc rand() % n, as
+ * RandomPlayer::getMove does (src/player.cpp:13-16).  Not reproducible; never used for parity. */
+#define OR_POLICY_RANDOM_LIBC 2
 
 /* Philox stream tags (third counter word). */
 #define OR_TAG_PLAYOUT 0u

@@ -16,6 +16,7 @@
 
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include <omp.h>
@@ -109,9 +110,17 @@ int playout(State state, uint32_t seed, uint32_t posIndex, uint32_t playoutIndex
       if (tr) h = or_hash_step(h, or_ply_record(L, n, -1));
       break;
     }
+    unsigned k;
+    if (policy == OR_POLICY_RANDOM_LIBC) {
+      k = (unsigned)(rand() % (int)moves.size());  // src/player.cpp:15, global-lock rand()
+      if (moves[0].jumps > 0) captures++;
+      if (tr) h = or_hash_step(h, or_ply_record(L, n, (int)k));
+      state.move(moves[k]);
+      plies++;
+      continue;
+    }
     if ((ply & 3) == 0) words.refill(seed, posIndex, ply >> 2, playoutIndex, OR_TAG_PLAYOUT, epoch);
     uint32_t r = words.w[ply & 3];
-    unsigned k;
     if (policy == OR_POLICY_HEURISTIC) {
       std::vector<std::array<int, NUM_PLAYERS>> moveScores(moves.size());
       std::vector<float> weights(moves.size());

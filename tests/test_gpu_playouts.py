@@ -236,3 +236,18 @@ def test_graph_replays_match_eager(engine, golden, policy):
             assert got.stats["plies"] == eager.stats["plies"]
         with pytest.raises(Exception):
             g.run(pos[:2], np.array([200000, 200000], dtype=np.uint32))
+
+
+def test_valu_rate_probe(engine):
+    """The measured int32 VALU issue rate (the roofline's peak) lies between a quarter of and the
+    device's CUs x 64 lanes x reported clock; bad arguments are rejected."""
+    info = engine.device_info()
+    announce(f"mi_valu_rate on {info['compute_units']} CUs at {info['clock_khz']} kHz")
+    rate, ms = engine.valu_rate(iters=4096, repeats=3)
+    ceiling = info["compute_units"] * 64 * info["clock_khz"] * 1e3
+    assert ms > 0.0
+    assert 0.25 * ceiling < rate <= 1.02 * ceiling, (rate, ceiling)
+    with pytest.raises(Exception):
+        engine.valu_rate(iters=0)
+    with pytest.raises(Exception):
+        engine.valu_rate(iters=(1 << 20) + 1)
