@@ -6,7 +6,7 @@
 // a ply index that is a multiple of four on those iterations and the Philox4x32-10 call that
 // yields the next four words is wave-uniform (one call per four plies instead of a divergent
 // call on almost every iteration).  Refill policy MI_REFILL_QUEUE claims playout indices for the
-// whole wave in chunks of 64 from a device counter (one atomic per 64 playouts, broadcast with a
+// whole wave in multiples of 32 from a device counter (one atomic per claim, broadcast with a
 // lane read), MI_REFILL_STATIC strides lanes over the index space without atomics.
 #include <hip/hip_runtime.h>
 
@@ -17,7 +17,10 @@ namespace gpuai {
 
 namespace {
 
-constexpr uint32_t kChunk = 64;
+// Queue claims are made in multiples of kChunk playouts (a wave holds at most one partly used
+// claim): 32 balances the atomics on the queue word against the work a wave still holds when
+// the queue runs dry.
+constexpr uint32_t kChunk = 32;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
@@ -120,8 +123,10 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
           const uint32_t cnt = (uint32_t)__popcll(idle);
           const uint32_t avail = res_end - res_next;
           uint32_t chunk = 0;
+          // the new claim covers every idle lane that the old one cannot
+          const uint32_t want = avail < cnt ? ((cnt - avail + kChunk - 1u) / kChunk) * kChunk : 0u;
           if (avail < cnt) {
-            if (lane == 0u) chunk = atomicAdd(&a.work->queue, kChunk);
+            if (lane == 0u) chunk = atomicAdd(&a.work->queue, want);
             chunk = __shfl(chunk, 0);
           }
           if (!active) {
@@ -134,7 +139,7 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
           }
           if (avail < cnt) {
             res_next = chunk + (cnt - avail);
-            res_end = chunk + kChunk;
+            res_end = chunk + want;
           } else {
             res_next += cnt;
           }
