@@ -35,6 +35,10 @@ struct mi_mlp {
   float *pW1 = nullptr, *pW2 = nullptr;  // split-K partials of dW1, dW2
   float *pV = nullptr;                    // column-sum partials of b1, b2, w3, b3
   mi_pos *pos = nullptr;
+  // pinned host staging of one batch: positions (max_rows), then values and targets (max_rows
+  // floats each); copies from and to the caller's pageable buffers go through it, so the DMA
+  // transfers need no driver-side staging
+  void *h_stage = nullptr;
   // device-resident training set
   DevBuf ds_pos, ds_t, ds_perm, ds_loss;
   uint32_t ds_n = 0;
@@ -45,6 +49,10 @@ struct mi_mlp {
 namespace {
 
 uint32_t round64(uint32_t n) { return (n + 63u) & ~63u; }
+
+mi_pos *stage_pos(mi_mlp *n) { return (mi_pos *)n->h_stage; }
+float *stage_values(mi_mlp *n) { return (float *)((char *)n->h_stage + sizeof(mi_pos) * n->max_rows); }
+float *stage_targets(mi_mlp *n) { return stage_values(n) + n->max_rows; }
 
 // Rows per split of the weight-gradient GEMMs (K = rows): 128, or 64 when rows is an odd
 // multiple of 64, so a 4096-row batch gives 32 splits (512 and 256 workgroups for dW2, dW1).
@@ -183,15 +191,28 @@ int check_batch(mi_mlp *n, uint32_t count, const void *pos) {
   return MI_OK;
 }
 
+// Uploads the positions of a batch through the pinned staging buffer.
+int upload_positions(mi_mlp *n, const mi_pos *pos, uint32_t count) {
+  mi_ctx *ctx = n->ctx;
+  mi_pos *hp = stage_pos(n);
+  std::memcpy(hp, pos, sizeof(mi_pos) * count);
+  MI_HIP(ctx, hipMemcpyAsync(n->pos, hp, sizeof(mi_pos) * count, hipMemcpyHostToDevice, ctx->stream));
+  return MI_OK;
+}
+
 // Uploads a batch (positions, optional targets) and encodes it; returns the padded row count.
 int stage_batch(mi_mlp *n, const mi_pos *pos, const float *targets, uint32_t count,
                 uint32_t &rows) {
   mi_ctx *ctx = n->ctx;
   rows = round64(count);
-  MI_HIP(ctx, hipMemcpyAsync(n->pos, pos, sizeof(mi_pos) * count, hipMemcpyHostToDevice, ctx->stream));
-  if (targets)
-    MI_HIP(ctx, hipMemcpyAsync(n->targets, targets, sizeof(float) * count, hipMemcpyHostToDevice,
+  int rc;
+  if ((rc = upload_positions(n, pos, count))) return rc;
+  if (targets) {
+    float *ht = stage_targets(n);
+    std::memcpy(ht, targets, sizeof(float) * count);
+    MI_HIP(ctx, hipMemcpyAsync(n->targets, ht, sizeof(float) * count, hipMemcpyHostToDevice,
                                ctx->stream));
+  }
   MI_HIP(ctx, K::encode(n->pos, nullptr, count, rows, n->X, ctx->stream));
   return MI_OK;
 }
@@ -231,6 +252,8 @@ int mi_mlp_create(mi_ctx *ctx, uint32_t max_batch, uint32_t seed, mi_mlp **out) 
   if (e == hipSuccess) e = dev_alloc(n, &n->pW2, (R / 64) * 65536);
   if (e == hipSuccess) e = dev_alloc(n, &n->pV, (R / 64) * K::kVecSlots);
   if (e == hipSuccess) e = dev_alloc(n, &n->pos, R);
+  if (e == hipSuccess)
+    e = hipHostMalloc(&n->h_stage, R * (sizeof(mi_pos) + 2 * sizeof(float)), hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreate(&n->e0);
   if (e == hipSuccess) e = hipEventCreate(&n->e1);
   int rc = MI_OK;
@@ -240,6 +263,9 @@ int mi_mlp_create(mi_ctx *ctx, uint32_t max_batch, uint32_t seed, mi_mlp **out) 
   if (!rc) rc = upload_params(n, w.data());
   if (rc) {
     for (void *p : n->allocs) (void)hipFree(p);
+    if (n->h_stage) (void)hipHostFree(n->h_stage);
+    if (n->e0) (void)hipEventDestroy(n->e0);
+    if (n->e1) (void)hipEventDestroy(n->e1);
     delete n;
     return rc;
   }
@@ -253,6 +279,7 @@ int mi_mlp_destroy(mi_mlp *n) {
   (void)hipSetDevice(n->ctx->device);
   (void)hipStreamSynchronize(n->ctx->stream);
   for (void *p : n->allocs) (void)hipFree(p);
+  if (n->h_stage) (void)hipHostFree(n->h_stage);
   gpuai::DevBuf *bufs[] = {&n->ds_pos, &n->ds_t, &n->ds_perm, &n->ds_loss};
   for (auto *b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
@@ -288,13 +315,14 @@ int mi_mlp_forward(mi_mlp *n, const mi_pos *pos, uint32_t count, float *values) 
   if ((rc = check_batch(n, count, pos))) return rc;
   if (count == 0) return MI_OK;
   if (!values) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "null values");
-  uint32_t rows;
-  if ((rc = stage_batch(n, pos, nullptr, count, rows))) return rc;
-  if ((rc = forward_rows(n, count, rows))) return rc;
-  MI_HIP(ctx, K::head_forward(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, count, rows, n->y,
-                              ctx->stream));
-  MI_HIP(ctx, hipMemcpyAsync(values, n->y, sizeof(float) * count, hipMemcpyDeviceToHost, ctx->stream));
+  // one launch: encoding, both hidden layers and the output layer (k_forward)
+  if ((rc = upload_positions(n, pos, count))) return rc;
+  MI_HIP(ctx, K::forward(n->pos, count, n->W1t, n->w + MI_MLP_OFF_B1, n->W2t, n->w + MI_MLP_OFF_B2,
+                         n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, n->y, ctx->stream));
+  float *hv = stage_values(n);
+  MI_HIP(ctx, hipMemcpyAsync(hv, n->y, sizeof(float) * count, hipMemcpyDeviceToHost, ctx->stream));
   MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(values, hv, sizeof(float) * count);
   return MI_OK;
 }
 

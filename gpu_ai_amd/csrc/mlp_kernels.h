@@ -49,9 +49,11 @@ struct AdamArgs {
 hipError_t gemm(const GemmArgs &g, int splits, bool a_bf16, bool b_bf16, hipStream_t stream);
 hipError_t encode(const mi_pos *pos, const uint32_t *idx, uint32_t n, uint32_t rows, uint16_t *X,
                   hipStream_t stream);
-// y = tanh(H2 . w3 + b3) for rows < n.
-hipError_t head_forward(const uint16_t *H2, const float *w3, const float *b3, uint32_t n,
-                        uint32_t rows, float *y, hipStream_t stream);
+// Inference in one launch (k_forward): y[r] = network(pos[r]) for r < n, 16 rows per
+// workgroup, bit-identical to encode + gemm (both layers) + the output layer.
+hipError_t forward(const mi_pos *pos, uint32_t n, const uint16_t *W1t, const float *b1,
+                   const uint16_t *W2t, const float *b2, const float *w3, const float *b3, float *y,
+                   hipStream_t stream);
 // Loss and output-layer gradients over 64-row blocks (rows a multiple of 64); the target of
 // row r is targets[tidx ? tidx[r] : r].  Writes dz, dH2 and the partial slots of b2, w3, b3
 // and the loss.

@@ -154,20 +154,102 @@ __global__ void __launch_bounds__(256) k_encode(const mi_pos *__restrict__ pos,
   for (int s = 0; s < 32; s++) dst[s] = ((bits >> s) & 1u) ? (uint16_t)0x3F80u : (uint16_t)0u;
 }
 
-// Output layer, forward only: y = tanh(H2 . w3 + b3) per row (one wave per row, four columns
-// per lane, a butterfly sum over the wave).
-__global__ void __launch_bounds__(256) k_head_fwd(const uint16_t *__restrict__ H2,
-                                                 const float *__restrict__ w3, const float *b3,
-                                                 uint32_t n, uint32_t rows, float *__restrict__ y) {
-  const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (row >= rows) return;
-  const uint16_t *h = H2 + (size_t)row * 256 + lane * 4;
-  float part = 0.f;
+// Inference in one launch: encoding, both hidden layers and the output layer for FR rows per
+// workgroup, with the activations in LDS.  The arithmetic is that of k_encode and k_gemm (bias,
+// ReLU, BF16 rounding of H1 and H2; the same 32-deep K slices accumulated in the same order by
+// the same MFMA), and the output layer sums four columns per lane and then over the wave with a
+// butterfly, so the values equal those of the layer-by-layer forward bit for bit (checked at
+// batches of 1 to 4096 rows).  Wave w computes output columns 64w..64w+63 of
+// each layer as four 16x16 tiles; the B operands (W1t, W2t: [out][in], in contiguous) are read
+// from global memory (they stay in L2), one 16-byte load per lane per tile and slice.
+constexpr int FR = 16, XP = 128 + 8, HP = 256 + 8;  // rows; LDS row pitches (BF16 elements)
+
+__global__ void __launch_bounds__(256) k_forward(const mi_pos *__restrict__ pos, uint32_t n,
+                                                const uint16_t *__restrict__ W1t,
+                                                const float *__restrict__ b1,
+                                                const uint16_t *__restrict__ W2t,
+                                                const float *__restrict__ b2,
+                                                const float *__restrict__ w3, const float *b3,
+                                                float *__restrict__ y) {
+  __shared__ __attribute__((aligned(16))) uint16_t X[FR][XP];
+  __shared__ __attribute__((aligned(16))) uint16_t H[FR][HP];  // H1, then H2
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint32_t r0 = blockIdx.x * FR;
+  {  // encode: thread t -> row t/16, plane (t/4)%4, squares 8(t%4)..+7
+    const int row = t >> 4, plane = (t >> 2) & 3, s0 = (t & 3) * 8;
+    uint32_t bits = 0;
+    if (r0 + row < n) {
+      const mi_pos p = pos[r0 + row];
+      uint32_t own = p.turn ? __builtin_bitreverse32(p.p2) : p.p1;
+      uint32_t opp = p.turn ? __builtin_bitreverse32(p.p1) : p.p2;
+      uint32_t kings = p.turn ? __builtin_bitreverse32(p.kings) : p.kings;
+      kings &= own | opp;
+      bits = plane == 0 ? (own & ~kings) : plane == 1 ? (own & kings)
+           : plane == 2 ? (opp & ~kings) : (opp & kings);
+    }
+    bits >>= s0;
+    uint32_t w[4];
 #pragma unroll
-  for (int c = 0; c < 4; c++) part += from_bf(h[c]) * w3[lane * 4 + c];
-  for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
-  if (lane == 0 && row < n) y[row] = tanhf(part + b3[0]);
+    for (int j = 0; j < 4; j++)
+      w[j] = ((bits >> (2 * j)) & 1u ? 0x3F80u : 0u) | ((bits >> (2 * j + 1)) & 1u ? 0x3F800000u : 0u);
+    *(uint4 *)&X[row][plane * 32 + s0] = uint4{w[0], w[1], w[2], w[3]};
+  }
+  __syncthreads();
+  const int kg = (lane >> 4) * 8;  // this lane's eight K values within a slice
+  f32x4 acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < 128; k0 += 32) {  // layer 1: X[16][128] . W1
+    const bf16x8 a = *(const bf16x8 *)&X[lane & 15][k0 + kg];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int col = wave * 64 + c * 16 + (lane & 15);
+      const bf16x8 b = *(const bf16x8 *)(W1t + col * 128 + k0 + kg);
+      acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[c], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int col = wave * 64 + c * 16 + (lane & 15);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      float x = acc[c][v] + b1[col];
+      x = x > 0.f ? x : 0.f;
+      H[4 * (lane >> 4) + v][col] = to_bf(x);
+      acc[c][v] = 0.f;
+    }
+  }
+  __syncthreads();
+  for (int k0 = 0; k0 < 256; k0 += 32) {  // layer 2: H1[16][256] . W2
+    const bf16x8 a = *(const bf16x8 *)&H[lane & 15][k0 + kg];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int col = wave * 64 + c * 16 + (lane & 15);
+      const bf16x8 b = *(const bf16x8 *)(W2t + col * 256 + k0 + kg);
+      acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[c], 0, 0, 0);
+    }
+  }
+  __syncthreads();  // every wave has read H1
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int col = wave * 64 + c * 16 + (lane & 15);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      float x = acc[c][v] + b2[col];
+      x = x > 0.f ? x : 0.f;
+      H[4 * (lane >> 4) + v][col] = to_bf(x);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < FR / 4; i++) {  // output layer: wave w, rows 4w..4w+3
+    const int row = wave * (FR / 4) + i;
+    float part = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; c++) part += from_bf(H[row][lane * 4 + c]) * w3[lane * 4 + c];
+    for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
+    if (lane == 0 && r0 + row < n) y[r0 + row] = tanhf(part + b3[0]);
+  }
 }
 
 // Output layer with its loss and gradients, over 64-row blocks (four waves of 16 rows; lane l
@@ -336,10 +418,11 @@ hipError_t encode(const mi_pos *pos, const uint32_t *idx, uint32_t n, uint32_t r
   return hipGetLastError();
 }
 
-hipError_t head_forward(const uint16_t *H2, const float *w3, const float *b3, uint32_t n,
-                        uint32_t rows, float *y, hipStream_t stream) {
-  hipLaunchKernelGGL(k_head_fwd, dim3((rows * 64 + 255) / 256), dim3(256), 0, stream, H2, w3, b3,
-                     n, rows, y);
+hipError_t forward(const mi_pos *pos, uint32_t n, const uint16_t *W1t, const float *b1,
+                   const uint16_t *W2t, const float *b2, const float *w3, const float *b3, float *y,
+                   hipStream_t stream) {
+  hipLaunchKernelGGL(k_forward, dim3((n + FR - 1) / FR), dim3(256), 0, stream, pos, n, W1t, b1,
+                     W2t, b2, w3, b3, y);
   return hipGetLastError();
 }
 

@@ -5,8 +5,10 @@ without it raises at load time, so a missing extension never degrades into a sil
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -145,6 +147,23 @@ class PlayoutResult:
     hashes: np.ndarray | None = None
 
 
+# Objects still open when the interpreter exits are closed by an atexit hook, children (graphs,
+# networks) before their device contexts, while the HIP runtime is certainly alive: finalizers
+# at interpreter shutdown run in no guaranteed order, possibly after native teardown.
+_open_children: "weakref.WeakSet" = weakref.WeakSet()
+_open_engines: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_open_objects() -> None:
+    for group in (_open_children, _open_engines):
+        for obj in list(group):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
 class Engine:
     """One device context (stream, events, buffers) of the playout engine."""
 
@@ -156,6 +175,7 @@ class Engine:
         rc = self.lib.mi_create(ctypes.byref(self.ctx), device, ctypes.byref(t))
         if rc != MI_OK:
             raise MiError(rc, self.lib.mi_strerror(rc).decode())
+        _open_engines.add(self)
 
     # -- helpers --
     def _check(self, rc: int) -> None:
@@ -326,6 +346,7 @@ class GraphBatch:
         self.handle = ctypes.c_void_p()
         engine._check(engine.lib.mi_graph_create(engine.ctx, max_positions, max_playouts, policy,
                                                  ctypes.byref(self.handle)))
+        _open_children.add(self)
 
     def run(self, pos, counts, seed: int = 12345, epoch: int = 0) -> PlayoutResult:
         pos = as_positions(pos)
@@ -375,6 +396,7 @@ class ValueNet:
         engine._check(self.lib.mi_mlp_create(engine.ctx, max_batch, seed,
                                              ctypes.byref(self.handle)))
         self.max_batch = max_batch
+        _open_children.add(self)
 
     def _check(self, rc: int) -> None:
         self.engine._check(rc)

@@ -178,6 +178,11 @@ int gpuai_net_match_run(const gpuai_net_match_config *cfg, const float *params,
     out->playout_iterations = r.playoutIterations;
     out->plies = r.plies;
     out->seconds = r.seconds;
+    for (int s = 0; s < 2; s++) {
+      out->select_ms[s] = r.selectMs[s];
+      out->score_ms[s] = r.scoreMs[s];
+      out->update_ms[s] = r.updateMs[s];
+    }
     if (outcome) std::copy(r.outcome.begin(), r.outcome.end(), outcome);
     return 0;
   } catch (const std::exception &e) {

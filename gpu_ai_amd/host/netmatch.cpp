@@ -25,10 +25,15 @@ struct Game {
   int8_t outcome = 0;
 };
 
-// One side's search phase over the games `ids` (whose mover is that side's player).
-template <class Eval>
+double msSince(Clock::time_point t) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+}
+
+// One side's search phase over the games `ids` (whose mover is that side's player): select
+// leaves in every tree, score them in one batch, update every tree, until moveMs has passed.
+template <class Score, class Update>
 void searchPhase(std::vector<Game> &games, const std::vector<uint32_t> &ids, int side,
-                 const NetMatchConfig &cfg, Eval eval, uint64_t &leaves, uint64_t &iterations) {
+                 const NetMatchConfig &cfg, Score score, Update update, NetMatchResult &res) {
   if (ids.empty()) return;
   const unsigned per = std::max<unsigned>(2u, cfg.leafBatch / (unsigned)ids.size());
   std::vector<std::vector<State>> sel(ids.size());
@@ -36,6 +41,7 @@ void searchPhase(std::vector<Game> &games, const std::vector<uint32_t> &ids, int
   std::vector<size_t> offs(ids.size() + 1);
   const auto t0 = Clock::now();
   do {
+    auto t = Clock::now();
 #pragma omp parallel for schedule(dynamic, 2)
     for (long i = 0; i < (long)ids.size(); i++) {
       sel[i].clear();
@@ -47,10 +53,16 @@ void searchPhase(std::vector<Game> &games, const std::vector<uint32_t> &ids, int
       batch.insert(batch.end(), sel[i].begin(), sel[i].end());
     }
     offs[ids.size()] = batch.size();
-    eval(batch, offs);
-    leaves += batch.size();
-    iterations++;
-  } while (std::chrono::duration<double, std::milli>(Clock::now() - t0).count() < cfg.moveMs);
+    res.selectMs[side] += msSince(t);
+    t = Clock::now();
+    score(batch);
+    res.scoreMs[side] += msSince(t);
+    t = Clock::now();
+    update(offs);
+    res.updateMs[side] += msSince(t);
+    (side == 0 ? res.netLeaves : res.playoutLeaves) += batch.size();
+    (side == 0 ? res.netIterations : res.playoutIterations)++;
+  } while (msSince(t0) < cfg.moveMs);
 }
 
 }  // namespace
@@ -80,13 +92,9 @@ NetMatchResult runNetMatch(const NetMatchConfig &cfg, const float *params) {
 
   // leaf scorers: player 1's win probability per leaf, or playout winners
   std::vector<float> p1;
-  auto netEval = [&](const std::vector<State> &batch, const std::vector<size_t> &) {
-    net.p1WinProb(batch, p1);
-  };
+  auto netScore = [&](const std::vector<State> &batch) { net.p1WinProb(batch, p1); };
   std::vector<PlayerId> winners;
-  auto playoutEval = [&](const std::vector<State> &batch, const std::vector<size_t> &) {
-    winners = driver.runPlayouts(batch);
-  };
+  auto playoutScore = [&](const std::vector<State> &batch) { winners = driver.runPlayouts(batch); };
 
   NetMatchResult res;
   res.games = cfg.games;
@@ -119,23 +127,21 @@ NetMatchResult runNetMatch(const NetMatchConfig &cfg, const float *params) {
     }
     for (int side = 0; side < 2; side++) {
       if (side == 0) {
-        searchPhase(games, ids[0], 0, cfg,
-                    [&](const std::vector<State> &b, const std::vector<size_t> &o) {
-                      netEval(b, o);
+        searchPhase(games, ids[0], 0, cfg, netScore,
+                    [&](const std::vector<size_t> &o) {
 #pragma omp parallel for schedule(dynamic, 2)
                       for (long i = 0; i < (long)ids[0].size(); i++)
                         games[ids[0][i]].tree[0]->updateValues(p1.data() + o[i], o[i + 1] - o[i]);
                     },
-                    res.netLeaves, res.netIterations);
+                    res);
       } else {
-        searchPhase(games, ids[1], 1, cfg,
-                    [&](const std::vector<State> &b, const std::vector<size_t> &o) {
-                      playoutEval(b, o);
+        searchPhase(games, ids[1], 1, cfg, playoutScore,
+                    [&](const std::vector<size_t> &o) {
 #pragma omp parallel for schedule(dynamic, 2)
                       for (long i = 0; i < (long)ids[1].size(); i++)
                         games[ids[1][i]].tree[1]->update(winners.data() + o[i], o[i + 1] - o[i]);
                     },
-                    res.playoutLeaves, res.playoutIterations);
+                    res);
       }
       for (uint32_t g : ids[side]) chosen[g] = games[g].tree[side]->getOptMove(games[g].state.turn);
     }

@@ -36,6 +36,9 @@ struct NetMatchResult {
   uint64_t netIterations = 0, playoutIterations = 0;
   uint64_t plies = 0;
   double seconds = 0;
+  // wall time of the search phases by part, [0] network player, [1] playout player: tree
+  // selection, leaf scoring (network evaluation or device playouts), tree update
+  double selectMs[2] = {0, 0}, scoreMs[2] = {0, 0}, updateMs[2] = {0, 0};
   std::vector<int8_t> outcome;  // per game: 1 network won, -1 playout player won, 0 draw
 };
 

@@ -42,7 +42,9 @@ class NetMatchResult(ctypes.Structure):
                 ("score", ctypes.c_double), ("score_stderr", ctypes.c_double),
                 ("net_leaves", ctypes.c_uint64), ("playout_leaves", ctypes.c_uint64),
                 ("net_iterations", ctypes.c_uint64), ("playout_iterations", ctypes.c_uint64),
-                ("plies", ctypes.c_uint64), ("seconds", ctypes.c_double)]
+                ("plies", ctypes.c_uint64), ("seconds", ctypes.c_double),
+                ("select_ms", ctypes.c_double * 2), ("score_ms", ctypes.c_double * 2),
+                ("update_ms", ctypes.c_double * 2)]
 
 
 def host_library() -> ctypes.CDLL:
@@ -135,5 +137,7 @@ def net_match(params, games: int = 400, seed: int = 2016, opening_min: int = 0,
                                           _ptr(outcome), err, 512) != 0:
         raise RuntimeError(err.value.decode())
     out = {k: getattr(res, k) for k, _ in NetMatchResult._fields_}
+    for k in ("select_ms", "score_ms", "update_ms"):  # [network player, playout player]
+        out[k] = [round(float(v), 1) for v in out[k]]
     out["outcome"] = outcome
     return out

@@ -85,6 +85,9 @@ typedef struct gpuai_net_match_result {
   double score, score_stderr;   /* network's (wins + draws / 2) / games */
   uint64_t net_leaves, playout_leaves, net_iterations, playout_iterations, plies;
   double seconds;
+  /* wall time of the search phases by part, [0] network player, [1] playout player: tree
+   * selection, leaf scoring (network evaluation or device playouts), tree update */
+  double select_ms[2], score_ms[2], update_ms[2];
 } gpuai_net_match_result;
 
 /* params: MI_MLP_PARAMS network weights; outcome (may be NULL): per game 1 network won,
