@@ -163,7 +163,9 @@ int gpuai_net_match_run(const gpuai_net_match_config *cfg, const float *params,
     c.playoutSeed = cfg->playout_seed;
     c.netBatch = cfg->net_batch;
     c.progressEvery = cfg->progress_every;
-    if (c.openingMax < c.openingMin || c.leafBatch == 0 || c.netBatch == 0 || !(c.moveMs >= 0))
+    c.netExploration = cfg->net_exploration;
+    if (c.openingMax < c.openingMin || c.leafBatch == 0 || c.netBatch == 0 || !(c.moveMs >= 0) ||
+        !(c.netExploration > 0))
       throw std::runtime_error("invalid match configuration");
     NetMatchResult r = runNetMatch(c, params);
     out->games = r.games;

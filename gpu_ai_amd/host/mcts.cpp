@@ -31,7 +31,9 @@ std::shared_ptr<GameTree> GameTree::move(const Move &m) {
   }
   State next(state);
   next.move(m);
-  return std::make_shared<GameTree>(next);
+  auto t = std::make_shared<GameTree>(next);
+  t->explore = explore;
+  return t;
 }
 
 double GameTree::getScore(PlayerId player) const {
@@ -79,6 +81,7 @@ void GameTree::selectInto(unsigned trials, std::vector<State> &out) {
         State next = state;
         next.move(m);
         children.push_back(std::make_shared<GameTree>(next, this));
+        children.back()->explore = explore;
       }
       expanded = true;
     } else {
@@ -95,7 +98,7 @@ void GameTree::selectInto(unsigned trials, std::vector<State> &out) {
   unsigned untried = 0;
   const double lnN = std::log((double)totalTrials);  // the parent's term, once per node
   for (size_t i = 0; i < n; i++) {
-    weight[i] = children[i]->ucb1(lnN);
+    weight[i] = children[i]->ucb1(lnN, explore);
     weightSum += weight[i];
     if (children[i]->totalTrials == 0) untried++;
     share[i] = 0;
@@ -202,11 +205,13 @@ void GameTree::updateValues(const float *p1WinProb, size_t n) {
 
 // The reference's arithmetic, including its long-double exploration term: the win rate of the
 // side to move in this child (src/mcts.cpp:189) plus sqrt(2 ln N / n).
-double GameTree::ucb1() const { return ucb1(std::log((double)parent->totalTrials)); }
+double GameTree::ucb1() const {
+  return ucb1(std::log((double)parent->totalTrials), parent->explore);
+}
 
 // lnParent = log(parent's trials) in double precision, as log(unsigned) evaluates in the
 // reference; the rest in long double as there.
-double GameTree::ucb1(double lnParent) const {
+double GameTree::ucb1(double lnParent, double k) const {
   if (totalTrials == 0) return INFINITY;
-  return (double)wins[state.turn] / totalTrials + std::sqrt(2.0L * lnParent / totalTrials);
+  return (double)wins[state.turn] / totalTrials + std::sqrt((long double)k * lnParent / totalTrials);
 }

@@ -40,6 +40,9 @@ class GameTree {
   // Whether select parallelises over the root's children with OpenMP (as the reference does at
   // every node, src/mcts.cpp:146-149); searches run in parallel with each other switch it off.
   void setParallel(bool on) { parallel = on; }
+  // The factor k of the exploration term sqrt(k ln N / n); the reference's UCB1 has k = 2
+  // (src/mcts.cpp:189-190), the default.  Nodes created later inherit it.
+  void setExploration(double k) { explore = k; }
 
   const State state;
 
@@ -52,7 +55,7 @@ class GameTree {
 
  private:
   double ucb1() const;  // src/mcts.cpp:182-191
-  double ucb1(double lnParent) const;
+  double ucb1(double lnParent, double k) const;
   template <class R, class Add>
   void updateImpl(const R *results, size_t n, Add add, double sums[NUM_PLAYERS]);
 
@@ -60,6 +63,7 @@ class GameTree {
   bool expanded = false;
   bool over = false;       // isGameOver() of the node's state, fixed at construction
   bool parallel = false;
+  double explore = 2.0;
   mutable std::vector<Move> moves;
   mutable bool movesReady = false;
   std::vector<std::shared_ptr<GameTree>> children;

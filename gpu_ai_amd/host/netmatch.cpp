@@ -88,6 +88,7 @@ NetMatchResult runNetMatch(const NetMatchConfig &cfg, const float *params) {
       G.tree[s] = std::make_shared<GameTree>(G.state);
       G.tree[s]->setParallel(false);
     }
+    G.tree[0]->setExploration(cfg.netExploration);
   }
 
   // leaf scorers: player 1's win probability per leaf, or playout winners

@@ -26,6 +26,9 @@ struct NetMatchConfig {
   uint32_t playoutSeed = 12345;
   uint32_t netBatch = 4096;      // network batch (max_batch)
   uint32_t progressEvery = 0;    // print a status line to stderr every this many plies (0: never)
+  // exploration factor k of the network player's tree, sqrt(k ln N / n) (the playout player
+  // keeps the reference's k = 2); the default is the run_ai network player's
+  double netExploration = 64.0;
   int device = 0;
 };
 

@@ -17,9 +17,7 @@ MCTSPlayer::MCTSPlayer(unsigned initialNumPlayouts, float numPlayoutsScale, doub
       numPlayoutsScale(numPlayoutsScale),
       timeout(timeout),
       playoutDriver(std::move(playoutDriver)),
-      tree(std::make_shared<GameTree>(getStartingState())) {
-  tree->setParallel(true);
-}
+      tree(newTree(getStartingState())) {}
 
 MCTSPlayer::MCTSPlayer(unsigned initialNumPlayouts, float numPlayoutsScale, double timeout,
                        std::unique_ptr<NetLeafEvaluator> net)
@@ -27,18 +25,22 @@ MCTSPlayer::MCTSPlayer(unsigned initialNumPlayouts, float numPlayoutsScale, doub
       numPlayoutsScale(numPlayoutsScale),
       timeout(timeout),
       net(std::move(net)),
-      tree(std::make_shared<GameTree>(getStartingState())) {
-  tree->setParallel(true);
-}
+      tree(newTree(getStartingState())) {}
 
 MCTSPlayer::~MCTSPlayer() { stop(); }
+
+std::shared_ptr<GameTree> MCTSPlayer::newTree(const State &state) const {
+  auto t = std::make_shared<GameTree>(state);
+  t->setParallel(true);
+  if (net) t->setExploration(kNetExploration);
+  return t;
+}
 
 Move MCTSPlayer::getMove(const State &state, bool verbose) {
   {
     std::lock_guard<std::mutex> lock(treeMutex);
     if (state != tree->state) {
-      tree = std::make_shared<GameTree>(state);
-      tree->setParallel(true);
+      tree = newTree(state);
       updateCancelled = true;
     }
   }
@@ -81,8 +83,7 @@ void MCTSPlayer::start() {
   }
   std::lock_guard<std::mutex> lock(treeMutex);
   if (running) return;
-  tree = std::make_shared<GameTree>(getStartingState());
-  tree->setParallel(true);
+  tree = newTree(getStartingState());
   workerError = nullptr;
   running = true;
   workerThread = std::thread(&MCTSPlayer::worker, this);

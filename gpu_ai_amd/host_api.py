@@ -34,7 +34,8 @@ class MultiSearchResult(ctypes.Structure):
 class NetMatchConfig(ctypes.Structure):
     _fields_ = [("games", _U32), ("seed", _U32), ("opening_min", _U32), ("opening_max", _U32),
                 ("move_ms", ctypes.c_double), ("leaf_batch", _U32), ("max_plies", _U32),
-                ("playout_seed", _U32), ("net_batch", _U32), ("progress_every", _U32)]
+                ("playout_seed", _U32), ("net_batch", _U32), ("progress_every", _U32),
+                ("net_exploration", ctypes.c_double)]
 
 
 class NetMatchResult(ctypes.Structure):
@@ -124,12 +125,12 @@ def multisearch(searches: int = 256, seed: int = 2016, playout_seed: int = 12345
 def net_match(params, games: int = 400, seed: int = 2016, opening_min: int = 0,
               opening_max: int = 6, move_ms: float = 100.0, leaf_batch: int = 4096,
               max_plies: int = 300, playout_seed: int = 12345, net_batch: int = 4096,
-              progress_every: int = 0) -> dict:
+              progress_every: int = 0, net_exploration: float = 64.0) -> dict:
     """Value-network leaves against playout leaves at equal wall time per move (netmatch.hpp);
     returns the tally, the network's score and the per-game outcomes (1 / -1 / 0)."""
     w = np.ascontiguousarray(params, dtype=np.float32)
     cfg = NetMatchConfig(games, seed, opening_min, opening_max, move_ms, leaf_batch, max_plies,
-                         playout_seed, net_batch, progress_every)
+                         playout_seed, net_batch, progress_every, net_exploration)
     res = NetMatchResult()
     outcome = np.zeros(games, dtype=np.int8)
     err = ctypes.create_string_buffer(512)

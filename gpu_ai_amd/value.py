@@ -108,6 +108,10 @@ def main(argv=None) -> int:
                     help="then play this many games of network leaves against playout leaves")
     ap.add_argument("--move-ms", type=float, default=100.0)
     ap.add_argument("--leaf-batch", type=int, default=4096)
+    ap.add_argument("--net-exploration", type=float, default=64.0,
+                    help="k of the network player's UCB1 term sqrt(k ln N / n) (the playout "
+                         "player keeps the reference's 2)")
+    ap.add_argument("--match-seed", type=int, default=2016, help="seed of the match openings")
     args = ap.parse_args(argv)
     eng = Engine(0)
     with ValueNet(eng, args.batch, seed=args.seed) as net:
@@ -131,9 +135,10 @@ def main(argv=None) -> int:
     eng.close()
     if args.match_games:
         from . import host_api
-        r = host_api.net_match(params, games=args.match_games, move_ms=args.move_ms,
+        r = host_api.net_match(params, games=args.match_games, seed=args.match_seed,
+                               move_ms=args.move_ms,
                                leaf_batch=args.leaf_batch, net_batch=args.batch,
-                               progress_every=25)
+                               progress_every=25, net_exploration=args.net_exploration)
         r["outcome"] = "".join({1: "W", -1: "L", 0: "D"}[int(x)] for x in r["outcome"])
         print(json.dumps({"match": r}), flush=True)
     return 0

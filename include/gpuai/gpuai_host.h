@@ -78,6 +78,8 @@ typedef struct gpuai_net_match_config {
   uint32_t playout_seed;   /* (12345) */
   uint32_t net_batch;      /* network batch (4096) */
   uint32_t progress_every; /* status line on stderr every this many plies (0: none) */
+  double net_exploration;  /* k of the network player's sqrt(k ln N / n) (64; the playout
+                            * player keeps the reference's 2) */
 } gpuai_net_match_config;
 
 typedef struct gpuai_net_match_result {

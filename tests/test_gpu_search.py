@@ -116,3 +116,25 @@ def test_run_ai_net_leaf_player(tmp_path):
                           "random", "--timeout", "0.02"], capture_output=True, text=True,
                          timeout=120, env=env)
     assert out.returncode != 0 and "cannot load the value network" in out.stdout + out.stderr
+
+
+def test_net_match_small():
+    """The net-leaf vs playout-leaf match harness on a few short games: consistent tallies and
+    per-part times, both exploration settings, and a rejected configuration."""
+    from gpu_ai_amd import host_api
+    from gpu_ai_amd.engine import Engine, ValueNet
+    with Engine(0) as eng, ValueNet(eng, 4096, seed=7) as net:
+        params = net.get_params()
+    for k in (2.0, 0.5):
+        announce(f"net match: 4 games, 10 ms per move, 40 plies, exploration {k}")
+        r = host_api.net_match(params, games=4, move_ms=10.0, max_plies=40, net_exploration=k)
+        assert r["games"] == 4
+        assert r["net_wins"] + r["playout_wins"] + r["draws"] == 4
+        out = np.asarray(r["outcome"])
+        assert sorted(set(out.tolist())) <= [-1, 0, 1]
+        assert int(np.sum(out == 1)) == r["net_wins"] and int(np.sum(out == -1)) == r["playout_wins"]
+        assert r["net_leaves"] > 0 and r["playout_leaves"] > 0 and r["plies"] > 0
+        assert all(v >= 0.0 for key in ("select_ms", "score_ms", "update_ms") for v in r[key])
+        assert r["score_ms"][0] > 0.0 and r["score_ms"][1] > 0.0
+    with pytest.raises(RuntimeError):
+        host_api.net_match(params, games=2, move_ms=5.0, max_plies=10, net_exploration=0.0)
