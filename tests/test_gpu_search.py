@@ -131,7 +131,7 @@ def test_net_match_small():
         assert r["games"] == 4
         assert r["net_wins"] + r["playout_wins"] + r["draws"] == 4
         out = np.asarray(r["outcome"])
-        assert sorted(set(out.tolist())) <= [-1, 0, 1]
+        assert set(out.tolist()) <= {-1, 0, 1}
         assert int(np.sum(out == 1)) == r["net_wins"] and int(np.sum(out == -1)) == r["playout_wins"]
         assert r["net_leaves"] > 0 and r["playout_leaves"] > 0 and r["plies"] > 0
         assert all(v >= 0.0 for key in ("select_ms", "score_ms", "update_ms") for v in r[key])
