@@ -329,8 +329,9 @@ int mi_destroy(mi_ctx *ctx) {
   if (!ctx) return MI_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  DevBuf *bufs[] = {&ctx->pos,     &ctx->offs,    &ctx->winners, &ctx->plies, &ctx->hashes,
-                    &ctx->work,    &ctx->tallies, &ctx->moves,   &ctx->counts};
+  DevBuf *bufs[] = {&ctx->pos,   &ctx->offs,    &ctx->winners, &ctx->plies,  &ctx->hashes,
+                    &ctx->work,  &ctx->tallies, &ctx->moves,   &ctx->counts, &ctx->ord_pos,
+                    &ctx->order, &ctx->ord_offs};
   for (DevBuf *b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
