@@ -50,6 +50,13 @@ namespace {
 
 uint32_t round64(uint32_t n) { return (n + 63u) & ~63u; }
 
+// Makes the network's device current on the calling thread (as every playout entry point does).
+int use_device(mi_mlp *n) {
+  if (hipSetDevice(n->ctx->device) != hipSuccess)
+    return fail(n->ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  return MI_OK;
+}
+
 mi_pos *stage_pos(mi_mlp *n) { return (mi_pos *)n->h_stage; }
 float *stage_values(mi_mlp *n) { return (float *)((char *)n->h_stage + sizeof(mi_pos) * n->max_rows); }
 float *stage_targets(mi_mlp *n) { return stage_values(n) + n->max_rows; }
@@ -293,6 +300,7 @@ int mi_mlp_get_params(mi_mlp *n, float *params) {
   if (!n || !params) return MI_ERR_INVALID_ARGUMENT;
   mi_ctx *ctx = n->ctx;
   std::lock_guard<std::mutex> lock(ctx->mu);
+  if (int rc = use_device(n)) return rc;
   MI_HIP(ctx, hipMemcpyAsync(params, n->w, sizeof(float) * MI_MLP_PARAMS, hipMemcpyDeviceToHost,
                              ctx->stream));
   MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -303,6 +311,7 @@ int mi_mlp_set_params(mi_mlp *n, const float *params) {
   if (!n || !params) return MI_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(n->ctx->mu);
   n->ctx->last_error.clear();
+  if (int rc = use_device(n)) return rc;
   return upload_params(n, params);
 }
 
@@ -315,6 +324,7 @@ int mi_mlp_forward(mi_mlp *n, const mi_pos *pos, uint32_t count, float *values) 
   if ((rc = check_batch(n, count, pos))) return rc;
   if (count == 0) return MI_OK;
   if (!values) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "null values");
+  if ((rc = use_device(n))) return rc;
   // one launch: encoding, both hidden layers and the output layer (k_forward)
   if ((rc = upload_positions(n, pos, count))) return rc;
   MI_HIP(ctx, K::forward(n->pos, count, n->W1t, n->w + MI_MLP_OFF_B1, n->W2t, n->w + MI_MLP_OFF_B2,
@@ -334,6 +344,7 @@ static int grad_common(mi_mlp *n, const mi_pos *pos, const float *targets, uint3
   int rc;
   if ((rc = check_batch(n, count, pos))) return rc;
   if (count == 0 || !targets) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "empty batch or null targets");
+  if ((rc = use_device(n))) return rc;
   uint32_t rows;
   if ((rc = stage_batch(n, pos, targets, count, rows))) return rc;
   if ((rc = forward_rows(n, count, rows))) return rc;
@@ -364,6 +375,7 @@ int mi_mlp_adam_state(mi_mlp *n, float *m, float *v, uint32_t *step) {
   if (!n) return MI_ERR_INVALID_ARGUMENT;
   mi_ctx *ctx = n->ctx;
   std::lock_guard<std::mutex> lock(ctx->mu);
+  if (int rc = use_device(n)) return rc;
   if (m) MI_HIP(ctx, hipMemcpyAsync(m, n->m, sizeof(float) * MI_MLP_PARAMS, hipMemcpyDeviceToHost, ctx->stream));
   if (v) MI_HIP(ctx, hipMemcpyAsync(v, n->v, sizeof(float) * MI_MLP_PARAMS, hipMemcpyDeviceToHost, ctx->stream));
   MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -377,6 +389,7 @@ int mi_mlp_set_dataset(mi_mlp *n, const mi_pos *pos, const float *targets, uint3
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->last_error.clear();
   int rc;
+  if ((rc = use_device(n))) return rc;
   if ((rc = ensure(ctx, n->ds_pos, sizeof(mi_pos) * (size_t)std::max<uint32_t>(count, 1)))) return rc;
   if ((rc = ensure(ctx, n->ds_t, sizeof(float) * (size_t)std::max<uint32_t>(count, 1)))) return rc;
   if ((rc = ensure(ctx, n->ds_perm, sizeof(uint32_t) * (size_t)std::max<uint32_t>(count, 1)))) return rc;
@@ -397,6 +410,7 @@ int mi_mlp_train_epoch(mi_mlp *n, uint32_t batch, float lr, uint32_t shuffle_see
     return fail(ctx, MI_ERR_INVALID_ARGUMENT, "batch must be a multiple of 64 within max_batch");
   const uint32_t nb = n->ds_n / batch;
   if (nb == 0) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "the dataset holds less than one batch");
+  if (int rc0 = use_device(n)) return rc0;
   // Fisher-Yates with Philox key (shuffle_seed, epoch), counter (i >> 2, 0, 3, 1)
   std::vector<uint32_t> perm(n->ds_n);
   std::iota(perm.begin(), perm.end(), 0u);
@@ -464,7 +478,8 @@ int mi_mlp_load(mi_mlp *n, const char *path) {
   if (!ok) return fail(n->ctx, MI_ERR_INVALID_ARGUMENT, std::string("not a checkpoint: ") + path);
   mi_ctx *ctx = n->ctx;
   std::lock_guard<std::mutex> lock(ctx->mu);
-  int rc = upload_params(n, w.data());
+  int rc = use_device(n);
+  if (!rc) rc = upload_params(n, w.data());
   if (rc) return rc;
   MI_HIP(ctx, hipMemcpyAsync(n->m, m.data(), sizeof(float) * MI_MLP_PARAMS, hipMemcpyHostToDevice, ctx->stream));
   MI_HIP(ctx, hipMemcpyAsync(n->v, v.data(), sizeof(float) * MI_MLP_PARAMS, hipMemcpyHostToDevice, ctx->stream));
