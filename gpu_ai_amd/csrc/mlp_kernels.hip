@@ -273,32 +273,45 @@ __global__ void __launch_bounds__(256) k_head_train(const uint16_t *__restrict__
   const float bias = b3[0];
   float acc_b3 = 0.f, acc_loss = 0.f;
   const uint32_t row0 = blockIdx.x * 64u + wv * 16u;
+  // the wave's 16 targets, fetched together up front (lane i holds row i's) instead of one
+  // dependent pair of loads per row
+  float tgt = 0.f;
+  if (lane < 16 && row0 + lane < n) tgt = targets[tidx ? tidx[row0 + lane] : row0 + lane];
+  // the 16 rows' activations, loaded together (4 BF16 per lane and row, one 8-byte load each);
+  // the row loop is unrolled so that they stay in registers
+  uint2 hrow[16];
+#pragma unroll
+  for (uint32_t i = 0; i < 16u; i++)
+    hrow[i] = *(const uint2 *)(H2 + (size_t)(row0 + i) * 256 + lane * 4);
+#pragma unroll
   for (uint32_t i = 0; i < 16u; i++) {
     const uint32_t row = row0 + i;
-    const uint16_t *h = H2 + (size_t)row * 256 + lane * 4;
-    float hv[4], dot = 0.f;
+    const float target = __shfl(tgt, (int)i);
+    const uint2 h = hrow[i];
+    const float hv[4] = {from_bf((uint16_t)(h.x & 0xFFFFu)), from_bf((uint16_t)(h.x >> 16)),
+                         from_bf((uint16_t)(h.y & 0xFFFFu)), from_bf((uint16_t)(h.y >> 16))};
+    float dot = 0.f;
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      hv[c] = from_bf(h[c]);
-      dot += hv[c] * wl[c];
-    }
+    for (int c = 0; c < 4; c++) dot += hv[c] * wl[c];
     for (int off = 32; off >= 1; off >>= 1) dot += __shfl_xor(dot, off);
     const float yy = tanhf(dot + bias);
     float d = 0.f;
     if (row < n) {
-      const float e = yy - targets[tidx ? tidx[row] : row];
+      const float e = yy - target;
       d = 2.f * e * (1.f - yy * yy) / (float)n;
       acc_loss += e * e / (float)n;
     }
     if (lane == 0) dz[row] = d;
     acc_b3 += d;
+    uint16_t gb[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      const uint16_t gb = to_bf(hv[c] > 0.f ? d * wl[c] : 0.f);
-      dH2[(size_t)row * 256 + lane * 4 + c] = gb;
-      acc_b2[c] += from_bf(gb);
+      gb[c] = to_bf(hv[c] > 0.f ? d * wl[c] : 0.f);
+      acc_b2[c] += from_bf(gb[c]);
       acc_w3[c] += d * hv[c];
     }
+    *(uint2 *)(dH2 + (size_t)row * 256 + lane * 4) =
+        uint2{gb[0] | ((uint32_t)gb[1] << 16), gb[2] | ((uint32_t)gb[3] << 16)};
   }
 #pragma unroll
   for (int c = 0; c < 4; c++) {
@@ -353,30 +366,15 @@ __global__ void __launch_bounds__(256) k_colsum(const T *__restrict__ src, uint3
 // b2, w3, b3 at 256-768, in parameter order).  The summed gradient is written to `grads` (for
 // inspection).  Refreshes the BF16 copies: W1t[o][i], W2t[o][i] (forward B operands) and
 // W2r[i][o] (backward).
-__global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= MI_MLP_PARAMS) return;
-  float g = 0.f;
-  if (p < MI_MLP_OFF_B1) {
-#pragma unroll 8
-    for (uint32_t s = 0; s < a.splits; s++) g += a.pW1[(size_t)s * 32768 + p];
-  } else if (p < MI_MLP_OFF_W2) {
-#pragma unroll 8
-    for (uint32_t s = 0; s < a.vsplits; s++) g += a.pV[(size_t)s * kVecSlots + (p - MI_MLP_OFF_B1)];
-  } else if (p < MI_MLP_OFF_B2) {
-#pragma unroll 8
-    for (uint32_t s = 0; s < a.splits; s++) g += a.pW2[(size_t)s * 65536 + (p - MI_MLP_OFF_W2)];
-  } else {
-#pragma unroll 8
-    for (uint32_t s = 0; s < a.vsplits; s++)
-      g += a.pV[(size_t)s * kVecSlots + 256 + (p - MI_MLP_OFF_B2)];
-  }
+// Adam over 16x16 tiles of W1 ([128][256]) and W2 ([256][256]), then the 769 vector parameters
+// (b1, b2, w3, b3; slot v of the column-sum partials).  Each parameter's gradient is the sum of
+// its partials in split order; the BF16 copies are refreshed, the transposed ones (W1t, W2t:
+// [out][in]) through an LDS tile so that their stores are contiguous.
+constexpr uint32_t AT = 16, kAdamW1 = (128 / AT) * (256 / AT), kAdamW2 = (256 / AT) * (256 / AT);
+constexpr uint32_t kAdamVec = (769 + 255) / 256;
+
+__device__ __forceinline__ float adam_step(const AdamArgs &a, uint32_t p, float g) {
   a.grads[p] = g;
-  if (p == 0u && a.loss_out) {  // the batch loss: sum of the head's per-block partials
-    float l = 0.f;
-    for (uint32_t s = 0; s < a.vsplits; s++) l += a.pV[(size_t)s * kVecSlots + 769];
-    *a.loss_out = l;
-  }
   float w = a.w[p];
   if (a.update) {
     const float m = a.beta1 * a.m[p] + a.omb1 * g;
@@ -386,14 +384,49 @@ __global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
     w -= a.lr * (m * a.c1) / (sqrtf(v * a.c2) + a.eps);
     a.w[p] = w;
   }
-  if (p < MI_MLP_OFF_B1) {
-    const uint32_t i = p >> 8, o = p & 255;
-    a.W1t[o * 128 + i] = to_bf(w);
-  } else if (p >= MI_MLP_OFF_W2 && p < MI_MLP_OFF_B2) {
-    const uint32_t q = p - MI_MLP_OFF_W2, i = q >> 8, o = q & 255;
-    a.W2t[o * 256 + i] = to_bf(w);
-    a.W2r[q] = to_bf(w);
+  return w;
+}
+
+__global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
+  __shared__ uint16_t tile[AT][AT + 2];
+  const uint32_t t = threadIdx.x, b = blockIdx.x;
+  if (b < kAdamW1 + kAdamW2) {
+    const bool w1 = b < kAdamW1;
+    const uint32_t tb = w1 ? b : b - kAdamW1;
+    const uint32_t i0 = (tb / (256 / AT)) * AT, o0 = (tb % (256 / AT)) * AT;
+    const uint32_t il = t / AT, ol = t % AT;
+    const uint32_t q = (i0 + il) * 256 + o0 + ol;  // index within the matrix ([in][out])
+    const float *pW = w1 ? a.pW1 : a.pW2;
+    const size_t stride = w1 ? 32768 : 65536;
+    float g = 0.f;
+#pragma unroll 8
+    for (uint32_t s = 0; s < a.splits; s++) g += pW[(size_t)s * stride + q];
+    const uint16_t h = to_bf(adam_step(a, (w1 ? MI_MLP_OFF_W1 : MI_MLP_OFF_W2) + q, g));
+    if (!w1) a.W2r[q] = h;
+    tile[ol][il] = h;
+    __syncthreads();
+    const uint32_t ro = t / AT, ri = t % AT;  // row (out) and column (in) of the transposed copy
+    (w1 ? a.W1t : a.W2t)[(o0 + ro) * (w1 ? 128u : 256u) + i0 + ri] = tile[ro][ri];
+    return;
   }
+  if (b == kAdamW1 + kAdamW2 && t < 64u && a.loss_out) {
+    // the batch loss: the sum of the head's per-block partials in block order; the first wave
+    // loads 64 partials at a time and every lane adds them in order from the lanes that hold
+    // them
+    float l = 0.f;
+    for (uint32_t base = 0; base < a.vsplits; base += 64u) {
+      const float part = base + t < a.vsplits ? a.pV[(size_t)(base + t) * kVecSlots + 769] : 0.f;
+      const uint32_t cnt = min(64u, a.vsplits - base);
+      for (uint32_t j = 0; j < cnt; j++) l += __shfl(part, (int)j);
+    }
+    if (t == 0u) *a.loss_out = l;
+  }
+  const uint32_t v = (b - kAdamW1 - kAdamW2) * 256 + t;  // vector slot: b1 0-255, b2 256-511,
+  if (v >= 769u) return;                                 // w3 512-767, b3 768
+  float g = 0.f;
+#pragma unroll 8
+  for (uint32_t s = 0; s < a.vsplits; s++) g += a.pV[(size_t)s * kVecSlots + v];
+  adam_step(a, v < 256u ? MI_MLP_OFF_B1 + v : MI_MLP_OFF_B2 + (v - 256u), g);
 }
 
 __global__ void k_zero(float *p, uint32_t n) {
@@ -450,7 +483,7 @@ hipError_t colsum_f32(const float *src, uint32_t rows, uint32_t cols, const floa
 }
 
 hipError_t adam(const AdamArgs &a, hipStream_t stream) {
-  hipLaunchKernelGGL(k_adam, dim3((MI_MLP_PARAMS + 255) / 256), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k_adam, dim3(kAdamW1 + kAdamW2 + kAdamVec), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
