@@ -54,27 +54,38 @@ NetLeafEvaluator::~NetLeafEvaluator() {
 
 void NetLeafEvaluator::p1WinProb(const std::vector<State> &leaves, std::vector<float> &out) {
   // host buffers are local, so calls from several threads only meet in the context's lock
-  std::vector<mi_pos> packed;
+  const size_t count = leaves.size();
+  std::vector<mi_pos> all(count), packed;
+  std::vector<uint8_t> over(count);
   std::vector<uint32_t> index;
-  out.assign(leaves.size(), 0.5f);
-  packed.reserve(leaves.size());
-  index.reserve(leaves.size());
-  for (size_t i = 0; i < leaves.size(); i++) {
+  out.assign(count, 0.5f);
+  // finished games are scored by their result, the rest packed for the network (the test and
+  // the packing are most of the host time of a call, so they run in parallel over the leaves)
+#pragma omp parallel for schedule(static) if (count >= 1024)
+  for (long i = 0; i < (long)count; i++) {
     const State &s = leaves[i];
-    if (s.isGameOver()) {
+    over[i] = s.isGameOver() ? 1 : 0;
+    if (over[i]) {
       const PlayerId w = s.getWinner();
       out[i] = w == PLAYER_1 ? 1.f : w == PLAYER_2 ? 0.f : 0.5f;
     } else {
-      packed.push_back(s.pack());
-      index.push_back((uint32_t)i);
+      all[i] = s.pack();
     }
   }
+  packed.reserve(count);
+  index.reserve(count);
+  for (size_t i = 0; i < count; i++)
+    if (!over[i]) {
+      packed.push_back(all[i]);
+      index.push_back((uint32_t)i);
+    }
   std::vector<float> values(packed.size());
   for (size_t a = 0; a < packed.size(); a += maxBatch_) {
     const uint32_t n = (uint32_t)std::min<size_t>(maxBatch_, packed.size() - a);
     checkMi(mi_mlp_forward(net_, packed.data() + a, n, values.data() + a), ctx_, "mi_mlp_forward");
   }
-  for (size_t j = 0; j < index.size(); j++) {
+#pragma omp parallel for schedule(static) if (index.size() >= 1024)
+  for (long j = 0; j < (long)index.size(); j++) {
     const float v = values[j];
     out[index[j]] = leaves[index[j]].turn == PLAYER_1 ? 0.5f * (1.f + v) : 0.5f * (1.f - v);
   }
