@@ -468,8 +468,10 @@ int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
   if ((rc = upload_batch(ctx, pos, npos, counts, total))) return rc;
   ctx->staged = false;
   const mi_tuning saved = ctx->tuning;
-  const uint32_t blocks[] = {64u, 128u, 256u, 512u, 1024u};
-  const uint32_t waves_per_simd[] = {1u, 2u, 4u, 8u};
+  const uint32_t blocks[] = {64u, 128u, 192u, 256u, 512u, 768u, 1024u};
+  // 3 and 4 waves per SIMD are the fast ones on random-1024x1024 (more lanes in flight lengthen
+  // the end of a batch, fewer leave the SIMDs short of waves to issue from)
+  const uint32_t waves_per_simd[] = {1u, 2u, 3u, 4u, 8u};
   mi_tuning best_t = saved;
   float best_ms = 3.0e38f;
   uint32_t n = 0;
@@ -480,6 +482,7 @@ int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
        for (uint32_t dm : defers) {
         const uint32_t waves_per_cu = 4u * w;           // four SIMDs per CU
         if ((b / 64u) > waves_per_cu) continue;          // a block larger than the residency
+        if ((waves_per_cu * 64u) % b) continue;          // whole blocks only
         mi_tuning t = {b, waves_per_cu * 64u / b, refill, dm};
         ctx->tuning = t;
         std::vector<float> ms;

@@ -300,15 +300,15 @@ class Engine:
         """Sweeps launch shapes on this batch; the fastest becomes the context's tuning."""
         pos = as_positions(pos)
         c, pp, _ = self._counts(len(pos), per_pos, counts)
-        res = (MiTuneResult * 128)()
+        res = (MiTuneResult * 256)()
         nres = _U32()
         best = MiTuning()
         self._check(self.lib.mi_autotune(self.ctx, _ptr(pos), len(pos), _ptr(c), pp, policy, reps,
-                                         res, 128, ctypes.byref(nres), ctypes.byref(best)))
+                                         res, 256, ctypes.byref(nres), ctypes.byref(best)))
         rows = [{"block_size": r.tuning.block_size, "blocks_per_cu": r.tuning.blocks_per_cu,
                  "refill": r.tuning.refill, "defer_min": r.tuning.defer_min,
                  "kernel_ms": r.kernel_ms, "playouts_per_s": r.playouts_per_s}
-                for r in res[:min(nres.value, 128)]]
+                for r in res[:min(nres.value, 256)]]
         return ({"block_size": best.block_size, "blocks_per_cu": best.blocks_per_cu,
                  "refill": best.refill, "defer_min": best.defer_min}, rows)
 

@@ -251,3 +251,27 @@ def test_valu_rate_probe(engine):
         engine.valu_rate(iters=0)
     with pytest.raises(Exception):
         engine.valu_rate(iters=(1 << 20) + 1)
+
+
+def test_autotune_picks_the_fastest_candidate(engine, golden):
+    """mi_autotune measures every launch shape on the batch, keeps the fastest as the context's
+    tuning, and the shape changes nothing but the schedule."""
+    pos = golden["positions"]["open1024"][:128]
+    before = engine.run_playouts(pos, per_pos=256, seed=21, epoch=2).winners
+    announce("mi_autotune on 128 positions x 256 playouts")
+    best, rows = engine.autotune(pos, per_pos=256, reps=1)
+    assert len(rows) >= 100
+    shapes = {(r["block_size"], r["blocks_per_cu"], r["refill"], r["defer_min"]) for r in rows}
+    key = (best["block_size"], best["blocks_per_cu"], best["refill"], best["defer_min"])
+    assert key in shapes
+    def shape(r):
+        return (r["block_size"], r["blocks_per_cu"], r["refill"], r["defer_min"])
+    assert len(shapes) == len(rows)  # each shape measured once
+    best_ms = next(r["kernel_ms"] for r in rows if shape(r) == key)
+    assert best_ms == min(r["kernel_ms"] for r in rows)
+    for r in rows:  # every candidate fills whole SIMDs with whole blocks
+        assert (r["block_size"] * r["blocks_per_cu"]) % 256 == 0
+    assert engine.tuning == best
+    after = engine.run_playouts(pos, per_pos=256, seed=21, epoch=2).winners
+    assert np.array_equal(before, after)
+    engine.set_tuning(512, 2, REFILL_QUEUE, 8)  # back to the defaults for the tests that follow
