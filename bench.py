@@ -135,7 +135,8 @@ def bench_multisearch() -> int:
     """Batched multi-search: 256 concurrent searches, budgets uniform in [1000, 64000] playouts,
     UCT c = sqrt(2).  One graph replay (persistent refill kernel) per iteration, the same with
     two halves of the searches overlapped with the host's tree work (pipelined), and one playout
-    launch per search per iteration (the baseline)."""
+    launch per search per iteration (the baseline); each mode's launch shape comes from the
+    autotuner, run before its timed search."""
     from gpu_ai_amd import host_api
     runs = {m: host_api.multisearch(mode=m) for m in ("graph", "pipelined", "per_search")}
     g, pl, p = runs["graph"], runs["pipelined"], runs["per_search"]
@@ -147,7 +148,10 @@ def bench_multisearch() -> int:
         "higher_is_better": True, "dtype": "int32", "data": "synthetic (seeded positions)",
         "config": {"workload": "multisearch-256: 256 searches from seeded distinct positions, budgets "
                                "uniform in [1000, 64000] playouts, 256 trials per search "
-                               "per iteration, UCT c=sqrt(2)", "mode": best[0]},
+                               "per iteration, UCT c=sqrt(2)", "mode": best[0],
+                   "launch_shape": {k: best[1][k] for k in ("block_size", "blocks_per_cu",
+                                                            "refill", "defer_min")},
+                   "autotuned": True},
         "graph": fmt(g), "pipelined": fmt(pl), "per_search": fmt(p),
         "speedup_vs_per_search": round(best[1]["playouts_per_second"] / p["playouts_per_second"], 3),
     }

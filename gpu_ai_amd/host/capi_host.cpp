@@ -122,6 +122,7 @@ int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch
     c.policy = cfg->policy;
     c.mode = (MultiSearchMode)cfg->mode;
     c.groups = cfg->groups ? cfg->groups : 2;
+    c.autotune = cfg->autotune != 0;
     if (c.maxBudget < c.minBudget || c.searches == 0 || cfg->mode > 3)
       throw std::runtime_error("invalid multi-search configuration");
     MultiSearchResult r = runMultiSearch(c);
@@ -138,6 +139,7 @@ int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch
     out->select_ms = r.selectMs;
     out->playout_ms = r.playoutMs;
     out->update_ms = r.updateMs;
+    out->tuning = r.tuning;
     return 0;
   } catch (const std::exception &e) {
     if (err && err_len) {

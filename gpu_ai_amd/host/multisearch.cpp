@@ -52,6 +52,20 @@ void multiSearchSetup(const MultiSearchConfig &cfg, std::vector<mi_pos> &roots,
 
 namespace {
 
+// The launch shape for the searches' batches: mi_autotune on a batch of the same kind as an
+// iteration's (every root with `initial` playouts), or the context's default.
+mi_tuning tuneSearches(mi_ctx *ctx, const MultiSearchConfig &cfg, const std::vector<mi_pos> &roots) {
+  mi_tuning t = {};
+  if (cfg.autotune) {
+    checkMi(mi_autotune(ctx, roots.data(), (uint32_t)roots.size(), nullptr,
+                        std::max<uint32_t>(cfg.initial, 1u), cfg.policy, 1, nullptr, 0, nullptr, &t),
+            ctx, "mi_autotune");
+  } else {
+    checkMi(mi_get_tuning(ctx, &t), ctx, "mi_get_tuning");
+  }
+  return t;
+}
+
 // Latency percentiles, throughput and the digest of every root's statistics.
 void finishResult(MultiSearchResult &res, const std::vector<std::unique_ptr<GameTree>> &trees) {
   std::vector<double> lat = res.iterationMs;
@@ -95,28 +109,33 @@ double msSince(Clock::time_point a) {
 // still alternates select -> playouts -> update.
 MultiSearchResult runPipelined(const MultiSearchConfig &cfg,
                                std::vector<std::unique_ptr<GameTree>> &trees,
-                               std::vector<uint32_t> &remaining) {
+                               std::vector<uint32_t> &remaining, const std::vector<mi_pos> &roots) {
   const uint32_t S = (uint32_t)trees.size();
   const uint32_t ngroups = std::max<uint32_t>(1u, std::min<uint32_t>(cfg.groups, S));
   std::vector<std::unique_ptr<SearchGroup>> groups;
   for (uint32_t k = 0; k < ngroups; k++) {
     groups.emplace_back(new SearchGroup());
     SearchGroup &Gk = *groups.back();
-    uint64_t cap = 0;
-    for (uint32_t s = k; s < S; s += ngroups) {
-      Gk.ids.push_back(s);
-      cap += remaining[s];
-    }
+    for (uint32_t s = k; s < S; s += ngroups) Gk.ids.push_back(s);
     // independent streams: group k draws under seed + k
     Gk.driver = std::make_unique<DeviceMI355XPlayoutDriver>(cfg.policy, cfg.device,
                                                             cfg.playoutSeed + k);
+  }
+  // one launch shape for every group (a graph keeps the shape it was created with)
+  MultiSearchResult res;
+  res.tuning = tuneSearches(groups[0]->driver->context(), cfg, roots);
+  for (uint32_t k = 0; k < ngroups; k++) {
+    SearchGroup &Gk = *groups[k];
+    checkMi(mi_set_tuning(Gk.driver->context(), &res.tuning), Gk.driver->context(),
+            "mi_set_tuning");
+    uint64_t cap = 0;
+    for (uint32_t s : Gk.ids) cap += remaining[s];
     cap = std::max<uint64_t>(cap, 1);
     checkMi(mi_graph_create(Gk.driver->context(), (uint32_t)std::min<uint64_t>(cap, 1u << 22),
                             (uint32_t)std::min<uint64_t>(cap, (1u << 31) - 1), cfg.policy,
                             &Gk.graph),
             Gk.driver->context(), "mi_graph_create");
   }
-  MultiSearchResult res;
   std::vector<std::vector<State>> leaves(S);
   std::vector<std::vector<PlayerId>> results(S);
 
@@ -221,11 +240,12 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
   for (uint32_t b : budgets) maxIter += b;
 
   if (cfg.mode == MultiSearchMode::PIPELINED) {
-    MultiSearchResult res = runPipelined(cfg, trees, remaining);
+    MultiSearchResult res = runPipelined(cfg, trees, remaining, roots);
     finishResult(res, trees);
     return res;
   }
   DeviceMI355XPlayoutDriver driver(cfg.policy, cfg.device, cfg.playoutSeed);
+  const mi_tuning tuning = tuneSearches(driver.context(), cfg, roots);
   mi_graph *graph = nullptr;
   if (cfg.mode == MultiSearchMode::GRAPH) {
     // the largest batch: every search's largest possible iteration
@@ -238,6 +258,7 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
   }
 
   MultiSearchResult res;
+  res.tuning = tuning;
   std::vector<std::vector<State>> leaves(S);
   std::vector<std::vector<PlayerId>> results(S);
   std::vector<mi_pos> pos;

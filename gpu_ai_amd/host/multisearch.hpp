@@ -33,6 +33,9 @@ struct MultiSearchConfig {
   uint32_t policy = MI_POLICY_RANDOM;
   MultiSearchMode mode = MultiSearchMode::GRAPH;
   uint32_t groups = 2;           // pipelined mode: groups of searches (device streams)
+  // launch shape from the autotuner (mi_autotune on every root with `initial` playouts, before
+  // the timed search; the shape changes no result), else the library's default
+  bool autotune = true;
   int device = 0;
 };
 
@@ -47,6 +50,7 @@ struct MultiSearchResult {
   double selectMs = 0, playoutMs = 0, updateMs = 0;
   double playoutsPerSecond = 0;
   uint64_t rootDigest = 0;       // digest of every root's statistics (determinism checks)
+  mi_tuning tuning = {};         // the launch shape the searches ran with
   std::vector<double> iterationMs;
 };
 

@@ -19,7 +19,7 @@ MODES = {"graph": 0, "batched": 1, "per_search": 2, "pipelined": 3}
 class MultiSearchConfig(ctypes.Structure):
     _fields_ = [("searches", _U32), ("seed", _U32), ("playout_seed", _U32), ("min_budget", _U32),
                 ("max_budget", _U32), ("initial", _U32), ("scale", ctypes.c_double),
-                ("policy", _U32), ("mode", _U32), ("groups", _U32)]
+                ("policy", _U32), ("mode", _U32), ("groups", _U32), ("autotune", _U32)]
 
 
 class MultiSearchResult(ctypes.Structure):
@@ -28,7 +28,9 @@ class MultiSearchResult(ctypes.Structure):
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("max_ms", ctypes.c_double), ("playouts_per_second", ctypes.c_double),
                 ("root_digest", ctypes.c_uint64), ("select_ms", ctypes.c_double),
-                ("playout_ms", ctypes.c_double), ("update_ms", ctypes.c_double)]
+                ("playout_ms", ctypes.c_double), ("update_ms", ctypes.c_double),
+                ("block_size", _U32), ("blocks_per_cu", _U32), ("refill", _U32),
+                ("defer_min", _U32)]
 
 
 class NetMatchConfig(ctypes.Structure):
@@ -110,11 +112,11 @@ def run_ai_path() -> str:
 def multisearch(searches: int = 256, seed: int = 2016, playout_seed: int = 12345,
                 min_budget: int = 1000, max_budget: int = 64000, initial: int = 256,
                 scale: float = 0.0, policy: int = 0, mode: str = "graph",
-                groups: int = 2) -> dict:
+                groups: int = 2, autotune: bool = True) -> dict:
     """Batched multi-search: concurrent searches sharing the device; returns the run's
     statistics."""
     cfg = MultiSearchConfig(searches, seed, playout_seed, min_budget, max_budget, initial, scale,
-                            policy, MODES[mode], groups)
+                            policy, MODES[mode], groups, int(autotune))
     res = MultiSearchResult()
     err = ctypes.create_string_buffer(512)
     if host_library().gpuai_multisearch_run(ctypes.byref(cfg), ctypes.byref(res), err, 512) != 0:

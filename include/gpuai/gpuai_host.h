@@ -49,6 +49,7 @@ typedef struct gpuai_multisearch_config {
   uint32_t mode;          /* 0 graph replay, 1 one plain launch, 2 one launch per search,
                              3 graph replays of `groups` groups overlapped with the host's work */
   uint32_t groups;        /* pipelined mode: groups of searches, one stream each (2) */
+  uint32_t autotune;      /* 1: launch shape from mi_autotune before the search, 0: default */
 } gpuai_multisearch_config;
 
 typedef struct gpuai_multisearch_result {
@@ -58,6 +59,7 @@ typedef struct gpuai_multisearch_result {
   double wall_seconds, device_ms, p50_ms, p99_ms, max_ms, playouts_per_second;
   uint64_t root_digest;
   double select_ms, playout_ms, update_ms;  /* wall time by phase, summed over iterations */
+  mi_tuning tuning;                          /* the launch shape the searches ran with */
 } gpuai_multisearch_result;
 
 /* Runs the multi-search on device 0; returns 0, or -1 with a message in `err` (err_len bytes). */
