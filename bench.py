@@ -19,7 +19,8 @@ apply, terminal test) priced at a frozen W = 192 int32 lane-ops; achieved = plie
 kernel time (HIP events on the library's stream), against the chip's int32 VALU issue rate
 CUs x 64 lanes x clock; peak_measured is that rate as a probe kernel of plain v_add_u32 /
 v_xor_b32 chains reaches it on this device (mi_valu_rate).  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
-recorded in profiles/ (null when absent).
+recorded in profiles/ (null when absent); roofline.hbm sets those bytes over the kernel time against
+the 8 TB/s HBM peak (the path is not memory bound).
 cpu_baseline: the reference's own host code (oracle/_ref/libref.so: State::genMoves twice per ply
 as src/playout.cpp:17-32 does, OpenMP over playouts, Philox draws) on a bounded sample of the same
 workload, on this host's cores.
@@ -333,6 +334,11 @@ def main() -> int:
     line["host_buffers"] = {"total_ms": round(hb.stats["total_ms"], 4),
                             "kernel_ms": round(hb.stats["kernel_ms"], 4),
                             "playouts_per_s": round(total / (hb.stats["total_ms"] / 1e3), 1)}
+    # the HBM roofline beside the VALU one: measured bytes per launch over the kernel time
+    if traffic:
+        line["roofline"]["hbm"] = {"achieved": round(traffic / avg_kernel_s / 1e9, 3),
+                                   "peak": 8000.0, "unit": "GB/s",
+                                   "frac": round(traffic / avg_kernel_s / 8e12, 6)}
     # the roofline's peak, measured: a probe kernel of independent int32 add/xor instructions
     valu_rate, valu_ms = eng.valu_rate()
     line["roofline"]["peak_measured"] = round(valu_rate / 1e12, 3)
