@@ -345,9 +345,9 @@ int mi_set_tuning(mi_ctx *ctx, const mi_tuning *tuning) {
   if (!ctx || !tuning) return MI_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->mu);
   mi_tuning t = *tuning;
-  if (!t.block_size) t.block_size = 512;
-  if (!t.blocks_per_cu) t.blocks_per_cu = 2;
-  if (!t.defer_min) t.defer_min = 8;
+  if (!t.block_size) t.block_size = kDefaultTuning.block_size;
+  if (!t.blocks_per_cu) t.blocks_per_cu = kDefaultTuning.blocks_per_cu;
+  if (!t.defer_min) t.defer_min = kDefaultTuning.defer_min;
   int rc = check_tuning(ctx, t);
   if (rc) return rc;
   ctx->tuning = t;

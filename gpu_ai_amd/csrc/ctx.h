@@ -16,11 +16,16 @@ struct DevBuf {
 };
 }  // namespace gpuai
 
+// Default launch shape: 64-thread blocks, 12 per CU (3 waves per SIMD), queue refill, defer_min
+// 16.  Among the fastest shapes for a 1M-playout batch, and a small batch spreads over many CUs
+// instead of sharing a few SIMDs.
+constexpr mi_tuning kDefaultTuning = {64u, 12u, MI_REFILL_QUEUE, 16u};
+
 struct mi_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-  mi_tuning tuning = {512u, 2u, MI_REFILL_QUEUE, 8u};
+  mi_tuning tuning = kDefaultTuning;
   uint32_t cus = 0, wave = 64, clock_khz = 0;
   gpuai::DevBuf pos, offs, winners, plies, hashes, work, tallies, moves, counts;
   // longest-first order of the uploaded batch (see upload_batch): positions permuted into it,

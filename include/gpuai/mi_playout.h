@@ -93,13 +93,16 @@ typedef struct mi_tune_result {
 
 typedef struct mi_ctx mi_ctx;
 
-/* Creates a context on HIP device `device`.  `tuning` may be NULL.  No counterpart in the
- * reference, whose drivers allocate on every call (src/singlePlayout.cu:76-84). */
+/* Creates a context on HIP device `device`.  `tuning` may be NULL (or have zero fields) for the
+ * default launch shape: 64-thread blocks, 12 per CU, queue refill, defer_min 16.  No
+ * counterpart in the reference, whose drivers allocate on every call
+ * (src/singlePlayout.cu:76-84). */
 int mi_create(mi_ctx **ctx, int device, const mi_tuning *tuning);
 int mi_destroy(mi_ctx *ctx);
 const char *mi_strerror(int code);
 /* Text of the last error raised on this context (empty if none). */
 const char *mi_last_error(const mi_ctx *ctx);
+/* Sets the launch shape; zero fields take the defaults of mi_create. */
 int mi_set_tuning(mi_ctx *ctx, const mi_tuning *tuning);
 int mi_get_tuning(const mi_ctx *ctx, mi_tuning *tuning);
 /* Device facts: compute units, wave size, clock in kHz. */

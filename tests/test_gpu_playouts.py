@@ -274,4 +274,4 @@ def test_autotune_picks_the_fastest_candidate(engine, golden):
     assert engine.tuning == best
     after = engine.run_playouts(pos, per_pos=256, seed=21, epoch=2).winners
     assert np.array_equal(before, after)
-    engine.set_tuning(512, 2, REFILL_QUEUE, 8)  # back to the defaults for the tests that follow
+    engine.set_tuning(64, 12, REFILL_QUEUE, 16)  # back to the defaults for the tests that follow
