@@ -665,7 +665,11 @@ struct mi_graph {
   size_t out_bytes = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipGraphNode_t n_h2d = nullptr, n_reset = nullptr, n_play = nullptr, n_d2h = nullptr;
+  hipGraphNode_t n_reset = nullptr, n_play = nullptr, n_d2h = nullptr;
+  // the playout node's arguments and launch shape, fixed at creation except for the grid, which
+  // each replay sizes to its batch (a small batch spread over every CU, as mi_run_playouts does)
+  gpuai::PlayoutArgs args;
+  uint32_t block = 0, resident = 0, grid = 0, args_refill = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // a replay launched by mi_graph_launch and not yet finished
   bool pending = false;
@@ -721,7 +725,7 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
   }
   std::memset(g->h_blob, 0, g->blob_bytes);
   gpuai::WorkWords *work = (gpuai::WorkWords *)g->d_out;
-  gpuai::PlayoutArgs a;
+  gpuai::PlayoutArgs &a = g->args;
   std::memset(&a, 0, sizeof(a));
   a.pos = (const mi_pos *)((char *)g->d_blob + g->pos_off);
   a.offs = (const uint32_t *)((char *)g->d_blob + g->offs_off);
@@ -729,7 +733,10 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
   a.work = work;
   a.defer_min = ctx->tuning.defer_min;
   a.desc = (const gpuai::LaunchDesc *)g->d_blob;
-  const uint32_t grid = grid_for(ctx, max_playouts);
+  g->block = ctx->tuning.block_size;
+  g->args_refill = ctx->tuning.refill;
+  g->resident = ctx->cus * ctx->tuning.blocks_per_cu;
+  g->grid = grid_for(ctx, max_playouts);
   void *reset_args[] = {&work};
   void *play_args[] = {&a};
   hipKernelNodeParams rp;
@@ -740,15 +747,14 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
   rp.kernelParams = reset_args;
   hipKernelNodeParams pp;
   std::memset(&pp, 0, sizeof(pp));
-  pp.func = const_cast<void *>(gpuai::playout_kernel(policy, ctx->tuning.refill));
-  pp.gridDim = dim3(grid);
-  pp.blockDim = dim3(ctx->tuning.block_size);
+  pp.func = const_cast<void *>(gpuai::playout_kernel(policy, g->args_refill));
+  pp.gridDim = dim3(g->grid);
+  pp.blockDim = dim3(g->block);
   pp.kernelParams = play_args;
   e = hipGraphCreate(&g->graph, 0);
-  if (e == hipSuccess)
-    e = hipGraphAddMemcpyNode1D(&g->n_h2d, g->graph, nullptr, 0, g->d_blob, g->h_blob,
-                                g->blob_bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipGraphAddKernelNode(&g->n_reset, g->graph, &g->n_h2d, 1, &rp);
+  // The batch's upload is not a node: an asynchronous copy on the stream before each replay
+  // moves a 64k-leaf batch (1.3 MB) in ~40 us, a memcpy node took ~320 us.
+  if (e == hipSuccess) e = hipGraphAddKernelNode(&g->n_reset, g->graph, nullptr, 0, &rp);
   if (e == hipSuccess) e = hipGraphAddKernelNode(&g->n_play, g->graph, &g->n_reset, 1, &pp);
   if (e == hipSuccess)
     e = hipGraphAddMemcpyNode1D(&g->n_d2h, g->graph, &g->n_play, 1, g->h_out, g->d_out,
@@ -788,11 +794,24 @@ int mi_graph_launch(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_
     std::memcpy((char *)g->h_blob + g->pos_off, pos, sizeof(mi_pos) * (size_t)npos);
     const size_t in_bytes = g->pos_off + sizeof(mi_pos) * (size_t)npos;
     const size_t out_bytes = sizeof(gpuai::WorkWords) + (size_t)total;
-    MI_HIP(ctx, hipGraphExecMemcpyNodeSetParams1D(g->exec, g->n_h2d, g->d_blob, g->h_blob, in_bytes,
-                                                  hipMemcpyHostToDevice));
     MI_HIP(ctx, hipGraphExecMemcpyNodeSetParams1D(g->exec, g->n_d2h, g->h_out, g->d_out, out_bytes,
                                                   hipMemcpyDeviceToHost));
+    // both refill policies are correct for any grid: static refill strides by the grid's threads
+    const uint64_t need = (total + g->block - 1u) / g->block;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(need, g->resident));
+    if (grid != g->grid) {
+      void *play_args[] = {&g->args};
+      hipKernelNodeParams pp;
+      std::memset(&pp, 0, sizeof(pp));
+      pp.func = const_cast<void *>(gpuai::playout_kernel(g->policy, g->args_refill));
+      pp.gridDim = dim3(grid);
+      pp.blockDim = dim3(g->block);
+      pp.kernelParams = play_args;
+      MI_HIP(ctx, hipGraphExecKernelNodeSetParams(g->exec, g->n_play, &pp));
+      g->grid = grid;
+    }
     MI_HIP(ctx, hipEventRecord(g->e0, ctx->stream));
+    MI_HIP(ctx, hipMemcpyAsync(g->d_blob, g->h_blob, in_bytes, hipMemcpyHostToDevice, ctx->stream));
     MI_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
     MI_HIP(ctx, hipEventRecord(g->e1, ctx->stream));
   }
@@ -822,9 +841,9 @@ int mi_graph_finish(mi_graph *g, int8_t *winners, mi_stats *stats) {
     stats->plies = w->plies;
     stats->kernel_ms = ms;
     stats->max_moves = w->max_moves;
-    stats->grid_blocks = grid_for(ctx, g->max_play);
-    stats->block_size = ctx->tuning.block_size;
-    stats->refill = ctx->tuning.refill;
+    stats->grid_blocks = g->grid;
+    stats->block_size = g->block;
+    stats->refill = g->args_refill;
   }
   return MI_OK;
 }

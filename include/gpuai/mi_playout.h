@@ -161,12 +161,13 @@ int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
                 uint32_t max_results, uint32_t *nresults, mi_tuning *best);
 
 /* Graph-replayed batches for search loops (no counterpart in the reference, which launches,
- * copies and frees per call).  mi_graph_create fixes the capacities and builds one hipGraph on
- * the context's stream: copy of the batch (positions, playout offsets, seed, epoch) from pinned
- * host memory, a reset kernel, the persistent playout kernel with the context's tuning, and the
- * copy of the winners back to pinned memory.  mi_graph_run fills the pinned buffers, resizes the
- * two copies to the batch, replays the graph and returns the winners as mi_run_playouts would
- * (same streams).  kernel_ms in the stats is the device time of the whole replay.  A graph is
+ * copies and frees per call).  mi_graph_create fixes the capacities and builds one hipGraph for
+ * the context's stream: a reset kernel, the persistent playout kernel with the context's tuning
+ * (its grid sized to each batch), and the copy of the winners back to pinned memory.
+ * mi_graph_run fills the pinned buffers, copies the batch (positions, playout offsets, seed,
+ * epoch) to the device with an asynchronous copy on the stream, replays the graph and returns
+ * the winners as mi_run_playouts would (same streams).  kernel_ms in the stats is the device
+ * time of the upload and the replay.  A graph is
  * used by one thread at a time and must be destroyed before its context. */
 typedef struct mi_graph mi_graph;
 int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, uint32_t policy,
