@@ -85,6 +85,61 @@ void finishResult(MultiSearchResult &res, const std::vector<std::unique_ptr<Game
   res.rootDigest = h;
 }
 
+// Packs the leaves of searches `ids` into one batch: each run of equal consecutive leaves of a
+// search becomes one position with a count (as DeviceMI355XPlayoutDriver::runPlayouts does), in
+// search order.  Searches are packed in parallel into slots given by a prefix sum of their run
+// counts.  Returns the batch's playouts; `first` receives each search's first playout index.
+uint64_t packLeaves(const std::vector<uint32_t> &ids, const std::vector<std::vector<State>> &leaves,
+                    std::vector<mi_pos> &pos, std::vector<uint32_t> &counts,
+                    std::vector<uint64_t> &first) {
+  const long m = (long)ids.size();
+  std::vector<size_t> runs(ids.size() + 1, 0);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (long k = 0; k < m; k++) {
+    const std::vector<State> &L = leaves[ids[k]];
+    size_t r = 0;
+    for (size_t i = 0; i < L.size(); i++) r += (i == 0 || !(L[i] == L[i - 1]));
+    runs[k + 1] = r;
+  }
+  first.assign(ids.size() + 1, 0);
+  for (long k = 0; k < m; k++) {
+    runs[k + 1] += runs[k];
+    first[k + 1] = first[k] + leaves[ids[k]].size();
+  }
+  pos.resize(runs[m]);
+  counts.resize(runs[m]);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (long k = 0; k < m; k++) {
+    const std::vector<State> &L = leaves[ids[k]];
+    size_t at = runs[k];
+    for (size_t i = 0; i < L.size(); i++) {
+      if (i > 0 && L[i] == L[i - 1]) {
+        counts[at - 1]++;
+      } else {
+        pos[at] = L[i].pack();
+        counts[at] = 1;
+        at++;
+      }
+    }
+  }
+  return first[m];
+}
+
+// Hands each search of `ids` its leaves' winners (leaf i of search k is playout first[k] + i).
+void scatterResults(const std::vector<uint32_t> &ids,
+                    const std::vector<std::vector<State>> &leaves,
+                    const std::vector<uint64_t> &first, const std::vector<int8_t> &winners,
+                    std::vector<std::vector<PlayerId>> &results) {
+  const long m = (long)ids.size();
+#pragma omp parallel for schedule(dynamic, 8)
+  for (long k = 0; k < m; k++) {
+    const uint32_t s = ids[k];
+    results[s].resize(leaves[s].size());
+    for (size_t i = 0; i < leaves[s].size(); i++)
+      results[s][i] = (PlayerId)winners[first[k] + i];
+  }
+}
+
 // Half of the searches with a device context and graph of its own (pipelined mode).
 struct SearchGroup {
   std::vector<uint32_t> ids;
@@ -93,6 +148,7 @@ struct SearchGroup {
   std::vector<mi_pos> pos;
   std::vector<uint32_t> counts;
   std::vector<int8_t> winners;
+  std::vector<uint64_t> first;  // each search's first playout in the batch
   bool launched = false;
   ~SearchGroup() {
     if (graph) mi_graph_destroy(graph);
@@ -154,19 +210,7 @@ MultiSearchResult runPipelined(const MultiSearchConfig &cfg,
     }
     res.selectMs += msSince(t0);
     const auto t1 = Clock::now();
-    G.pos.clear();
-    G.counts.clear();
-    for (uint32_t s : G.ids) {
-      const std::vector<State> &L = leaves[s];
-      for (size_t i = 0; i < L.size(); i++) {
-        if (i > 0 && L[i] == L[i - 1]) {
-          G.counts.back()++;
-        } else {
-          G.pos.push_back(L[i].pack());
-          G.counts.push_back(1);
-        }
-      }
-    }
+    packLeaves(G.ids, leaves, G.pos, G.counts, G.first);
     G.launched = !G.pos.empty();
     if (G.launched)
       checkMi(mi_graph_launch(G.graph, G.pos.data(), (uint32_t)G.pos.size(), G.counts.data(),
@@ -176,19 +220,14 @@ MultiSearchResult runPipelined(const MultiSearchConfig &cfg,
   };
   auto finishAndUpdate = [&](SearchGroup &G) {
     const auto t0 = Clock::now();
-    uint64_t total = 0;
-    for (uint32_t c : G.counts) total += c;
+    const uint64_t total = G.first.back();
     G.winners.resize(total);
     mi_stats st;
     checkMi(mi_graph_finish(G.graph, G.winners.data(), &st), G.driver->context(),
             "mi_graph_finish");
     G.launched = false;
     res.deviceMs += st.kernel_ms;
-    size_t at = 0;
-    for (uint32_t s : G.ids) {
-      results[s].resize(leaves[s].size());
-      for (size_t i = 0; i < leaves[s].size(); i++) results[s][i] = (PlayerId)G.winners[at++];
-    }
+    scatterResults(G.ids, leaves, G.first, G.winners, results);
     res.playoutMs += msSince(t0);
     const auto t1 = Clock::now();
     const long m = (long)G.ids.size();
@@ -264,6 +303,9 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
   std::vector<mi_pos> pos;
   std::vector<uint32_t> counts;
   std::vector<int8_t> winners;
+  std::vector<uint64_t> first;
+  std::vector<uint32_t> all(S);
+  for (uint32_t s = 0; s < S; s++) all[s] = s;
   const auto wall0 = Clock::now();
   for (;;) {
     const auto t0 = Clock::now();
@@ -292,20 +334,7 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
         iterPlayouts += leaves[s].size();
       }
     } else {
-      pos.clear();
-      counts.clear();
-      for (uint32_t s = 0; s < S; s++) {
-        const std::vector<State> &L = leaves[s];
-        for (size_t i = 0; i < L.size(); i++) {
-          if (i > 0 && L[i] == L[i - 1]) {
-            counts.back()++;
-          } else {
-            pos.push_back(L[i].pack());
-            counts.push_back(1);
-          }
-        }
-        iterPlayouts += L.size();
-      }
+      iterPlayouts = packLeaves(all, leaves, pos, counts, first);
       mi_stats st;
       if (cfg.mode == MultiSearchMode::GRAPH) {
         winners.resize(iterPlayouts);
@@ -316,11 +345,7 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
         driver.runGrouped(pos, counts, winners, &st);
       }
       res.deviceMs += st.kernel_ms;
-      size_t at = 0;
-      for (uint32_t s = 0; s < S; s++) {
-        results[s].resize(leaves[s].size());
-        for (size_t i = 0; i < leaves[s].size(); i++) results[s][i] = (PlayerId)winners[at++];
-      }
+      scatterResults(all, leaves, first, winners, results);
     }
     res.playoutMs += ms_since(t1);
     const auto t2 = Clock::now();
