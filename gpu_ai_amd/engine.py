@@ -339,7 +339,8 @@ class Engine:
 
 
 class GraphBatch:
-    """A hipGraph of (copy in, reset, persistent playout kernel, copy out) replayed per batch."""
+    """A hipGraph of (reset, persistent playout kernel, copy out) replayed per batch, after an
+    asynchronous copy of the batch; the playout node's grid is sized to each batch."""
 
     def __init__(self, engine: Engine, max_positions: int, max_playouts: int, policy: int):
         self.engine = engine

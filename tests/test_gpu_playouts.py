@@ -218,24 +218,35 @@ def test_device_start_positions_match_reference_golden(engine, golden):
     assert mid1024.tobytes() == golden["positions"]["mid1024"].tobytes()
 
 
+@pytest.mark.parametrize("refill", [REFILL_QUEUE, REFILL_STATIC])
 @pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
-def test_graph_replays_match_eager(engine, golden, policy):
-    """Replays one captured graph on batches of different sizes, with eager launches and
-    synchronisations in between, and checks every replay against mi_run_playouts."""
+def test_graph_replays_match_eager(engine, golden, policy, refill):
+    """Replays one graph on batches of different sizes (each replay sizes the playout grid to its
+    batch), with eager launches and synchronisations in between, and checks every replay against
+    mi_run_playouts under both refill policies."""
     pos = golden["positions"]["mid1024" if policy else "open1024"]
-    rng = np.random.default_rng(9 + policy)
-    with engine.graph(max_positions=1024, max_playouts=300000, policy=policy) as g:
-        for rep in range(6):
-            npos = int(rng.integers(1, 1024))
-            sel = pos[rng.choice(len(pos), npos, replace=False)]
-            counts = rng.integers(0, 400, npos).astype(np.uint32)
-            announce(f"graph replay {rep}: {npos} positions, {int(counts.sum())} playouts, policy {policy}")
-            got = g.run(sel, counts, seed=31 + rep, epoch=rep)
-            eager = engine.run_playouts(sel, counts=counts, seed=31 + rep, epoch=rep, policy=policy)
-            assert np.array_equal(got.winners, eager.winners), rep
-            assert got.stats["plies"] == eager.stats["plies"]
-        with pytest.raises(Exception):
-            g.run(pos[:2], np.array([200000, 200000], dtype=np.uint32))
+    rng = np.random.default_rng(9 + policy + 2 * refill)
+    engine.set_tuning(64, 12, refill, 16)
+    try:
+        with engine.graph(max_positions=1024, max_playouts=300000, policy=policy) as g:
+            for rep in range(6):
+                npos = int(rng.integers(1, 1024))
+                sel = pos[rng.choice(len(pos), npos, replace=False)]
+                # small and large batches alternate, so the grid shrinks and grows
+                counts = rng.integers(0, 8 if rep % 2 else 400, npos).astype(np.uint32)
+                announce(f"graph replay {rep}: {npos} positions, {int(counts.sum())} playouts, "
+                         f"policy {policy}, refill {refill}")
+                got = g.run(sel, counts, seed=31 + rep, epoch=rep)
+                eager = engine.run_playouts(sel, counts=counts, seed=31 + rep, epoch=rep,
+                                            policy=policy)
+                assert np.array_equal(got.winners, eager.winners), rep
+                assert got.stats["plies"] == eager.stats["plies"]
+                if counts.sum():
+                    assert got.stats["grid_blocks"] == eager.stats["grid_blocks"], rep
+            with pytest.raises(Exception):
+                g.run(pos[:2], np.array([200000, 200000], dtype=np.uint32))
+    finally:
+        engine.set_tuning(64, 12, REFILL_QUEUE, 16)
 
 
 def test_valu_rate_probe(engine):
