@@ -50,11 +50,28 @@ int ensure(mi_ctx *ctx, DevBuf &b, size_t bytes) {
   return MI_OK;
 }
 
+int ensure_host(mi_ctx *ctx, HostBuf &b, size_t bytes) {
+  if (bytes <= b.bytes && b.ptr) return MI_OK;
+  if (b.ptr) {
+    hipError_t e = hipHostFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipHostFree");
+  }
+  size_t want = std::max<size_t>(bytes, 4096);
+  want = (want + 4095) & ~(size_t)4095;
+  hipError_t e = hipHostMalloc(&b.ptr, want, hipHostMallocDefault);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipHostMalloc");
+  b.bytes = want;
+  return MI_OK;
+}
+
 }  // namespace ctxutil
 }  // namespace gpuai
 
 using gpuai::DevBuf;
 using gpuai::ctxutil::ensure;
+using gpuai::ctxutil::ensure_host;
 using gpuai::ctxutil::fail;
 using gpuai::ctxutil::hip_fail;
 
@@ -126,8 +143,11 @@ void plan_order(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
   }
 }
 
+// Uploads a batch to the context's device buffers.  With `pinned`, positions and offsets go
+// through the context's pinned staging (the stream is synchronised first, so that no earlier
+// copy still reads it); otherwise straight from the caller's memory.
 int upload_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
-                 uint64_t total) {
+                 uint64_t total, bool pinned = false) {
   int rc;
   plan_order(ctx, pos, npos, counts, total);
   if (ctx->ordered) {
@@ -148,14 +168,22 @@ int upload_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *
     return rc;
   if ((rc = ensure(ctx, ctx->winners, (size_t)total))) return rc;
   if ((rc = ensure(ctx, ctx->work, sizeof(gpuai::WorkWords)))) return rc;
-  MI_HIP(ctx, hipMemcpyAsync(ctx->pos.ptr, pos, sizeof(mi_pos) * npos, hipMemcpyHostToDevice,
-                             ctx->stream));
-  if (counts) {
-    if ((rc = ensure(ctx, ctx->offs, sizeof(uint32_t) * ((size_t)npos + 1)))) return rc;
-    MI_HIP(ctx, hipMemcpyAsync(ctx->offs.ptr, ctx->host_offs.data(),
-                               sizeof(uint32_t) * ((size_t)npos + 1), hipMemcpyHostToDevice,
-                               ctx->stream));
+  if (counts && (rc = ensure(ctx, ctx->offs, sizeof(uint32_t) * ((size_t)npos + 1)))) return rc;
+  const size_t pos_bytes = sizeof(mi_pos) * npos;
+  const size_t offs_bytes = counts ? sizeof(uint32_t) * ((size_t)npos + 1) : 0;
+  const void *src_pos = pos, *src_offs = ctx->host_offs.data();
+  if (pinned) {
+    MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = ensure_host(ctx, ctx->h_in, pos_bytes + offs_bytes))) return rc;
+    std::memcpy(ctx->h_in.ptr, pos, pos_bytes);
+    if (counts) std::memcpy((char *)ctx->h_in.ptr + pos_bytes, ctx->host_offs.data(), offs_bytes);
+    src_pos = ctx->h_in.ptr;
+    src_offs = (const char *)ctx->h_in.ptr + pos_bytes;
   }
+  MI_HIP(ctx, hipMemcpyAsync(ctx->pos.ptr, src_pos, pos_bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (counts)
+    MI_HIP(ctx, hipMemcpyAsync(ctx->offs.ptr, src_offs, offs_bytes, hipMemcpyHostToDevice,
+                               ctx->stream));
   return MI_OK;
 }
 
@@ -166,9 +194,15 @@ uint32_t grid_for(const mi_ctx *ctx, uint64_t total) {
   return (uint32_t)std::max<uint64_t>(1, std::min(need, resident));
 }
 
-// Launches the playout kernel on the staged device buffers and reads back the statistics.
+// Launches the playout kernel on the staged device buffers and reads back the statistics and,
+// when `winners` is set, the winners (both through the context's pinned read-back buffer, with
+// one synchronisation).
 int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, bool with_offs,
-               uint32_t seed, uint32_t epoch, uint32_t policy, bool trace, mi_stats *stats) {
+               uint32_t seed, uint32_t epoch, uint32_t policy, bool trace, mi_stats *stats,
+               int8_t *winners = nullptr) {
+  int rc;
+  if ((rc = ensure_host(ctx, ctx->h_back, sizeof(gpuai::WorkWords) + (winners ? total : 0))))
+    return rc;
   gpuai::PlayoutArgs a;
   a.pos = (const mi_pos *)ctx->pos.ptr;
   a.offs = with_offs ? (const uint32_t *)ctx->offs.ptr : nullptr;
@@ -192,9 +226,14 @@ int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
   MI_HIP(ctx, gpuai::launch_playouts(a, policy, trace, ctx->tuning.refill, grid,
                                      ctx->tuning.block_size, ctx->stream));
   MI_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
-  gpuai::WorkWords w;
-  MI_HIP(ctx, hipMemcpyAsync(&w, a.work, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
+  gpuai::WorkWords *hw = (gpuai::WorkWords *)ctx->h_back.ptr;
+  int8_t *hwin = (int8_t *)ctx->h_back.ptr + sizeof(gpuai::WorkWords);
+  MI_HIP(ctx, hipMemcpyAsync(hw, a.work, sizeof(*hw), hipMemcpyDeviceToHost, ctx->stream));
+  if (winners)
+    MI_HIP(ctx, hipMemcpyAsync(hwin, a.winners, total, hipMemcpyDeviceToHost, ctx->stream));
   MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const gpuai::WorkWords w = *hw;
+  if (winners) std::memcpy(winners, hwin, total);
   float ms = 0.f;
   MI_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
   if (stats) {
@@ -242,18 +281,15 @@ int run_common(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *co
   }
   if (!winners && !tallies) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "winners pointer is null");
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
-  if ((rc = upload_batch(ctx, pos, npos, counts, total))) return rc;
+  if ((rc = upload_batch(ctx, pos, npos, counts, total, true))) return rc;
   if (trace) {
     if ((rc = ensure(ctx, ctx->plies, sizeof(uint32_t) * total))) return rc;
     if ((rc = ensure(ctx, ctx->hashes, sizeof(uint32_t) * total))) return rc;
   }
   ctx->staged = false;  // the batch buffers now hold this call's data
   if ((rc = run_kernel(ctx, npos, per_pos, total, counts != nullptr, seed, epoch, policy, trace,
-                       stats)))
+                       stats, winners)))
     return rc;
-  if (winners)
-    MI_HIP(ctx, hipMemcpyAsync(winners, ctx->winners.ptr, total, hipMemcpyDeviceToHost,
-                               ctx->stream));
   if (trace && plies)
     MI_HIP(ctx, hipMemcpyAsync(plies, ctx->plies.ptr, sizeof(uint32_t) * total,
                                hipMemcpyDeviceToHost, ctx->stream));
@@ -334,6 +370,8 @@ int mi_destroy(mi_ctx *ctx) {
                     &ctx->order, &ctx->ord_offs};
   for (DevBuf *b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
+  if (ctx->h_in.ptr) (void)hipHostFree(ctx->h_in.ptr);
+  if (ctx->h_back.ptr) (void)hipHostFree(ctx->h_back.ptr);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

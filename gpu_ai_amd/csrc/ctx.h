@@ -14,6 +14,12 @@ struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
 };
+// Pinned host memory (hipHostMalloc): copies from and to it run asynchronously on the copy
+// engines, where copies of pageable memory are staged through the runtime's own buffers.
+struct HostBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
 }  // namespace gpuai
 
 // Default launch shape: 64-thread blocks, 12 per CU (3 waves per SIMD), queue refill, defer_min
@@ -34,6 +40,9 @@ struct mi_ctx {
   bool ordered = false;
   std::vector<uint32_t> host_order, host_ord_offs;
   std::vector<mi_pos> host_ord_pos;
+  // pinned staging of mi_run_playouts: positions | offsets on the way in, work words | winners
+  // on the way back.  Rewritten only while the stream is idle.
+  gpuai::HostBuf h_in, h_back;
   // staged batch
   uint32_t staged_npos = 0, staged_per_pos = 0, staged_total = 0;
   bool staged = false, staged_offs = false;
@@ -50,6 +59,8 @@ int fail(mi_ctx *ctx, int code, const std::string &msg);
 int hip_fail(mi_ctx *ctx, hipError_t e, const char *what);
 // Grows a device buffer to at least `bytes`.
 int ensure(mi_ctx *ctx, DevBuf &b, size_t bytes);
+// Grows a pinned host buffer to at least `bytes` (the caller's stream must be idle).
+int ensure_host(mi_ctx *ctx, HostBuf &b, size_t bytes);
 }  // namespace ctxutil
 }  // namespace gpuai
 
