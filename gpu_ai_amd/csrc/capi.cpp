@@ -707,7 +707,7 @@ struct mi_graph {
   // the playout node's arguments and launch shape, fixed at creation except for the grid, which
   // each replay sizes to its batch (a small batch spread over every CU, as mi_run_playouts does)
   gpuai::PlayoutArgs args;
-  uint32_t block = 0, resident = 0, grid = 0, args_refill = 0;
+  uint32_t block = 0, resident = 0, grid = 0, refill = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // a replay launched by mi_graph_launch and not yet finished
   bool pending = false;
@@ -772,7 +772,7 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
   a.defer_min = ctx->tuning.defer_min;
   a.desc = (const gpuai::LaunchDesc *)g->d_blob;
   g->block = ctx->tuning.block_size;
-  g->args_refill = ctx->tuning.refill;
+  g->refill = ctx->tuning.refill;
   g->resident = ctx->cus * ctx->tuning.blocks_per_cu;
   g->grid = grid_for(ctx, max_playouts);
   void *reset_args[] = {&work};
@@ -785,7 +785,7 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
   rp.kernelParams = reset_args;
   hipKernelNodeParams pp;
   std::memset(&pp, 0, sizeof(pp));
-  pp.func = const_cast<void *>(gpuai::playout_kernel(policy, g->args_refill));
+  pp.func = const_cast<void *>(gpuai::playout_kernel(policy, g->refill));
   pp.gridDim = dim3(g->grid);
   pp.blockDim = dim3(g->block);
   pp.kernelParams = play_args;
@@ -841,7 +841,7 @@ int mi_graph_launch(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_
       void *play_args[] = {&g->args};
       hipKernelNodeParams pp;
       std::memset(&pp, 0, sizeof(pp));
-      pp.func = const_cast<void *>(gpuai::playout_kernel(g->policy, g->args_refill));
+      pp.func = const_cast<void *>(gpuai::playout_kernel(g->policy, g->refill));
       pp.gridDim = dim3(grid);
       pp.blockDim = dim3(g->block);
       pp.kernelParams = play_args;
@@ -881,7 +881,7 @@ int mi_graph_finish(mi_graph *g, int8_t *winners, mi_stats *stats) {
     stats->max_moves = w->max_moves;
     stats->grid_blocks = g->grid;
     stats->block_size = g->block;
-    stats->refill = g->args_refill;
+    stats->refill = g->refill;
   }
   return MI_OK;
 }
