@@ -633,11 +633,10 @@ struct PlyResult {
 };
 
 // Plays one ply of the random policy with word r (src/playout.cpp:24-27,
-// src/player.cpp:13-16).  With TRACE, also returns the list digest.
+// src/player.cpp:13-16), given the position's analysis g.  With TRACE, also returns the list
+// digest.
 template <bool TRACE>
-CK_HD PlyResult ply_random(Lane &L, uint32_t r, uint32_t *list_dig) {
-  Gen g;
-  analyse(L, g);
+CK_HD PlyResult ply_random_given(Lane &L, const Gen &g, uint32_t r, uint32_t *list_dig) {
   PlyResult res;
   uint32_t packed = 0;
   LeafCache lc = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
@@ -667,12 +666,19 @@ CK_HD PlyResult ply_random(Lane &L, uint32_t r, uint32_t *list_dig) {
   return res;
 }
 
+// Plays one ply of the random policy with word r.
+template <bool TRACE>
+CK_HD PlyResult ply_random(Lane &L, uint32_t r, uint32_t *list_dig) {
+  Gen g;
+  analyse(L, g);
+  return ply_random_given<TRACE>(L, g, r, list_dig);
+}
+
 // The common part of a random ply: plays it unless it is a capture ply with a multi-jump origin,
 // in which case the position is left untouched and `deferred` is returned set (the caller plays
-// it later with ply_random and the same word).
+// it later with ply_random_given, the same word and the analysis left in g).
 template <bool TRACE>
-CK_HD PlyResult ply_random_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool &deferred) {
-  Gen g;
+CK_HD PlyResult ply_random_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool &deferred, Gen &g) {
   analyse(L, g);
   PlyResult res;
   deferred = g.multi != 0u;
@@ -867,12 +873,18 @@ CK_HD PlyResult ply_heuristic(Lane &L, uint32_t r, uint32_t *list_dig) {
   return heuristic_body<TRACE, true>(L, g, r, list_dig);
 }
 
+// The heuristic ply given the position's analysis g.
+template <bool TRACE>
+CK_HD PlyResult ply_heuristic_given(Lane &L, const Gen &g, uint32_t r, uint32_t *list_dig) {
+  return heuristic_body<TRACE, true>(L, g, r, list_dig);
+}
+
 // The common part of a heuristic ply: plays it unless it is a capture ply with a sequence
 // longer than one jump, which is left untouched with `deferred` set (the caller plays it later
-// with ply_heuristic and the same word).
+// with ply_heuristic_given, the same word and the analysis left in g).
 template <bool TRACE>
-CK_HD PlyResult ply_heuristic_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool &deferred) {
-  Gen g;
+CK_HD PlyResult ply_heuristic_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool &deferred,
+                                   Gen &g) {
   analyse(L, g);
   deferred = g.capture && g.multi;
   if (deferred) {

@@ -102,6 +102,7 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
   Words w = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, false, false};
   uint64_t plies_acc = 0;
   ck::Lane L = {0u, 0u, 0u, 0u, 0u};
+  ck::Gen gd;  // the analysis of a deferred ply, kept for its walk (saves a second analyse)
   // wave-uniform refill and deferral state
   uint32_t res_next = 0, res_end = 0, since_walk = 0;
   bool exhausted = false;
@@ -187,8 +188,8 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
       // a deferred ply takes the same word again in the walk phase (take_word is idempotent
       // for a given ply)
       bool d = false;
-      if (POLICY == MI_POLICY_RANDOM) res = ck::ply_random_fast<TRACE>(L, r, &ld, d);
-      else res = ck::ply_heuristic_fast<TRACE>(L, r, &ld, d);
+      if (POLICY == MI_POLICY_RANDOM) res = ck::ply_random_fast<TRACE>(L, r, &ld, d, gd);
+      else res = ck::ply_heuristic_fast<TRACE>(L, r, &ld, d, gd);
       deferred = d;
       play = !d;
     }
@@ -201,8 +202,8 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
           since_walk = 0;
           if (deferred) {
             const uint32_t r = take_word(w, ply);
-            if (POLICY == MI_POLICY_RANDOM) res = ck::ply_random<TRACE>(L, r, &ld);
-            else res = ck::ply_heuristic<TRACE>(L, r, &ld);
+            if (POLICY == MI_POLICY_RANDOM) res = ck::ply_random_given<TRACE>(L, gd, r, &ld);
+            else res = ck::ply_heuristic_given<TRACE>(L, gd, r, &ld);
             deferred = false;
             play = true;
           }

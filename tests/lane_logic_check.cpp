@@ -3,7 +3,8 @@
 //
 // Each playout runs the sequence k_playouts runs for one lane: the ply's Philox word, the fast
 // ply (ply_random_fast / ply_heuristic_fast), and, when that defers a multi-jump capture, the
-// full ply (ply_random / ply_heuristic) with the same word.  Winners, ply counts, the largest
+// full ply (ply_random_given / ply_heuristic_given) with the same word and the fast ply's
+// analysis of the position.  Winners, ply counts, the largest
 // list length and the per-ply hash chain over every move list and chosen index must equal the
 // oracle's traces (oracle/oracle.c, which follows src/state.cu and src/playout.cpp:17-32).
 // Move lists of positions met along the way are compared entry by entry with or_gen_moves.
@@ -30,11 +31,14 @@ int play(const or_pos &p, uint32_t seed, uint32_t pi, uint32_t j, uint32_t epoch
     const uint32_t r = w[ply & 3u];
     uint32_t ld = 0;
     bool d = false;
-    PlyResult res = policy == OR_POLICY_RANDOM ? ply_random_fast<true>(L, r, &ld, d)
-                                               : ply_heuristic_fast<true>(L, r, &ld, d);
+    Gen g;
+    PlyResult res = policy == OR_POLICY_RANDOM ? ply_random_fast<true>(L, r, &ld, d, g)
+                                               : ply_heuristic_fast<true>(L, r, &ld, d, g);
     if (d) {
       deferred++;
-      res = policy == OR_POLICY_RANDOM ? ply_random<true>(L, r, &ld) : ply_heuristic<true>(L, r, &ld);
+      // the kernel's walk phase: the full ply with the same word and the fast ply's analysis
+      res = policy == OR_POLICY_RANDOM ? ply_random_given<true>(L, g, r, &ld)
+                                       : ply_heuristic_given<true>(L, g, r, &ld);
     }
     h = hash_step(h, ply_record(ld, res.n, res.chosen));
     if (res.n > maxn) maxn = res.n;
