@@ -316,7 +316,7 @@ def main() -> int:
                    "block_size": tuning["block_size"], "blocks_per_cu": tuning["blocks_per_cu"],
                    "refill": "queue" if tuning["refill"] == 1 else "static",
                    "defer_min": tuning["defer_min"], "autotuned": tuned is not None,
-                   "longest_first": total >= 2 * info["compute_units"] * 1024},
+                   "longest_first": total >= 256 * info["compute_units"]},
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 4),
                      "peak": round(peak_spec / 1e12, 3), "unit": "Tops/s (int32 lane-ops)",
                      "frac": round(achieved / peak_spec, 4), "traffic": traffic,

@@ -161,7 +161,7 @@ def test_uneven_counts_empty_and_single(engine, oracle, golden):
 
 @pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
 def test_longest_first_order_uneven_counts(engine, oracle, golden, policy):
-    """Batches large enough for the longest-first hand-out (at least 2 x 1024 playouts per CU):
+    """Batches large enough for the longest-first hand-out (at least 256 playouts per CU):
     uneven counts, zeros and repeated positions; every result slot and trace bit-exact."""
     pos = np.concatenate([golden["positions"]["open1024"], golden["positions"]["mid1024"]])
     rng = np.random.default_rng(17 + policy)
