@@ -144,9 +144,14 @@ void plan_order(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
   }
 }
 
-// Uploads a batch to the context's device buffers.  With `pinned`, positions and offsets go
-// through the context's pinned staging (the stream is synchronised first, so that no earlier
-// copy still reads it); otherwise straight from the caller's memory.
+// Pinned staging is used up to this many bytes per direction; larger batches copy straight from
+// and to the caller's memory (their copies take far longer than the staging saves).
+constexpr size_t kPinnedMax = size_t(64) << 20;
+
+// Uploads a batch to the context's device buffers.  With `pinned` (and at most kPinnedMax
+// bytes), positions and offsets go through the context's pinned staging (the stream is
+// synchronised first, so that no earlier copy still reads it); otherwise straight from the
+// caller's memory.
 int upload_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
                  uint64_t total, bool pinned = false) {
   int rc;
@@ -173,7 +178,7 @@ int upload_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *
   const size_t pos_bytes = sizeof(mi_pos) * npos;
   const size_t offs_bytes = counts ? sizeof(uint32_t) * ((size_t)npos + 1) : 0;
   const void *src_pos = pos, *src_offs = ctx->host_offs.data();
-  if (pinned) {
+  if (pinned && pos_bytes + offs_bytes <= kPinnedMax) {
     MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if ((rc = ensure_host(ctx, ctx->h_in, pos_bytes + offs_bytes))) return rc;
     std::memcpy(ctx->h_in.ptr, pos, pos_bytes);
@@ -288,9 +293,13 @@ int run_common(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *co
     if ((rc = ensure(ctx, ctx->hashes, sizeof(uint32_t) * total))) return rc;
   }
   ctx->staged = false;  // the batch buffers now hold this call's data
+  const bool pinned_back = total <= kPinnedMax;
   if ((rc = run_kernel(ctx, npos, per_pos, total, counts != nullptr, seed, epoch, policy, trace,
-                       stats, winners)))
+                       stats, pinned_back ? winners : nullptr)))
     return rc;
+  if (winners && !pinned_back)
+    MI_HIP(ctx, hipMemcpyAsync(winners, ctx->winners.ptr, total, hipMemcpyDeviceToHost,
+                               ctx->stream));
   if (trace && plies)
     MI_HIP(ctx, hipMemcpyAsync(plies, ctx->plies.ptr, sizeof(uint32_t) * total,
                                hipMemcpyDeviceToHost, ctx->stream));

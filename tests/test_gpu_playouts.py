@@ -175,6 +175,20 @@ def test_longest_first_order_uneven_counts(engine, oracle, golden, policy):
     assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
 
 
+def test_batch_beyond_pinned_staging(engine, golden):
+    """A batch whose winners exceed the 64 MiB of pinned staging (copied straight to the
+    caller's memory) returns what the same positions give in a small, staged call."""
+    pos = golden["positions"]["open1024"]
+    per = 65600  # 1024 x 65,600 = 67.2M playouts > 64 MiB of winners
+    announce(f"mi_run_playouts beyond pinned staging: {len(pos) * per} playouts")
+    big = engine.run_playouts(pos, per_pos=per, seed=8, epoch=1)
+    assert big.winners.size == len(pos) * per
+    assert set(np.unique(big.winners)) <= {-1, 0, 1}
+    # a position's stream is keyed by its index in the batch: compare the batch's first three
+    first = engine.run_playouts(pos[:3], per_pos=per, seed=8, epoch=1)
+    assert np.array_equal(big.winners[:3 * per], first.winners)
+
+
 def test_terminal_and_odd_positions(engine, oracle, golden):
     ml = golden["movelists"]["positions"]
     # random placements and king-loop positions sit at the end of the golden set; they include
