@@ -250,7 +250,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the playout path runs on a single MI355X here "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "(sharding over several GPUs is designed, not run)", file=sys.stderr)
         return 2
 
     if args.workload == "cpu":

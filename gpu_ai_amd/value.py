@@ -8,7 +8,7 @@ Its error is measured against independent 1024-playout estimates of the same pos
 the error of k-playout estimates (k = 1 is what the reference's MCTS takes per leaf,
 src/mcts.cpp:57-61), which prices one network evaluation in playouts.
 
-    python -m gpu_ai_amd.value --positions 2000000 --epochs 24 --out gpurun_out/value.ckpt
+    python -m gpu_ai_amd.value --positions 2000000 --epochs 24 --out value.ckpt
 """
 from __future__ import annotations
 

@@ -3,7 +3,7 @@
  * position seen by the side to move, trained and evaluated by hand-written gfx950 kernels
  * (BF16 operands on MFMA v_mfma_f32_16x16x32_bf16, FP32 accumulation, FP32 master weights and
  * Adam moments).  No counterpart in the reference, whose leaves are only ever played out
- * (src/mcts.cpp:57-61); plain playouts (mi_playout.h) stay the fallback and the judge.
+ * (src/mcts.cpp:57-61); plain playouts (mi_playout.h) stay the fallback and the yardstick.
  *
  * Input encoding (128 values, 0 or 1): four planes of 32 squares in the frame of the side to
  * move (its men, its kings, the opponent's men, the opponent's kings); for player 2 the board is

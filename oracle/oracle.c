@@ -406,7 +406,7 @@ void or_score_move(const or_pos *p, const or_move *m, int32_t score[2]) {
   score_move(&s, &bm, score);
 }
 
-/* ---- move-list hashing (verification mode, defined by this build) ---- */
+/* ---- move-list hashing (verification mode, defined by this project) ---- */
 
 uint32_t or_fmix32(uint32_t x) {   /* MurmurHash3 finaliser */
   x ^= x >> 16;

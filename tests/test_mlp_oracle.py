@@ -1,4 +1,4 @@
-"""Pins the value network's CPU restatement (oracle/mlp_ref.py) before it judges the device.
+"""Pins the value network's CPU restatement (oracle/mlp_ref.py) before it checks the device.
 
 Each oracle function is checked against something that fixes it independently of its own
 formulas: torch's bfloat16 conversion (rounding), torch autograd on the same network in float64
