@@ -54,7 +54,7 @@ class DeviceMI355XPlayoutDriver : public PlayoutDriver {
   std::atomic<uint32_t> epoch_{0};  // a player's worker and getMove may both launch
 };
 
-// Name -> driver.  The reference's driver names (src/playout.cpp:358-392) all select the MI355X
+// Name -> driver.  The reference's driver names (src/playout.cpp:189-223) all select the MI355X
 // engine: host, device_single, device_multiple, device_coarse, hybrid and optimal run random
 // playouts; host_heuristic, device_heuristic, hybrid_heuristic and optimal_heuristic run
 // heuristic playouts.  device_mi355x and device_mi355x_heuristic name them directly.

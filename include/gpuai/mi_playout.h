@@ -43,7 +43,7 @@ extern "C" {
 
 /* ---- refill policies of the persistent playout kernel ---- */
 #define MI_REFILL_STATIC 0     /* lane-strided assignment, no atomics */
-#define MI_REFILL_QUEUE 1      /* wave-level claims of 64 playouts from a device counter */
+#define MI_REFILL_QUEUE 1      /* wave-level claims of 32 playouts from a device counter */
 
 /* Packed position (16 bytes).  Replaces the 776-byte State (src/state.hpp:119-122). */
 typedef struct mi_pos {
