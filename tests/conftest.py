@@ -45,7 +45,18 @@ def golden():
     }
     with open(os.path.join(GOLDEN, "perft.json")) as f:
         g["perft"] = json.load(f)
+    full = load("full_configs.npz")
+    g["full"] = {name: {"winners": unpack2(full[name + "_winners2"]),
+                        "plies": full[name + "_plies"]}
+                 for name in ("random_1024x1024", "heuristic_1024x1024")}
     return g
+
+
+def unpack2(packed: np.ndarray) -> np.ndarray:
+    """Inverse of make_golden.pack2: four two-bit codes (winner + 1) per byte, low bits first."""
+    p = packed.astype(np.uint8)
+    codes = np.stack([(p >> s) & 3 for s in (0, 2, 4, 6)], axis=1).reshape(-1)
+    return codes.astype(np.int8) - 1
 
 
 @pytest.fixture(scope="session")

@@ -12,7 +12,13 @@ Every vector below is produced by the reference build (oracle/_ref/libref_assert
   perft.json      move-tree leaf counts from the start position and from midgame positions
   playouts.npz    per-playout winner / ply count / move-list hash chain, random and heuristic
   scores.npz      scoreState / scoreMove / getWeight (src/heuristic.cu:20-49) samples
+  full_configs.npz  every winner of the two full benchmark batches, 1024 positions x 1024
+                  playouts (random on the k in [0, 30] openings, heuristic on the k in [20, 60]
+                  midgame positions; seed 12345, epoch 0), two bits per winner (winner + 1),
+                  and each position's summed ply count
   MANIFEST.json   SHA-256 of each file
+
+    python tests/golden/make_golden.py --full-only   # rewrites full_configs.npz only
 """
 from __future__ import annotations
 
@@ -33,8 +39,43 @@ SEED_POS = 2016
 SEED_PLAY = 12345
 
 
+def pack2(w: np.ndarray) -> np.ndarray:
+    """Winners (-1, 0, 1) as two-bit codes winner + 1, four per byte, first in the low bits."""
+    c = (w.astype(np.int16) + 1).astype(np.uint8).reshape(-1, 4)
+    return (c[:, 0] | (c[:, 1] << 2) | (c[:, 2] << 4) | (c[:, 3] << 6)).astype(np.uint8)
+
+
+def full_configs(ref, open1024: np.ndarray, mid1024: np.ndarray) -> None:
+    """The reference build's winners and per-position ply sums of the full benchmark batches."""
+    out = {}
+    for name, pos, policy in (("random_1024x1024", open1024, pyoracle.POLICY_RANDOM),
+                              ("heuristic_1024x1024", mid1024, pyoracle.POLICY_HEURISTIC)):
+        w, tr = ref.run_playouts(pos, per_pos=1024, seed=SEED_PLAY, epoch=0, policy=policy,
+                                 threads=os.cpu_count() or 1)
+        out[name + "_winners2"] = pack2(w)
+        out[name + "_plies"] = tr["plies"].reshape(len(pos), 1024).sum(axis=1, dtype=np.uint64)
+        print(f"{name}: {len(w)} playouts, {int(out[name + '_plies'].sum())} plies")
+    np.savez_compressed(os.path.join(HERE, "full_configs.npz"), **out)
+
+
+def write_manifest() -> None:
+    manifest = {}
+    for name in ("positions.npz", "movelists.npz", "perft.json", "playouts.npz", "scores.npz",
+                 "full_configs.npz"):
+        with open(os.path.join(HERE, name), "rb") as f:
+            manifest[name] = hashlib.sha256(f.read()).hexdigest()
+    manifest["source"] = "oracle/_ref/libref_assert.so (reference host code, asserts enabled)"
+    with open(os.path.join(HERE, "MANIFEST.json"), "w") as f:
+        json.dump(manifest, f, indent=1)
+
+
 def main() -> None:
     ref = pyoracle.Reference(pyoracle.REF_ASSERT_SO)
+    if "--full-only" in sys.argv:
+        with np.load(os.path.join(HERE, "positions.npz")) as z:
+            full_configs(ref, z["open1024"], z["mid1024"])
+        write_manifest()
+        return
 
     # ---- start positions
     open1024 = ref.start_positions(SEED_POS, 1024, 0, 30, False)
@@ -142,13 +183,8 @@ def main() -> None:
         sc[i] = r
     np.savez_compressed(os.path.join(HERE, "scores.npz"), scores=sc)
 
-    manifest = {}
-    for name in ("positions.npz", "movelists.npz", "perft.json", "playouts.npz", "scores.npz"):
-        with open(os.path.join(HERE, name), "rb") as f:
-            manifest[name] = hashlib.sha256(f.read()).hexdigest()
-    manifest["source"] = "oracle/_ref/libref_assert.so (reference host code, asserts enabled)"
-    with open(os.path.join(HERE, "MANIFEST.json"), "w") as f:
-        json.dump(manifest, f, indent=1)
+    full_configs(ref, open1024, mid1024)
+    write_manifest()
     print(f"positions {len(positions)}, moves {len(moves)}, max n {counts.max()}, "
           f"max jumps {moves['jumps'].max()}, king loops {len(loops)} ({tries} tries), "
           f"scores {len(sc)}")

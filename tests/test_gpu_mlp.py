@@ -37,9 +37,10 @@ def _params(seed=5):
 @pytest.fixture(scope="module")
 def positions(oracle):
     # openings to endgames (kings appear late), non-terminal
-    a = oracle.start_positions(21, 1500, 0, 40, reject_terminal=True)
-    b = oracle.start_positions(22, 1500, 40, 120, reject_terminal=True)
-    return np.concatenate([a, b])
+    a = oracle.start_positions(21, 6200, 0, 40, reject_terminal=True)
+    b = oracle.start_positions(22, 6200, 40, 120, reject_terminal=True)
+    rng = np.random.default_rng(23)
+    return np.concatenate([a, b])[rng.permutation(12400)]  # every batch mixes both
 
 
 def _net(engine, max_batch=1024, seed=1):
@@ -61,11 +62,11 @@ def test_mlp_params_round_trip(engine):
         np.testing.assert_array_equal(net.get_params(), w)
 
 
-@pytest.mark.parametrize("n", [1, 64, 1000, 3000])
+@pytest.mark.parametrize("n", [1, 64, 1000, 3000, 4096])
 def test_mlp_forward_matches_oracle(engine, positions, n):
     w = _params(5)
     pos = positions[:n]
-    with _net(engine, 3008) as net:
+    with _net(engine, 4096) as net:
         net.set_params(w)
         y = net.forward(pos)
     ref = M.forward(w, M.encode(pos))["y"]
@@ -90,12 +91,12 @@ def test_mlp_forward_colour_symmetry(engine, positions):
         np.testing.assert_array_equal(net.forward(pos), net.forward(M.mirror(pos)))
 
 
-@pytest.mark.parametrize("n", [200, 512, 2048])
+@pytest.mark.parametrize("n", [200, 512, 2048, 4096])
 def test_mlp_gradients_match_oracle(engine, positions, n):
     w = _params(6)
     pos = positions[:n]
     t = np.random.default_rng(n).uniform(-1, 1, n).astype(np.float32)
-    with _net(engine, 2048) as net:
+    with _net(engine, 4096) as net:
         net.set_params(w)
         g, loss = net.gradients(pos, t)
         np.testing.assert_array_equal(net.get_params(), w)  # nothing updated
@@ -109,15 +110,16 @@ def test_mlp_gradients_match_oracle(engine, positions, n):
         assert err <= 2e-3 and cos >= 0.99999, name
 
 
-def test_mlp_adam_steps_match_oracle(engine, positions):
+@pytest.mark.parametrize("batch", [512, 4096])
+def test_mlp_adam_steps_match_oracle(engine, positions, batch):
     w = _params(7)
     opt = M.Adam(M.N_PARAMS, lr=1e-3)
-    with _net(engine, 512) as net:
+    with _net(engine, batch) as net:
         net.set_params(w)
         ref_w = w.copy()
         for k in range(3):
-            pos = positions[k * 512:(k + 1) * 512]
-            t = np.random.default_rng(k).uniform(-1, 1, 512).astype(np.float32)
+            pos = positions[k * batch:(k + 1) * batch]
+            t = np.random.default_rng(k).uniform(-1, 1, batch).astype(np.float32)
             g, _ = net.gradients(pos, t)
             net.train_step(pos, t, 1e-3)
             ref_w = opt.step(ref_w, g)
@@ -132,16 +134,17 @@ def test_mlp_adam_steps_match_oracle(engine, positions):
             ref_w = got  # continue from the device's parameters
 
 
-def test_mlp_epoch_equals_steps(engine, oracle, positions):
-    pos = positions[:1100]
+@pytest.mark.parametrize("batch,n", [(256, 1100), (4096, 12400)])
+def test_mlp_epoch_equals_steps(engine, oracle, positions, batch, n):
+    pos = positions[:n]
     t = np.random.default_rng(11).uniform(-1, 1, len(pos)).astype(np.float32)
-    with _net(engine, 256, seed=4) as a, _net(engine, 256, seed=4) as b:
+    with _net(engine, batch, seed=4) as a, _net(engine, batch, seed=4) as b:
         a.set_dataset(pos, t)
-        mean_loss, ms = a.train_epoch(256, 1e-3, shuffle_seed=5, epoch=2)
+        mean_loss, ms = a.train_epoch(batch, 1e-3, shuffle_seed=5, epoch=2)
         order = M.epoch_order(len(pos), 5, 2, oracle.philox)
         losses = []
-        for k in range(len(pos) // 256):  # the trailing partial batch is dropped
-            idx = order[k * 256:(k + 1) * 256]
+        for k in range(len(pos) // batch):  # the trailing partial batch is dropped
+            idx = order[k * batch:(k + 1) * batch]
             losses.append(b.train_step(pos[idx], t[idx], 1e-3))
         np.testing.assert_array_equal(a.get_params(), b.get_params())
         assert abs(mean_loss - np.mean(losses)) <= 1e-5 * np.mean(losses)

@@ -99,6 +99,10 @@ def test_random_1024x1024_full_bitexact(engine, oracle, golden):
     w, tr = oracle.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=POLICY_RANDOM)
     assert np.array_equal(res.winners, w)
     assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
+    # every winner the reference build wrote for this batch
+    expect = golden["full"]["random_1024x1024"]
+    assert np.array_equal(res.winners, expect["winners"])
+    assert res.stats["plies"] == int(expect["plies"].sum(dtype=np.uint64))
 
 
 def test_heuristic_1024x1024_full_bitexact(engine, oracle, golden):
@@ -108,6 +112,9 @@ def test_heuristic_1024x1024_full_bitexact(engine, oracle, golden):
     w, tr = oracle.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=POLICY_HEURISTIC)
     assert np.array_equal(res.winners, w)
     assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
+    expect = golden["full"]["heuristic_1024x1024"]
+    assert np.array_equal(res.winners, expect["winners"])
+    assert res.stats["plies"] == int(expect["plies"].sum(dtype=np.uint64))
 
 
 def test_random_64x4096_bitexact(engine, oracle, golden):

@@ -279,3 +279,16 @@ def test_oracle_matches_reference_build_64_positions(oracle):
         _, a = oracle.run_playouts(pos, per_pos=64, seed=4242, epoch=3, policy=policy)
         _, b = ref.run_playouts(pos, per_pos=64, seed=4242, epoch=3, policy=policy)
         assert compare_traces(a, b) == 0
+
+
+@pytest.mark.parametrize("name", ["random_1024x1024", "heuristic_1024x1024"])
+def test_oracle_full_configs_match_reference_golden(oracle, golden, name):
+    """The restatement on the full benchmark batches (1024 x 1024 playouts, seed 12345, epoch 0)
+    against every winner and per-position ply sum the reference build wrote."""
+    policy = 0 if name.startswith("random") else 1
+    pos = golden["positions"]["open1024" if policy == 0 else "mid1024"]
+    w, tr = oracle.run_playouts(pos, per_pos=1024, seed=12345, epoch=0, policy=policy)
+    expect = golden["full"][name]
+    assert np.array_equal(w, expect["winners"])
+    plies = tr["plies"].reshape(1024, 1024).sum(axis=1, dtype=np.uint64)
+    assert np.array_equal(plies, expect["plies"])
