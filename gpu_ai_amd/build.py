@@ -33,7 +33,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I" + os.path
           "-Wall", "-Wno-unused-function"]
 
 # (source, object) pairs of the device library
-LIB_SOURCES = ["playout_kernels.hip", "capi.cpp", "mlp_kernels.hip", "mlp.cpp"]
+LIB_SOURCES = ["playout_kernels.hip", "capi.cpp", "noalloc.hip", "mlp_kernels.hip", "mlp.cpp"]
 # the C++ host layer (caller side: State, PlayoutDriver, GameTree, players, CLI)
 HOST_SOURCES = ["state.cpp", "playout.cpp", "mcts.cpp", "player.cpp", "multisearch.cpp",
                 "netleaf.cpp", "netmatch.cpp", "capi_host.cpp"]

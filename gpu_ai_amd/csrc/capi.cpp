@@ -367,6 +367,10 @@ int mi_create(mi_ctx **out, int device, const mi_tuning *tuning) {
   ctx->cus = (uint32_t)prop.multiProcessorCount;
   ctx->wave = (uint32_t)prop.warpSize;
   ctx->clock_khz = (uint32_t)prop.clockRate;
+  if (gpuai::noalloc_setup(ctx) != MI_OK) {
+    mi_destroy(ctx);
+    return MI_ERR_DEVICE;
+  }
   *out = ctx;
   return MI_OK;
 }

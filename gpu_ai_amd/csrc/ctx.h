@@ -62,6 +62,9 @@ int ensure(mi_ctx *ctx, DevBuf &b, size_t bytes);
 // Grows a pinned host buffer to at least `bytes` (the caller's stream must be idle).
 int ensure_host(mi_ctx *ctx, HostBuf &b, size_t bytes);
 }  // namespace ctxutil
+// Looks up the device address of the allocation-free entry point's scratch array for the
+// context's device (once per device; the device must be current).
+int noalloc_setup(mi_ctx *ctx);
 }  // namespace gpuai
 
 #define MI_HIP(ctx, call)                                                 \

@@ -25,7 +25,8 @@ struct LaunchDesc {
   uint32_t total;
   uint32_t seed;
   uint32_t epoch;
-  uint32_t pad[12];
+  uint32_t ordered;  // 0: the hand-out order of the arguments is ignored (positions in index order)
+  uint32_t pad[11];
 };
 
 struct PlayoutArgs {

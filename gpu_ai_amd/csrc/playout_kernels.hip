@@ -89,11 +89,12 @@ __device__ __forceinline__ uint32_t take_word(Words &w, uint32_t ply) {
 
 template <uint32_t POLICY, bool TRACE, uint32_t REFILL>
 __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
-  if (a.desc) {  // graph replay: the batch description lives in device memory
+  if (a.desc) {  // graph replays and allocation-free launches: the batch lives in device memory
     a.npos = a.desc->npos;
     a.total = a.desc->total;
     a.seed = a.desc->seed;
     a.epoch = a.desc->epoch;
+    if (!a.desc->ordered) a.order = nullptr;
   }
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nthreads = gridDim.x * blockDim.x;

@@ -77,6 +77,8 @@ _SIGS = {
                                 ctypes.POINTER(MiStats)], ctypes.c_int),
     "mi_run_playouts_tally": ([_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _P,
                                ctypes.POINTER(MiStats)], ctypes.c_int),
+    "mi_run_playouts_noalloc": ([_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _P, _P, _P],
+                                ctypes.c_int),
     "mi_gen_moves": ([_P, _P, _U32, _P, _U32, _P], ctypes.c_int),
     "mi_stage": ([_P, _P, _U32, _P, _U32], ctypes.c_int),
     "mi_launch_staged": ([_P, _U32, _U32, _U32, ctypes.POINTER(MiStats)], ctypes.c_int),
@@ -261,6 +263,16 @@ class Engine:
                                                    seed, epoch, policy, _ptr(tallies),
                                                    ctypes.byref(st)))
         return tallies, st.as_dict()
+
+    def run_playouts_noalloc(self, d_pos: int, npos: int, d_counts: int | None, per_pos: int,
+                             d_winners: int, seed: int = 12345, epoch: int = 0,
+                             policy: int = POLICY_RANDOM, d_status: int | None = None,
+                             stream: int | None = None) -> None:
+        """mi_run_playouts_noalloc: device addresses in, winners left in device memory; enqueued
+        on `stream` (a hipStream_t handle; None: the context's stream) without waiting."""
+        self._check(self.lib.mi_run_playouts_noalloc(self.ctx, d_pos, npos, d_counts, per_pos,
+                                                     seed, epoch, policy, d_winners, d_status,
+                                                     stream))
 
     def gen_moves(self, pos, cap: int = 128) -> tuple[np.ndarray, np.ndarray]:
         pos = as_positions(pos)

@@ -192,6 +192,27 @@ int mi_launch_staged(mi_ctx *ctx, uint32_t seed, uint32_t epoch, uint32_t policy
                      mi_stats *stats);
 int mi_fetch_winners(mi_ctx *ctx, int8_t *winners, uint64_t n);
 
+/* Allocation-free variant of mi_run_playouts, for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops that forbid hipMalloc).  No counterpart in the
+ * reference.  Positions, counts and winners are DEVICE pointers owned by the caller: pos holds
+ * npos positions; counts (may be NULL: per_pos playouts each) npos counts; winners one byte per
+ * playout, position-major, as mi_run_playouts returns them (same streams, same results).  The
+ * call enqueues its kernels on `stream` (a hipStream_t; NULL: the context's stream) and returns
+ * without waiting; the results are ready when the stream has run them.  It allocates nothing:
+ * the kernels' work words, offsets and hand-out order live in a fixed device array of about
+ * 1.75 MiB per device, set up by mi_create.  At most MI_NOALLOC_MAX_POSITIONS positions (larger
+ * batches return MI_ERR_INVALID_ARGUMENT), fewer than 2^31 playouts.  Positions and counts in
+ * device memory are checked on the device: status (a device word, may be NULL) receives MI_OK,
+ * or MI_ERR_INVALID_ARGUMENT when a position is malformed or the counts add up to 2^31 or more,
+ * in which case nothing is played.  All calls on one device share its scratch array: calls that
+ * overlap in time on one device (other streams or threads) race; calls in sequence on one
+ * stream do not. */
+#define MI_NOALLOC_MAX_POSITIONS 65536u
+int mi_run_playouts_noalloc(mi_ctx *ctx, const mi_pos *pos, uint32_t npos,
+                            const uint32_t *counts, uint32_t per_pos, uint32_t seed,
+                            uint32_t epoch, uint32_t policy, int8_t *winners, uint32_t *status,
+                            void *stream);
+
 /* Diagnostic: runs the device's exact quotient/remainder routine (used to pick r mod n for
  * every ply) on caller data, q[i] = r[i] / n[i] and rem[i] = r[i] % n[i], n[i] >= 1.  No
  * counterpart in the reference, whose draws use the % operator on the host (src/player.cpp:15). */
