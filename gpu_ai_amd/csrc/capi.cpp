@@ -153,7 +153,7 @@ constexpr size_t kPinnedMax = size_t(64) << 20;
 // synchronised first, so that no earlier copy still reads it); otherwise straight from the
 // caller's memory.
 int upload_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
-                 uint64_t total, bool pinned = false) {
+                 uint64_t total, bool pinned = false, const uint32_t *keys = nullptr) {
   int rc;
   plan_order(ctx, pos, npos, counts, total);
   if (ctx->ordered) {
@@ -175,20 +175,29 @@ int upload_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *
   if ((rc = ensure(ctx, ctx->winners, (size_t)total))) return rc;
   if ((rc = ensure(ctx, ctx->work, sizeof(gpuai::WorkWords)))) return rc;
   if (counts && (rc = ensure(ctx, ctx->offs, sizeof(uint32_t) * ((size_t)npos + 1)))) return rc;
+  if (keys && (rc = ensure(ctx, ctx->keys, sizeof(uint32_t) * (size_t)std::max<uint32_t>(npos, 1))))
+    return rc;
   const size_t pos_bytes = sizeof(mi_pos) * npos;
   const size_t offs_bytes = counts ? sizeof(uint32_t) * ((size_t)npos + 1) : 0;
-  const void *src_pos = pos, *src_offs = ctx->host_offs.data();
-  if (pinned && pos_bytes + offs_bytes <= kPinnedMax) {
+  const size_t keys_bytes = keys ? sizeof(uint32_t) * (size_t)npos : 0;
+  const void *src_pos = pos, *src_offs = ctx->host_offs.data(), *src_keys = keys;
+  if (pinned && pos_bytes + offs_bytes + keys_bytes <= kPinnedMax) {
     MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if ((rc = ensure_host(ctx, ctx->h_in, pos_bytes + offs_bytes))) return rc;
-    std::memcpy(ctx->h_in.ptr, pos, pos_bytes);
-    if (counts) std::memcpy((char *)ctx->h_in.ptr + pos_bytes, ctx->host_offs.data(), offs_bytes);
-    src_pos = ctx->h_in.ptr;
-    src_offs = (const char *)ctx->h_in.ptr + pos_bytes;
+    if ((rc = ensure_host(ctx, ctx->h_in, pos_bytes + offs_bytes + keys_bytes))) return rc;
+    char *h = (char *)ctx->h_in.ptr;
+    std::memcpy(h, pos, pos_bytes);
+    if (counts) std::memcpy(h + pos_bytes, ctx->host_offs.data(), offs_bytes);
+    if (keys) std::memcpy(h + pos_bytes + offs_bytes, keys, keys_bytes);
+    src_pos = h;
+    src_offs = h + pos_bytes;
+    src_keys = h + pos_bytes + offs_bytes;
   }
   MI_HIP(ctx, hipMemcpyAsync(ctx->pos.ptr, src_pos, pos_bytes, hipMemcpyHostToDevice, ctx->stream));
   if (counts)
     MI_HIP(ctx, hipMemcpyAsync(ctx->offs.ptr, src_offs, offs_bytes, hipMemcpyHostToDevice,
+                               ctx->stream));
+  if (keys)
+    MI_HIP(ctx, hipMemcpyAsync(ctx->keys.ptr, src_keys, keys_bytes, hipMemcpyHostToDevice,
                                ctx->stream));
   return MI_OK;
 }
@@ -205,7 +214,7 @@ uint32_t grid_for(const mi_ctx *ctx, uint64_t total) {
 // one synchronisation).
 int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, bool with_offs,
                uint32_t seed, uint32_t epoch, uint32_t policy, bool trace, mi_stats *stats,
-               int8_t *winners = nullptr) {
+               int8_t *winners = nullptr, bool keyed = false) {
   int rc;
   if ((rc = ensure_host(ctx, ctx->h_back, sizeof(gpuai::WorkWords) + (winners ? total : 0))))
     return rc;
@@ -226,6 +235,7 @@ int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
   a.ord_pos = ctx->ordered ? (const mi_pos *)ctx->ord_pos.ptr : nullptr;
   a.order = ctx->ordered ? (const uint32_t *)ctx->order.ptr : nullptr;
   a.ord_offs = ctx->ordered && with_offs ? (const uint32_t *)ctx->ord_offs.ptr : nullptr;
+  a.keys = keyed ? (const uint32_t *)ctx->keys.ptr : nullptr;
   const uint32_t grid = grid_for(ctx, total);
   MI_HIP(ctx, gpuai::launch_reset(a.work, ctx->stream));
   MI_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
@@ -270,7 +280,7 @@ void finish_stats(mi_stats *stats, Clock::time_point t0) {
 int run_common(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
                uint32_t per_pos, uint32_t seed, uint32_t epoch, uint32_t policy, bool trace,
                int8_t *winners, uint32_t *plies, uint32_t *hashes, uint32_t *tallies,
-               mi_stats *stats) {
+               mi_stats *stats, const uint32_t *keys = nullptr) {
   if (!ctx) return MI_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->mu);
   const auto t0 = Clock::now();
@@ -287,7 +297,7 @@ int run_common(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *co
   }
   if (!winners && !tallies) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "winners pointer is null");
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
-  if ((rc = upload_batch(ctx, pos, npos, counts, total, true))) return rc;
+  if ((rc = upload_batch(ctx, pos, npos, counts, total, true, keys))) return rc;
   if (trace) {
     if ((rc = ensure(ctx, ctx->plies, sizeof(uint32_t) * total))) return rc;
     if ((rc = ensure(ctx, ctx->hashes, sizeof(uint32_t) * total))) return rc;
@@ -295,7 +305,7 @@ int run_common(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *co
   ctx->staged = false;  // the batch buffers now hold this call's data
   const bool pinned_back = total <= kPinnedMax;
   if ((rc = run_kernel(ctx, npos, per_pos, total, counts != nullptr, seed, epoch, policy, trace,
-                       stats, pinned_back ? winners : nullptr)))
+                       stats, pinned_back ? winners : nullptr, keys != nullptr)))
     return rc;
   if (winners && !pinned_back)
     MI_HIP(ctx, hipMemcpyAsync(winners, ctx->winners.ptr, total, hipMemcpyDeviceToHost,
@@ -381,7 +391,7 @@ int mi_destroy(mi_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DevBuf *bufs[] = {&ctx->pos,   &ctx->offs,    &ctx->winners, &ctx->plies,  &ctx->hashes,
                     &ctx->work,  &ctx->tallies, &ctx->moves,   &ctx->counts, &ctx->ord_pos,
-                    &ctx->order, &ctx->ord_offs};
+                    &ctx->order, &ctx->ord_offs, &ctx->keys};
   for (DevBuf *b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->h_in.ptr) (void)hipHostFree(ctx->h_in.ptr);
@@ -426,6 +436,13 @@ int mi_run_playouts(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_
                     int8_t *winners, mi_stats *stats) {
   return run_common(ctx, pos, npos, counts, per_pos, seed, epoch, policy, false, winners, nullptr,
                     nullptr, nullptr, stats);
+}
+
+int mi_run_playouts_keyed(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                          uint32_t per_pos, const uint32_t *keys, uint32_t seed, uint32_t epoch,
+                          uint32_t policy, int8_t *winners, mi_stats *stats) {
+  return run_common(ctx, pos, npos, counts, per_pos, seed, epoch, policy, false, winners, nullptr,
+                    nullptr, nullptr, stats, keys);
 }
 
 int mi_run_playouts_traced(mi_ctx *ctx, const mi_pos *pos, uint32_t npos,
@@ -709,9 +726,10 @@ int mi_valu_rate(mi_ctx *ctx, uint32_t iters, uint32_t repeats, double *lane_ops
 struct mi_graph {
   mi_ctx *ctx = nullptr;
   uint32_t max_pos = 0, max_play = 0, policy = 0;
-  // pinned host blob and its device twin: LaunchDesc | offsets (max_pos + 1) | positions
+  // pinned host blob and its device twin: LaunchDesc | positions | offsets (npos + 1) | keys
+  // (npos, keyed batches only), packed per batch so that one copy of the used prefix uploads it
   void *h_blob = nullptr, *d_blob = nullptr;
-  size_t offs_off = 0, pos_off = 0, blob_bytes = 0;
+  size_t blob_bytes = 0;
   // pinned host output and its device twin: WorkWords | winners (max_play)
   void *h_out = nullptr, *d_out = nullptr;
   size_t out_bytes = 0;
@@ -760,9 +778,12 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
   g->max_pos = max_positions;
   g->max_play = max_playouts;
   g->policy = policy;
-  g->offs_off = sizeof(gpuai::LaunchDesc);
-  g->pos_off = (g->offs_off + sizeof(uint32_t) * ((size_t)max_positions + 1) + 15) & ~(size_t)15;
-  g->blob_bytes = g->pos_off + sizeof(mi_pos) * (size_t)max_positions;
+  g->blob_bytes = sizeof(gpuai::LaunchDesc) + (sizeof(mi_pos) + 2 * sizeof(uint32_t)) *
+                                                  (size_t)max_positions + sizeof(uint32_t);
+  if (g->blob_bytes > 0xFFFFFFF0ull) {
+    delete g;
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "too many positions for one graph batch");
+  }
   g->out_bytes = sizeof(gpuai::WorkWords) + (size_t)max_playouts;
   hipError_t e = hipHostMalloc(&g->h_blob, g->blob_bytes, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc(&g->h_out, g->out_bytes, hipHostMallocDefault);
@@ -779,8 +800,10 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
   gpuai::WorkWords *work = (gpuai::WorkWords *)g->d_out;
   gpuai::PlayoutArgs &a = g->args;
   std::memset(&a, 0, sizeof(a));
-  a.pos = (const mi_pos *)((char *)g->d_blob + g->pos_off);
-  a.offs = (const uint32_t *)((char *)g->d_blob + g->offs_off);
+  // positions, offsets and keys are found through the description's section offsets
+  a.pos = (const mi_pos *)((char *)g->d_blob + sizeof(gpuai::LaunchDesc));
+  a.offs = nullptr;
+  a.keys = nullptr;
   a.winners = (int8_t *)((char *)g->d_out + sizeof(gpuai::WorkWords));
   a.work = work;
   a.defer_min = ctx->tuning.defer_min;
@@ -822,7 +845,7 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
 }
 
 int mi_graph_launch(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
-                    uint32_t seed, uint32_t epoch) {
+                    const uint32_t *keys, uint32_t seed, uint32_t epoch) {
   if (!g) return MI_ERR_INVALID_ARGUMENT;
   mi_ctx *ctx = g->ctx;
   std::lock_guard<std::mutex> lock(ctx->mu);
@@ -837,14 +860,23 @@ int mi_graph_launch(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_
   if (total > 0) {
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
     gpuai::LaunchDesc *d = (gpuai::LaunchDesc *)g->h_blob;
+    const size_t pos_at = sizeof(gpuai::LaunchDesc);
+    const size_t offs_at = pos_at + sizeof(mi_pos) * (size_t)npos;
+    const size_t keys_at = offs_at + sizeof(uint32_t) * ((size_t)npos + 1);
+    const size_t in_bytes = keys_at + (keys ? sizeof(uint32_t) * (size_t)npos : 0);
+    std::memset(d, 0, sizeof(*d));
     d->npos = npos;
     d->total = (uint32_t)total;
     d->seed = seed;
     d->epoch = epoch;
-    std::memcpy((char *)g->h_blob + g->offs_off, ctx->host_offs.data(),
-                sizeof(uint32_t) * ((size_t)npos + 1));
-    std::memcpy((char *)g->h_blob + g->pos_off, pos, sizeof(mi_pos) * (size_t)npos);
-    const size_t in_bytes = g->pos_off + sizeof(mi_pos) * (size_t)npos;
+    d->ordered = 0;
+    d->pos_at = (uint32_t)pos_at;
+    d->offs_at = (uint32_t)offs_at;
+    d->keys_at = keys ? (uint32_t)keys_at : 0u;
+    char *h = (char *)g->h_blob;
+    std::memcpy(h + pos_at, pos, sizeof(mi_pos) * (size_t)npos);
+    std::memcpy(h + offs_at, ctx->host_offs.data(), sizeof(uint32_t) * ((size_t)npos + 1));
+    if (keys) std::memcpy(h + keys_at, keys, sizeof(uint32_t) * (size_t)npos);
     const size_t out_bytes = sizeof(gpuai::WorkWords) + (size_t)total;
     MI_HIP(ctx, hipGraphExecMemcpyNodeSetParams1D(g->exec, g->n_d2h, g->h_out, g->d_out, out_bytes,
                                                   hipMemcpyDeviceToHost));
@@ -901,7 +933,8 @@ int mi_graph_finish(mi_graph *g, int8_t *winners, mi_stats *stats) {
 }
 
 int mi_graph_run(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
-                 uint32_t seed, uint32_t epoch, int8_t *winners, mi_stats *stats) {
+                 const uint32_t *keys, uint32_t seed, uint32_t epoch, int8_t *winners,
+                 mi_stats *stats) {
   const auto t0 = Clock::now();
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (npos > 0 && counts && pos) {  // a null winners buffer is refused before anything runs
@@ -910,7 +943,7 @@ int mi_graph_run(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_t *
     if (total > 0 && !winners) return g ? fail(g->ctx, MI_ERR_INVALID_ARGUMENT, "winners pointer is null")
                                         : MI_ERR_INVALID_ARGUMENT;
   }
-  int rc = mi_graph_launch(g, pos, npos, counts, seed, epoch);
+  int rc = mi_graph_launch(g, pos, npos, counts, keys, seed, epoch);
   if (rc) return rc;
   rc = mi_graph_finish(g, winners, stats);
   finish_stats(stats, t0);

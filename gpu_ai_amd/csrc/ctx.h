@@ -33,7 +33,7 @@ struct mi_ctx {
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   mi_tuning tuning = kDefaultTuning;
   uint32_t cus = 0, wave = 64, clock_khz = 0;
-  gpuai::DevBuf pos, offs, winners, plies, hashes, work, tallies, moves, counts;
+  gpuai::DevBuf pos, offs, winners, plies, hashes, work, tallies, moves, counts, keys;
   // longest-first order of the uploaded batch (see upload_batch): positions permuted into it,
   // the original index of each, and (uneven counts) prefix offsets in that order
   gpuai::DevBuf ord_pos, order, ord_offs;

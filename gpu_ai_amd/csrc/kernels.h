@@ -26,7 +26,11 @@ struct LaunchDesc {
   uint32_t seed;
   uint32_t epoch;
   uint32_t ordered;  // 0: the hand-out order of the arguments is ignored (positions in index order)
-  uint32_t pad[11];
+  // Byte offsets from this description of the batch's sections when they follow it in one
+  // block of device memory (graph replays pack positions, offsets and keys behind it, so that one
+  // copy of the used prefix uploads a batch); 0: the pointer of the arguments (keys: none).
+  uint32_t pos_at, offs_at, keys_at;
+  uint32_t pad[8];
 };
 
 struct PlayoutArgs {
@@ -39,6 +43,9 @@ struct PlayoutArgs {
   const mi_pos *ord_pos;
   const uint32_t *order;
   const uint32_t *ord_offs;
+  // Optional stream keys: the Philox key of position i's playouts is (seed, keys[i]) instead of
+  // (seed, i), so that a playout's stream need not depend on where its position sits in a batch.
+  const uint32_t *keys;
   uint32_t npos;
   uint32_t per_pos;
   uint32_t total;          // total playouts (< 2^31)

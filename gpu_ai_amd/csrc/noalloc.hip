@@ -156,6 +156,9 @@ __global__ void __launch_bounds__(kPrepThreads)
     S->desc.seed = seed;
     S->desc.epoch = epoch;
     S->desc.ordered = ordered ? 1u : 0u;
+    S->desc.pos_at = 0u;  // positions, offsets and order are the kernel arguments
+    S->desc.offs_at = 0u;
+    S->desc.keys_at = 0u;
     if (status) *status = ok ? (uint32_t)MI_OK : (uint32_t)MI_ERR_INVALID_ARGUMENT;
   }
 }
@@ -213,6 +216,7 @@ extern "C" int mi_run_playouts_noalloc(mi_ctx *ctx, const mi_pos *pos, uint32_t 
   a.ord_pos = S->ord_pos;  // used only when the prep kernel ordered the batch (desc.ordered)
   a.order = S->order;
   a.ord_offs = counts ? S->ord_offs : nullptr;
+  a.keys = nullptr;
   a.npos = npos;
   a.per_pos = per_pos;
   a.total = 0;  // read from the description

@@ -90,11 +90,16 @@ __device__ __forceinline__ uint32_t take_word(Words &w, uint32_t ply) {
 template <uint32_t POLICY, bool TRACE, uint32_t REFILL>
 __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
   if (a.desc) {  // graph replays and allocation-free launches: the batch lives in device memory
-    a.npos = a.desc->npos;
-    a.total = a.desc->total;
-    a.seed = a.desc->seed;
-    a.epoch = a.desc->epoch;
-    if (!a.desc->ordered) a.order = nullptr;
+    const LaunchDesc *d = a.desc;
+    a.npos = d->npos;
+    a.total = d->total;
+    a.seed = d->seed;
+    a.epoch = d->epoch;
+    if (!d->ordered) a.order = nullptr;
+    const char *base = (const char *)d;
+    if (d->pos_at) a.pos = (const mi_pos *)(base + d->pos_at);
+    if (d->offs_at) a.offs = (const uint32_t *)(base + d->offs_at);
+    a.keys = d->keys_at ? (const uint32_t *)(base + d->keys_at) : nullptr;
   }
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nthreads = gridDim.x * blockDim.x;
@@ -153,6 +158,7 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
         deferred = false;
         mi_pos q;
         locate(a, g, p, j, g, q);  // from here on g is the playout's result slot
+        if (a.keys) p = a.keys[p];  // from here on p is the stream key
         L = ck::lane_from_abs(q.p1, q.p2, q.kings, q.turn, q.counter);
         ply = 0;
         h = 0;

@@ -73,6 +73,8 @@ _SIGS = {
     "mi_device_info": ([_P, _P, _P, _P], ctypes.c_int),
     "mi_run_playouts": ([_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _P, ctypes.POINTER(MiStats)],
                         ctypes.c_int),
+    "mi_run_playouts_keyed": ([_P, _P, _U32, _P, _U32, _P, _U32, _U32, _U32, _P,
+                               ctypes.POINTER(MiStats)], ctypes.c_int),
     "mi_run_playouts_traced": ([_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _P, _P, _P,
                                 ctypes.POINTER(MiStats)], ctypes.c_int),
     "mi_run_playouts_tally": ([_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _P,
@@ -88,9 +90,10 @@ _SIGS = {
     "mi_autotune": ([_P, _P, _U32, _P, _U32, _U32, _U32, _P, _U32, _P, ctypes.POINTER(MiTuning)],
                     ctypes.c_int),
     "mi_graph_create": ([_P, _U32, _U32, _U32, ctypes.POINTER(_P)], ctypes.c_int),
-    "mi_graph_run": ([_P, _P, _U32, _P, _U32, _U32, _P, ctypes.POINTER(MiStats)], ctypes.c_int),
+    "mi_graph_run": ([_P, _P, _U32, _P, _P, _U32, _U32, _P, ctypes.POINTER(MiStats)],
+                     ctypes.c_int),
     "mi_graph_destroy": ([_P], ctypes.c_int),
-    "mi_graph_launch": ([_P, _P, _U32, _P, _U32, _U32], ctypes.c_int),
+    "mi_graph_launch": ([_P, _P, _U32, _P, _P, _U32, _U32], ctypes.c_int),
     "mi_graph_finish": ([_P, _P, ctypes.POINTER(MiStats)], ctypes.c_int),
     "mi_copy_calibration": ([_P, ctypes.c_uint64, _U32, ctypes.POINTER(ctypes.c_float)],
                             ctypes.c_int),
@@ -231,13 +234,23 @@ class Engine:
 
     # -- entry points --
     def run_playouts(self, pos, per_pos: int | None = None, counts=None, seed: int = 12345,
-                     epoch: int = 0, policy: int = POLICY_RANDOM) -> PlayoutResult:
+                     epoch: int = 0, policy: int = POLICY_RANDOM, keys=None) -> PlayoutResult:
+        """mi_run_playouts, or mi_run_playouts_keyed when `keys` (one u32 per position) is
+        given."""
         pos = as_positions(pos)
         c, pp, total = self._counts(len(pos), per_pos, counts)
         winners = np.empty(total, dtype=np.int8)
         st = MiStats()
-        self._check(self.lib.mi_run_playouts(self.ctx, _ptr(pos), len(pos), _ptr(c), pp, seed,
-                                             epoch, policy, _ptr(winners), ctypes.byref(st)))
+        if keys is None:
+            self._check(self.lib.mi_run_playouts(self.ctx, _ptr(pos), len(pos), _ptr(c), pp, seed,
+                                                 epoch, policy, _ptr(winners), ctypes.byref(st)))
+        else:
+            k = np.ascontiguousarray(keys, dtype=np.uint32)
+            if k.shape != (len(pos),):
+                raise ValueError("keys must have one entry per position")
+            self._check(self.lib.mi_run_playouts_keyed(self.ctx, _ptr(pos), len(pos), _ptr(c), pp,
+                                                       _ptr(k), seed, epoch, policy,
+                                                       _ptr(winners), ctypes.byref(st)))
         return PlayoutResult(winners, st.as_dict())
 
     def run_traced(self, pos, per_pos: int | None = None, counts=None, seed: int = 12345,
@@ -361,23 +374,34 @@ class GraphBatch:
                                                  ctypes.byref(self.handle)))
         _open_children.add(self)
 
-    def run(self, pos, counts, seed: int = 12345, epoch: int = 0) -> PlayoutResult:
+    @staticmethod
+    def _keys(keys, npos: int):
+        if keys is None:
+            return None
+        k = np.ascontiguousarray(keys, dtype=np.uint32)
+        if k.shape != (npos,):
+            raise ValueError("keys must have one entry per position")
+        return k
+
+    def run(self, pos, counts, seed: int = 12345, epoch: int = 0, keys=None) -> PlayoutResult:
         pos = as_positions(pos)
         c = np.ascontiguousarray(counts, dtype=np.uint32)
+        k = self._keys(keys, len(pos))
         winners = np.empty(int(c.sum(dtype=np.uint64)), dtype=np.int8)
         st = MiStats()
         self.engine._check(self.engine.lib.mi_graph_run(self.handle, _ptr(pos), len(pos), _ptr(c),
-                                                        seed, epoch, _ptr(winners),
+                                                        _ptr(k), seed, epoch, _ptr(winners),
                                                         ctypes.byref(st)))
         return PlayoutResult(winners, st.as_dict())
 
-    def launch(self, pos, counts, seed: int = 12345, epoch: int = 0) -> None:
+    def launch(self, pos, counts, seed: int = 12345, epoch: int = 0, keys=None) -> None:
         """Enqueues one replay and returns at once (the batch is copied before returning)."""
         pos = as_positions(pos)
         c = np.ascontiguousarray(counts, dtype=np.uint32)
+        k = self._keys(keys, len(pos))
         self._total = int(c.sum(dtype=np.uint64))
         self.engine._check(self.engine.lib.mi_graph_launch(self.handle, _ptr(pos), len(pos),
-                                                           _ptr(c), seed, epoch))
+                                                           _ptr(c), _ptr(k), seed, epoch))
 
     def finish(self) -> PlayoutResult:
         """Waits for the replay enqueued by launch and returns its winners."""

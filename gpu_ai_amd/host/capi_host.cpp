@@ -8,6 +8,7 @@
 #include "mcts.hpp"
 #include "multisearch.hpp"
 #include "netmatch.hpp"
+#include "playout.hpp"
 #include "state.hpp"
 
 namespace {
@@ -125,7 +126,28 @@ int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch
     c.autotune = cfg->autotune != 0;
     if (c.maxBudget < c.minBudget || c.searches == 0 || cfg->mode > 3)
       throw std::runtime_error("invalid multi-search configuration");
+    if (cfg->hook) {
+      const gpuai_batch_hook hook = cfg->hook;
+      void *user = cfg->hook_user;
+      c.hook = [hook, user](const MultiSearchBatch &b) {
+        gpuai_batch_record rec;
+        std::memset(&rec, 0, sizeof(rec));
+        rec.iteration = b.iteration;
+        rec.group = b.group;
+        rec.seed = b.seed;
+        rec.policy = b.policy;
+        rec.npos = (uint32_t)b.pos->size();
+        rec.playouts = b.winners->size();
+        rec.pos = b.pos->data();
+        rec.counts = b.counts->data();
+        rec.keys = b.keys->data();
+        rec.winners = b.winners->data();
+        hook(user, &rec);
+      };
+    }
     MultiSearchResult r = runMultiSearch(c);
+    if (cfg->budgets) std::copy(r.budgets.begin(), r.budgets.end(), cfg->budgets);
+    if (cfg->root_trials) std::copy(r.rootTrials.begin(), r.rootTrials.end(), cfg->root_trials);
     out->playouts = r.playouts;
     out->iterations = r.iterations;
     out->pad = 0;
@@ -140,6 +162,72 @@ int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch
     out->playout_ms = r.playoutMs;
     out->update_ms = r.updateMs;
     out->tuning = r.tuning;
+    return 0;
+  } catch (const std::exception &e) {
+    if (err && err_len) {
+      std::strncpy(err, e.what(), err_len - 1);
+      err[err_len - 1] = 0;
+    }
+    return -1;
+  }
+}
+
+int gpuai_driver_run_playouts(const mi_pos *states, uint32_t n, uint32_t policy, uint32_t seed,
+                              uint32_t calls, int8_t *winners, char *err, uint32_t err_len) {
+  try {
+    if ((n && !states) || (n && calls && !winners)) throw std::runtime_error("null argument");
+    DeviceMI355XPlayoutDriver driver(policy, 0, seed);
+    std::vector<State> v;
+    for (uint32_t i = 0; i < n; i++) v.push_back(State::unpack(states[i]));
+    for (uint32_t c = 0; c < calls; c++) {
+      const std::vector<PlayerId> r = driver.runPlayouts(v);
+      if (r.size() != n) throw std::runtime_error("runPlayouts returned the wrong count");
+      for (uint32_t i = 0; i < n; i++) winners[(size_t)c * n + i] = (int8_t)r[i];
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    if (err && err_len) {
+      std::strncpy(err, e.what(), err_len - 1);
+      err[err_len - 1] = 0;
+    }
+    return -1;
+  }
+}
+
+int gpuai_search_record(const mi_pos *root, uint32_t iterations, uint32_t initial, double scale,
+                        uint32_t policy, uint32_t seed, gpuai_batch_hook hook, void *hook_user,
+                        char *err, uint32_t err_len) {
+  try {
+    if (!root) throw std::runtime_error("null root");
+    DeviceMI355XPlayoutDriver driver(policy, 0, seed);
+    GameTree tree(State::unpack(*root));
+    std::vector<mi_pos> packed;
+    std::vector<int8_t> w;
+    for (uint32_t it = 0; it < iterations; it++) {
+      unsigned n = (unsigned)(tree.getTotalTrials() * scale);  // src/player.cpp:137-139
+      if (n == 0) n = initial;
+      const std::vector<State> leaves = tree.select(n);
+      const std::vector<PlayerId> results = driver.runPlayouts(leaves);
+      if (hook) {
+        packed.resize(leaves.size());
+        w.resize(leaves.size());
+        for (size_t i = 0; i < leaves.size(); i++) {
+          packed[i] = leaves[i].pack();
+          w[i] = (int8_t)results[i];
+        }
+        gpuai_batch_record rec;
+        std::memset(&rec, 0, sizeof(rec));
+        rec.iteration = it;
+        rec.seed = seed;
+        rec.policy = policy;
+        rec.npos = (uint32_t)leaves.size();
+        rec.playouts = leaves.size();
+        rec.pos = packed.data();
+        rec.winners = w.data();
+        hook(hook_user, &rec);
+      }
+      tree.update(results);
+    }
     return 0;
   } catch (const std::exception &e) {
     if (err && err_len) {

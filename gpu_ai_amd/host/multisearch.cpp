@@ -68,6 +68,8 @@ mi_tuning tuneSearches(mi_ctx *ctx, const MultiSearchConfig &cfg, const std::vec
 
 // Latency percentiles, throughput and the digest of every root's statistics.
 void finishResult(MultiSearchResult &res, const std::vector<std::unique_ptr<GameTree>> &trees) {
+  res.rootTrials.clear();
+  for (auto &t : trees) res.rootTrials.push_back(t->getTotalTrials());
   std::vector<double> lat = res.iterationMs;
   std::sort(lat.begin(), lat.end());
   if (!lat.empty()) {
@@ -87,11 +89,12 @@ void finishResult(MultiSearchResult &res, const std::vector<std::unique_ptr<Game
 
 // Packs the leaves of searches `ids` into one batch: each run of equal consecutive leaves of a
 // search becomes one position with a count (as DeviceMI355XPlayoutDriver::runPlayouts does), in
-// search order.  Searches are packed in parallel into slots given by a prefix sum of their run
-// counts.  Returns the batch's playouts; `first` receives each search's first playout index.
+// search order, keyed (s << 16 | r) for run r of search s.  Searches are packed in parallel into
+// slots given by a prefix sum of their run counts.  Returns the batch's playouts; `first`
+// receives each search's first playout index.
 uint64_t packLeaves(const std::vector<uint32_t> &ids, const std::vector<std::vector<State>> &leaves,
                     std::vector<mi_pos> &pos, std::vector<uint32_t> &counts,
-                    std::vector<uint64_t> &first) {
+                    std::vector<uint32_t> &keys, std::vector<uint64_t> &first) {
   const long m = (long)ids.size();
   std::vector<size_t> runs(ids.size() + 1, 0);
 #pragma omp parallel for schedule(dynamic, 8)
@@ -108,21 +111,56 @@ uint64_t packLeaves(const std::vector<uint32_t> &ids, const std::vector<std::vec
   }
   pos.resize(runs[m]);
   counts.resize(runs[m]);
+  keys.resize(runs[m]);
 #pragma omp parallel for schedule(dynamic, 8)
   for (long k = 0; k < m; k++) {
     const std::vector<State> &L = leaves[ids[k]];
-    size_t at = runs[k];
+    const size_t base = runs[k];
+    size_t at = base;
     for (size_t i = 0; i < L.size(); i++) {
       if (i > 0 && L[i] == L[i - 1]) {
         counts[at - 1]++;
       } else {
         pos[at] = L[i].pack();
         counts[at] = 1;
+        keys[at] = (ids[k] << 16) | (uint32_t)(at - base);  // fewer than 2^16 runs per search
         at++;
       }
     }
   }
   return first[m];
+}
+
+// Trials search s hands out this iteration: max(initial, totalTrials * scale), capped by the
+// remaining budget and by kMaxTrialsPerIteration.
+unsigned trialsFor(const MultiSearchConfig &cfg, const GameTree &tree, uint32_t remaining) {
+  unsigned n = (unsigned)(tree.getTotalTrials() * cfg.scale);
+  n = std::max<unsigned>(n, cfg.initial);
+  n = std::min<unsigned>(n, remaining);
+  return std::min<unsigned>(n, kMaxTrialsPerIteration);
+}
+
+// The most trials a search with this budget can hand out in one iteration (graph capacity).
+uint64_t iterationCap(const MultiSearchConfig &cfg, uint32_t budget) {
+  uint64_t n = std::min<uint64_t>(budget, kMaxTrialsPerIteration);
+  if (cfg.scale <= 0.0) n = std::min<uint64_t>(n, std::max<uint32_t>(cfg.initial, 1u));
+  return n;
+}
+
+void callHook(const MultiSearchConfig &cfg, uint32_t iteration, uint32_t group,
+              const std::vector<mi_pos> &pos, const std::vector<uint32_t> &counts,
+              const std::vector<uint32_t> &keys, const std::vector<int8_t> &winners) {
+  if (!cfg.hook) return;
+  MultiSearchBatch b;
+  b.iteration = iteration;
+  b.group = group;
+  b.seed = cfg.playoutSeed;
+  b.policy = cfg.policy;
+  b.pos = &pos;
+  b.counts = &counts;
+  b.keys = &keys;
+  b.winners = &winners;
+  cfg.hook(b);
 }
 
 // Hands each search of `ids` its leaves' winners (leaf i of search k is playout first[k] + i).
@@ -146,9 +184,11 @@ struct SearchGroup {
   std::unique_ptr<DeviceMI355XPlayoutDriver> driver;
   mi_graph *graph = nullptr;
   std::vector<mi_pos> pos;
-  std::vector<uint32_t> counts;
+  std::vector<uint32_t> counts, keys;
   std::vector<int8_t> winners;
   std::vector<uint64_t> first;  // each search's first playout in the batch
+  uint32_t index = 0;           // the group's number
+  uint32_t launches = 0;        // the group's searches' iteration (the next batch's epoch)
   bool launched = false;
   ~SearchGroup() {
     if (graph) mi_graph_destroy(graph);
@@ -172,10 +212,11 @@ MultiSearchResult runPipelined(const MultiSearchConfig &cfg,
   for (uint32_t k = 0; k < ngroups; k++) {
     groups.emplace_back(new SearchGroup());
     SearchGroup &Gk = *groups.back();
+    Gk.index = k;
     for (uint32_t s = k; s < S; s += ngroups) Gk.ids.push_back(s);
-    // independent streams: group k draws under seed + k
+    // one seed for every group: the stream keys tell the searches apart
     Gk.driver = std::make_unique<DeviceMI355XPlayoutDriver>(cfg.policy, cfg.device,
-                                                            cfg.playoutSeed + k);
+                                                            cfg.playoutSeed);
   }
   // one launch shape for every group (a graph keeps the shape it was created with)
   MultiSearchResult res;
@@ -185,7 +226,7 @@ MultiSearchResult runPipelined(const MultiSearchConfig &cfg,
     checkMi(mi_set_tuning(Gk.driver->context(), &res.tuning), Gk.driver->context(),
             "mi_set_tuning");
     uint64_t cap = 0;
-    for (uint32_t s : Gk.ids) cap += remaining[s];
+    for (uint32_t s : Gk.ids) cap += iterationCap(cfg, remaining[s]);
     cap = std::max<uint64_t>(cap, 1);
     checkMi(mi_graph_create(Gk.driver->context(), (uint32_t)std::min<uint64_t>(cap, 1u << 22),
                             (uint32_t)std::min<uint64_t>(cap, (1u << 31) - 1), cfg.policy,
@@ -203,18 +244,15 @@ MultiSearchResult runPipelined(const MultiSearchConfig &cfg,
       const uint32_t s = G.ids[i];
       leaves[s].clear();
       if (remaining[s] == 0) continue;
-      unsigned n = (unsigned)(trees[s]->getTotalTrials() * cfg.scale);
-      n = std::max<unsigned>(n, cfg.initial);
-      n = std::min<unsigned>(n, remaining[s]);
-      trees[s]->selectInto(n, leaves[s]);
+      trees[s]->selectInto(trialsFor(cfg, *trees[s], remaining[s]), leaves[s]);
     }
     res.selectMs += msSince(t0);
     const auto t1 = Clock::now();
-    packLeaves(G.ids, leaves, G.pos, G.counts, G.first);
+    packLeaves(G.ids, leaves, G.pos, G.counts, G.keys, G.first);
     G.launched = !G.pos.empty();
     if (G.launched)
       checkMi(mi_graph_launch(G.graph, G.pos.data(), (uint32_t)G.pos.size(), G.counts.data(),
-                              G.driver->seed(), G.driver->nextEpoch()),
+                              G.keys.data(), G.driver->seed(), G.launches),
               G.driver->context(), "mi_graph_launch");
     res.playoutMs += msSince(t1);
   };
@@ -227,6 +265,8 @@ MultiSearchResult runPipelined(const MultiSearchConfig &cfg,
             "mi_graph_finish");
     G.launched = false;
     res.deviceMs += st.kernel_ms;
+    callHook(cfg, G.launches, G.index, G.pos, G.counts, G.keys, G.winners);
+    G.launches++;
     scatterResults(G.ids, leaves, G.first, G.winners, results);
     res.playoutMs += msSince(t0);
     const auto t1 = Clock::now();
@@ -268,6 +308,8 @@ MultiSearchResult runPipelined(const MultiSearchConfig &cfg,
 }  // namespace
 
 MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
+  if (cfg.searches == 0 || cfg.searches > 65536)
+    throw std::runtime_error("the multi-search takes 1 to 65,536 searches");
   std::vector<mi_pos> roots;
   std::vector<uint32_t> budgets;
   multiSearchSetup(cfg, roots, budgets);
@@ -280,6 +322,7 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
 
   if (cfg.mode == MultiSearchMode::PIPELINED) {
     MultiSearchResult res = runPipelined(cfg, trees, remaining, roots);
+    res.budgets = budgets;
     finishResult(res, trees);
     return res;
   }
@@ -289,8 +332,8 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
   if (cfg.mode == MultiSearchMode::GRAPH) {
     // the largest batch: every search's largest possible iteration
     uint64_t cap = 0;
-    for (uint32_t b : budgets) cap += b;
-    cap = std::min<uint64_t>(cap, (uint64_t)S * std::max<uint64_t>(cfg.initial, cfg.maxBudget));
+    for (uint32_t b : budgets) cap += iterationCap(cfg, b);
+    cap = std::max<uint64_t>(cap, 1);
     checkMi(mi_graph_create(driver.context(), (uint32_t)std::min<uint64_t>(cap, 1u << 22),
                             (uint32_t)std::min<uint64_t>(cap, (1u << 31) - 1), cfg.policy, &graph),
             driver.context(), "mi_graph_create");
@@ -298,10 +341,11 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
 
   MultiSearchResult res;
   res.tuning = tuning;
+  res.budgets = budgets;
   std::vector<std::vector<State>> leaves(S);
   std::vector<std::vector<PlayerId>> results(S);
   std::vector<mi_pos> pos;
-  std::vector<uint32_t> counts;
+  std::vector<uint32_t> counts, keys;
   std::vector<int8_t> winners;
   std::vector<uint64_t> first;
   std::vector<uint32_t> all(S);
@@ -312,42 +356,46 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
     bool any = false;
     for (uint32_t s = 0; s < S; s++) any |= remaining[s] > 0;
     if (!any) break;
-    auto ms_since = [](Clock::time_point a) {
-      return std::chrono::duration<double, std::milli>(Clock::now() - a).count();
-    };
+    // every unfinished search takes part in every iteration, so the iteration count is each
+    // active search's own iteration: the epoch of its batch in every mode
+    const uint32_t iteration = res.iterations;
 #pragma omp parallel for schedule(dynamic, 4)
     for (long s = 0; s < (long)S; s++) {
       leaves[s].clear();
       if (remaining[s] == 0) continue;
-      unsigned n = (unsigned)(trees[s]->getTotalTrials() * cfg.scale);
-      n = std::max<unsigned>(n, cfg.initial);
-      n = std::min<unsigned>(n, remaining[s]);
-      leaves[s] = trees[s]->select(n);
+      trees[s]->selectInto(trialsFor(cfg, *trees[s], remaining[s]), leaves[s]);
     }
-    res.selectMs += ms_since(t0);
+    res.selectMs += msSince(t0);
     const auto t1 = Clock::now();
     uint64_t iterPlayouts = 0;
     if (cfg.mode == MultiSearchMode::PER_SEARCH) {
+      // one device call per search (the reference's pattern: each search calls its driver)
       for (uint32_t s = 0; s < S; s++) {
         if (leaves[s].empty()) continue;
-        results[s] = driver.runPlayouts(leaves[s]);
-        iterPlayouts += leaves[s].size();
+        const std::vector<uint32_t> one(1, s);
+        iterPlayouts += packLeaves(one, leaves, pos, counts, keys, first);
+        mi_stats st;
+        driver.runKeyed(pos, counts, keys, iteration, winners, &st);
+        res.deviceMs += st.kernel_ms;
+        callHook(cfg, iteration, s, pos, counts, keys, winners);
+        scatterResults(one, leaves, first, winners, results);
       }
     } else {
-      iterPlayouts = packLeaves(all, leaves, pos, counts, first);
+      iterPlayouts = packLeaves(all, leaves, pos, counts, keys, first);
       mi_stats st;
       if (cfg.mode == MultiSearchMode::GRAPH) {
         winners.resize(iterPlayouts);
-        checkMi(mi_graph_run(graph, pos.data(), (uint32_t)pos.size(), counts.data(), driver.seed(),
-                             driver.nextEpoch(), winners.data(), &st),
+        checkMi(mi_graph_run(graph, pos.data(), (uint32_t)pos.size(), counts.data(), keys.data(),
+                             driver.seed(), iteration, winners.data(), &st),
                 driver.context(), "mi_graph_run");
       } else {
-        driver.runGrouped(pos, counts, winners, &st);
+        driver.runKeyed(pos, counts, keys, iteration, winners, &st);
       }
       res.deviceMs += st.kernel_ms;
+      callHook(cfg, iteration, 0, pos, counts, keys, winners);
       scatterResults(all, leaves, first, winners, results);
     }
-    res.playoutMs += ms_since(t1);
+    res.playoutMs += msSince(t1);
     const auto t2 = Clock::now();
 #pragma omp parallel for schedule(dynamic, 4)
     for (long s = 0; s < (long)S; s++) {
@@ -355,10 +403,10 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
       trees[s]->update(results[s]);
       remaining[s] -= (uint32_t)leaves[s].size();
     }
-    res.updateMs += ms_since(t2);
+    res.updateMs += msSince(t2);
     res.playouts += iterPlayouts;
     res.iterations++;
-    res.iterationMs.push_back(std::chrono::duration<double, std::milli>(Clock::now() - t0).count());
+    res.iterationMs.push_back(msSince(t0));
     if (res.iterations > maxIter) throw std::runtime_error("multi-search did not converge");
   }
   res.wallSeconds = std::chrono::duration<double>(Clock::now() - wall0).count();

@@ -12,13 +12,32 @@
 //                own, taking turns so that one group's selection and update on the host overlap
 //                the other groups' replays,
 // and every search updates its tree with its own results.
+//
+// Streams: run r of search s's leaves in the search's iteration i is played with Philox key
+// (playoutSeed, s << 16 | r) and epoch i (mi_run_playouts_keyed, keyed graph replays), so a
+// search's playouts do not depend on which other searches share its launches: every mode grows
+// the same trees.  A search hands out at most 65,536 trials per iteration (the run index's 16
+// bits), and there are at most 65,536 searches.
 #pragma once
 
 #include <stdint.h>
 
+#include <functional>
 #include <vector>
 
 #include "gpuai/mi_playout.h"
+
+// One device batch of the searches, with its winners (handed to MultiSearchConfig::hook after
+// the winners arrive and before the trees are updated).
+struct MultiSearchBatch {
+  uint32_t iteration = 0;  // the searches' iteration, also the batch's epoch
+  uint32_t group = 0;      // pipelined mode: the group; per-search mode: the search; else 0
+  uint32_t seed = 0, policy = 0;
+  const std::vector<mi_pos> *pos = nullptr;
+  const std::vector<uint32_t> *counts = nullptr;
+  const std::vector<uint32_t> *keys = nullptr;
+  const std::vector<int8_t> *winners = nullptr;
+};
 
 enum class MultiSearchMode { GRAPH = 0, BATCHED = 1, PER_SEARCH = 2, PIPELINED = 3 };
 
@@ -37,7 +56,10 @@ struct MultiSearchConfig {
   // the timed search; the shape changes no result), else the library's default
   bool autotune = true;
   int device = 0;
+  std::function<void(const MultiSearchBatch &)> hook;  // test hook: every batch (may be empty)
 };
+
+constexpr uint32_t kMaxTrialsPerIteration = 65536;  // leaves per search per iteration
 
 struct MultiSearchResult {
   uint64_t playouts = 0;
@@ -52,6 +74,8 @@ struct MultiSearchResult {
   uint64_t rootDigest = 0;       // digest of every root's statistics (determinism checks)
   mi_tuning tuning = {};         // the launch shape the searches ran with
   std::vector<double> iterationMs;
+  std::vector<uint32_t> budgets;     // each search's playout budget
+  std::vector<uint32_t> rootTrials;  // each search's root trials at the end
 };
 
 // Seeded distinct start positions (1..30 random plies, as the configuration prescribes) and

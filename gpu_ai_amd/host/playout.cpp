@@ -32,6 +32,20 @@ void DeviceMI355XPlayoutDriver::runGrouped(const std::vector<mi_pos> &pos,
           ctx_, "mi_run_playouts");
 }
 
+void DeviceMI355XPlayoutDriver::runKeyed(const std::vector<mi_pos> &pos,
+                                         const std::vector<uint32_t> &counts,
+                                         const std::vector<uint32_t> &keys, uint32_t epoch,
+                                         std::vector<int8_t> &winners, mi_stats *stats) {
+  if (keys.size() != pos.size() || counts.size() != pos.size())
+    throw std::runtime_error("runKeyed: one count and one key per position");
+  uint64_t total = 0;
+  for (uint32_t c : counts) total += c;
+  winners.resize(total);
+  checkMi(mi_run_playouts_keyed(ctx_, pos.data(), (uint32_t)pos.size(), counts.data(), 0,
+                                keys.data(), seed_, epoch, policy_, winners.data(), stats),
+          ctx_, "mi_run_playouts_keyed");
+}
+
 std::vector<PlayerId> DeviceMI355XPlayoutDriver::runPlayouts(std::vector<State> states) {
   std::vector<PlayerId> results(states.size());
   if (states.empty()) return results;  // src/singlePlayout.cu:73-75

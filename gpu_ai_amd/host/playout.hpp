@@ -42,6 +42,11 @@ class DeviceMI355XPlayoutDriver : public PlayoutDriver {
   // position-major order.
   void runGrouped(const std::vector<mi_pos> &pos, const std::vector<uint32_t> &counts,
                   std::vector<int8_t> &winners, mi_stats *stats = nullptr);
+  // Grouped form with stream keys and an explicit epoch (mi_run_playouts_keyed): position i's
+  // playouts draw from Philox key (seed, keys[i]), whatever else shares the launch.
+  void runKeyed(const std::vector<mi_pos> &pos, const std::vector<uint32_t> &counts,
+                const std::vector<uint32_t> &keys, uint32_t epoch, std::vector<int8_t> &winners,
+                mi_stats *stats = nullptr);
   mi_ctx *context() const { return ctx_; }
   uint32_t nextEpoch() { return epoch_++; }
   uint32_t seed() const { return seed_; }

@@ -16,10 +16,37 @@ _lib = None
 MODES = {"graph": 0, "batched": 1, "per_search": 2, "pipelined": 3}
 
 
+class BatchRecord(ctypes.Structure):
+    _fields_ = [("iteration", _U32), ("group", _U32), ("seed", _U32), ("policy", _U32),
+                ("npos", _U32), ("pad", _U32), ("playouts", ctypes.c_uint64), ("pos", _P),
+                ("counts", _P), ("keys", _P), ("winners", _P)]
+
+
+BATCH_HOOK = ctypes.CFUNCTYPE(None, _P, ctypes.POINTER(BatchRecord))
+
+
+def batch_from_record(rec: BatchRecord) -> dict:
+    """Copies a batch record into numpy arrays (pos, counts and keys may be absent)."""
+    n, total = rec.npos, rec.playouts
+
+    def arr(ptr, dtype, count):
+        if not ptr or count == 0:
+            return None
+        buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+        return np.frombuffer(bytes(buf), dtype=dtype).copy()
+
+    return {"iteration": rec.iteration, "group": rec.group, "seed": rec.seed,
+            "policy": rec.policy, "pos": arr(rec.pos, POS_DTYPE, n),
+            "counts": arr(rec.counts, np.uint32, n), "keys": arr(rec.keys, np.uint32, n),
+            "winners": arr(rec.winners, np.int8, total)}
+
+
 class MultiSearchConfig(ctypes.Structure):
     _fields_ = [("searches", _U32), ("seed", _U32), ("playout_seed", _U32), ("min_budget", _U32),
                 ("max_budget", _U32), ("initial", _U32), ("scale", ctypes.c_double),
-                ("policy", _U32), ("mode", _U32), ("groups", _U32), ("autotune", _U32)]
+                ("policy", _U32), ("mode", _U32), ("groups", _U32), ("autotune", _U32),
+                ("hook", BATCH_HOOK), ("hook_user", _P), ("budgets", _P),
+                ("root_trials", _P)]
 
 
 class MultiSearchResult(ctypes.Structure):
@@ -67,6 +94,12 @@ def host_library() -> ctypes.CDLL:
                                               ctypes.POINTER(MultiSearchResult), ctypes.c_char_p,
                                               _U32]
         lib.gpuai_multisearch_run.restype = ctypes.c_int
+        lib.gpuai_driver_run_playouts.argtypes = [_P, _U32, _U32, _U32, _U32, _P,
+                                                  ctypes.c_char_p, _U32]
+        lib.gpuai_driver_run_playouts.restype = ctypes.c_int
+        lib.gpuai_search_record.argtypes = [_P, _U32, _U32, ctypes.c_double, _U32, _U32,
+                                            BATCH_HOOK, _P, ctypes.c_char_p, _U32]
+        lib.gpuai_search_record.restype = ctypes.c_int
         lib.gpuai_net_match_run.argtypes = [ctypes.POINTER(NetMatchConfig), _P,
                                             ctypes.POINTER(NetMatchResult), _P, ctypes.c_char_p,
                                             _U32]
@@ -112,16 +145,69 @@ def run_ai_path() -> str:
 def multisearch(searches: int = 256, seed: int = 2016, playout_seed: int = 12345,
                 min_budget: int = 1000, max_budget: int = 64000, initial: int = 256,
                 scale: float = 0.0, policy: int = 0, mode: str = "graph",
-                groups: int = 2, autotune: bool = True) -> dict:
+                groups: int = 2, autotune: bool = True, on_batch=None) -> dict:
     """Batched multi-search: concurrent searches sharing the device; returns the run's
-    statistics."""
+    statistics, each search's budget and its root's trials.  on_batch (optional) is called
+    with every device batch as a dict of arrays (batch_from_record) before the trees update."""
+    budgets = np.zeros(searches, dtype=np.uint32)
+    root_trials = np.zeros(searches, dtype=np.uint32)
+    failure = []
+
+    def hook(_user, rec):
+        try:
+            on_batch(batch_from_record(rec.contents))
+        except BaseException as e:  # reported after the search returns
+            failure.append(e)
+
+    cb = BATCH_HOOK(hook) if on_batch is not None else BATCH_HOOK()
     cfg = MultiSearchConfig(searches, seed, playout_seed, min_budget, max_budget, initial, scale,
-                            policy, MODES[mode], groups, int(autotune))
+                            policy, MODES[mode], groups, int(autotune), cb, None, _ptr(budgets),
+                            _ptr(root_trials))
     res = MultiSearchResult()
     err = ctypes.create_string_buffer(512)
     if host_library().gpuai_multisearch_run(ctypes.byref(cfg), ctypes.byref(res), err, 512) != 0:
         raise RuntimeError(err.value.decode())
-    return {k: getattr(res, k) for k, _ in MultiSearchResult._fields_ if k != "pad"}
+    if failure:
+        raise failure[0]
+    out = {k: getattr(res, k) for k, _ in MultiSearchResult._fields_ if k != "pad"}
+    out["budgets"] = budgets
+    out["root_trials"] = root_trials
+    return out
+
+
+def driver_run_playouts(states, policy: int = 0, seed: int = 12345, calls: int = 1) -> np.ndarray:
+    """DeviceMI355XPlayoutDriver::runPlayouts on `states`, `calls` times (call c draws epoch c);
+    returns winners [calls, len(states)]."""
+    st = np.ascontiguousarray(states, dtype=POS_DTYPE)
+    out = np.zeros((calls, len(st)), dtype=np.int8)
+    err = ctypes.create_string_buffer(512)
+    if host_library().gpuai_driver_run_playouts(_ptr(st), len(st), policy, seed, calls, _ptr(out),
+                                                err, 512) != 0:
+        raise RuntimeError(err.value.decode())
+    return out
+
+
+def search_record(root, iterations: int, initial: int = 256, scale: float = 0.0,
+                  policy: int = 0, seed: int = 12345) -> list[dict]:
+    """One GameTree search driven by DeviceMI355XPlayoutDriver::runPlayouts; returns each
+    round's leaves (in trial order) and the winners the driver gave them."""
+    p = np.ascontiguousarray(root, dtype=POS_DTYPE).reshape(1)
+    rounds, failure = [], []
+
+    def hook(_user, rec):
+        try:
+            rounds.append(batch_from_record(rec.contents))
+        except BaseException as e:
+            failure.append(e)
+
+    cb = BATCH_HOOK(hook)
+    err = ctypes.create_string_buffer(512)
+    if host_library().gpuai_search_record(_ptr(p), iterations, initial, scale, policy, seed, cb,
+                                          None, err, 512) != 0:
+        raise RuntimeError(err.value.decode())
+    if failure:
+        raise failure[0]
+    return rounds
 
 
 def net_match(params, games: int = 400, seed: int = 2016, opening_min: int = 0,

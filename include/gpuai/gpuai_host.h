@@ -36,7 +36,25 @@ int gpuai_host_apply(mi_pos *pos, const mi_move *move);
 int gpuai_mcts_trace(const mi_pos *root, uint32_t iterations, uint32_t initial, double scale,
                      uint32_t *trace, uint32_t trace_cap);
 
-/* Batched multi-search: concurrent MCTS searches sharing the device (multisearch.hpp). */
+/* One device batch of playouts as a search loop handed it over, with the winners that came
+ * back (test hook: the batch can be replayed on another implementation).  Run r of search s is
+ * keyed (s << 16 | r); counts[i] playouts of pos[i] with stream key keys[i] under `seed` and
+ * epoch `iteration`; winners position-major. */
+typedef struct gpuai_batch_record {
+  uint32_t iteration;  /* the searches' iteration = the batch's epoch */
+  uint32_t group;      /* pipelined mode: the group; per-search mode: the search; else 0 */
+  uint32_t seed, policy, npos, pad;
+  uint64_t playouts;
+  const mi_pos *pos;
+  const uint32_t *counts;
+  const uint32_t *keys;
+  const int8_t *winners;
+} gpuai_batch_record;
+typedef void (*gpuai_batch_hook)(void *user, const gpuai_batch_record *record);
+
+/* Batched multi-search: concurrent MCTS searches sharing the device (multisearch.hpp).  The
+ * playouts of run r of search s's leaves in the search's iteration i draw from Philox key
+ * (playout_seed, s << 16 | r) and epoch i, so every mode grows the same trees. */
 typedef struct gpuai_multisearch_config {
   uint32_t searches;      /* 256 */
   uint32_t seed;          /* start positions and budgets (2016) */
@@ -50,6 +68,10 @@ typedef struct gpuai_multisearch_config {
                              3 graph replays of `groups` groups overlapped with the host's work */
   uint32_t groups;        /* pipelined mode: groups of searches, one stream each (2) */
   uint32_t autotune;      /* 1: launch shape from mi_autotune before the search, 0: default */
+  gpuai_batch_hook hook;  /* called with every device batch (may be NULL) */
+  void *hook_user;
+  uint32_t *budgets;      /* out (may be NULL): each search's playout budget */
+  uint32_t *root_trials;  /* out (may be NULL): each search's root trials at the end */
 } gpuai_multisearch_config;
 
 typedef struct gpuai_multisearch_result {
@@ -65,6 +87,20 @@ typedef struct gpuai_multisearch_result {
 /* Runs the multi-search on device 0; returns 0, or -1 with a message in `err` (err_len bytes). */
 int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch_result *out,
                           char *err, uint32_t err_len);
+
+/* Test hooks of the drop-in driver (DeviceMI355XPlayoutDriver, playout.hpp).
+ * gpuai_driver_run_playouts: `calls` calls of runPlayouts (src/playout.hpp:31) on the same n
+ * states; winners[c * n + i] = result i of call c (call c draws epoch c).
+ * gpuai_search_record: one search as MCTSPlayer::worker runs it (src/player.cpp:134-150): a
+ * GameTree grown from `root` for `iterations` rounds of select (max(initial, totalTrials * scale)
+ * trials), runPlayouts and update; `hook` receives each round's leaves (states in trial order,
+ * ungrouped, in record->pos with npos = trials and counts = keys = NULL) and the winners the
+ * driver returned.  Both return 0, or -1 with a message in `err`. */
+int gpuai_driver_run_playouts(const mi_pos *states, uint32_t n, uint32_t policy, uint32_t seed,
+                              uint32_t calls, int8_t *winners, char *err, uint32_t err_len);
+int gpuai_search_record(const mi_pos *root, uint32_t iterations, uint32_t initial, double scale,
+                        uint32_t policy, uint32_t seed, gpuai_batch_hook hook, void *hook_user,
+                        char *err, uint32_t err_len);
 
 /* Match of value-network leaves against playout leaves (netmatch.hpp): two MCTS players that
  * differ only in how a leaf is scored, in lockstep games at equal wall time per move.  No

@@ -121,6 +121,16 @@ int mi_run_playouts(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_
                     uint32_t per_pos, uint32_t seed, uint32_t epoch, uint32_t policy,
                     int8_t *winners, mi_stats *stats);
 
+/* As mi_run_playouts, with stream keys: the playouts of position i draw from Philox key
+ * (seed, keys[i]) instead of (seed, i), so that a playout's stream depends on the caller's key
+ * rather than on where its position sits in the batch (batched searches key each leaf by its
+ * search and its place in that search's batch, so that any grouping of the searches into
+ * launches plays the same games).  No counterpart in the reference.  keys may be NULL
+ * (mi_run_playouts). */
+int mi_run_playouts_keyed(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
+                          uint32_t per_pos, const uint32_t *keys, uint32_t seed, uint32_t epoch,
+                          uint32_t policy, int8_t *winners, mi_stats *stats);
+
 /* As mi_run_playouts, and also returns per playout the number of plies and the hash chain of
  * every generated move list (verification mode; slower).  `plies` or `hashes` may be NULL. */
 int mi_run_playouts_traced(mi_ctx *ctx, const mi_pos *pos, uint32_t npos,
@@ -164,22 +174,23 @@ int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
  * copies and frees per call).  mi_graph_create fixes the capacities and builds one hipGraph for
  * the context's stream: a reset kernel, the persistent playout kernel with the context's tuning
  * (its grid sized to each batch), and the copy of the winners back to pinned memory.
- * mi_graph_run fills the pinned buffers, copies the batch (positions, playout offsets, seed,
- * epoch) to the device with an asynchronous copy on the stream, replays the graph and returns
- * the winners as mi_run_playouts would (same streams).  kernel_ms in the stats is the device
- * time of the upload and the replay.  A graph is
+ * mi_graph_run packs the batch (seed, epoch, positions, playout offsets and, when keys is not
+ * NULL, stream keys) into pinned memory, copies it to the device with one asynchronous copy on
+ * the stream, replays the graph and returns the winners as mi_run_playouts_keyed would (same
+ * streams).  kernel_ms in the stats is the device time of the upload and the replay.  A graph is
  * used by one thread at a time and must be destroyed before its context. */
 typedef struct mi_graph mi_graph;
 int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, uint32_t policy,
                     mi_graph **graph);
 int mi_graph_run(mi_graph *graph, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
-                 uint32_t seed, uint32_t epoch, int8_t *winners, mi_stats *stats);
+                 const uint32_t *keys, uint32_t seed, uint32_t epoch, int8_t *winners,
+                 mi_stats *stats);
 /* The two halves of mi_graph_run, so that the caller can work while the replay runs: launch
  * copies the batch into the graph's pinned buffer and enqueues the replay on the context's
  * stream; finish waits for it and returns its winners.  A graph holds one replay in flight;
  * graphs of different contexts run concurrently on their own streams. */
 int mi_graph_launch(mi_graph *graph, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
-                    uint32_t seed, uint32_t epoch);
+                    const uint32_t *keys, uint32_t seed, uint32_t epoch);
 int mi_graph_finish(mi_graph *graph, int8_t *winners, mi_stats *stats);
 int mi_graph_destroy(mi_graph *graph);
 

@@ -531,6 +531,13 @@ int or_playout(or_pos p, uint32_t seed, uint32_t pos_index, uint32_t playout, ui
 int or_run_playouts(const or_pos *pos, uint32_t npos, const uint32_t *counts, uint32_t per_pos,
                     uint32_t seed, uint32_t epoch, int policy, int8_t *winners, or_trace *traces,
                     int threads) {
+  return or_run_playouts_keyed(pos, npos, counts, per_pos, NULL, seed, epoch, policy, winners,
+                               traces, threads);
+}
+
+int or_run_playouts_keyed(const or_pos *pos, uint32_t npos, const uint32_t *counts,
+                          uint32_t per_pos, const uint32_t *keys, uint32_t seed, uint32_t epoch,
+                          int policy, int8_t *winners, or_trace *traces, int threads) {
   uint64_t *offs = (uint64_t *)malloc(sizeof(uint64_t) * (npos + 1));
   if (!offs) return -1;
   offs[0] = 0;
@@ -550,7 +557,7 @@ int or_run_playouts(const or_pos *pos, uint32_t npos, const uint32_t *counts, ui
     }
     uint32_t j = (uint32_t)((uint64_t)g - offs[lo]);
     or_trace t;
-    int w = or_playout(pos[lo], seed, lo, j, epoch, policy, traces ? &t : NULL);
+    int w = or_playout(pos[lo], seed, keys ? keys[lo] : lo, j, epoch, policy, traces ? &t : NULL);
     if (winners) winners[g] = (int8_t)w;
     if (traces) traces[g] = t;
   }

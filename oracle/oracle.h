@@ -104,6 +104,11 @@ int or_playout(or_pos p, uint32_t seed, uint32_t pos_index, uint32_t playout, ui
 int or_run_playouts(const or_pos *pos, uint32_t npos, const uint32_t *counts, uint32_t per_pos,
                     uint32_t seed, uint32_t epoch, int policy, int8_t *winners, or_trace *traces,
                     int threads);
+/* As or_run_playouts with stream keys: position i's playouts use Philox key (seed, keys[i])
+ * instead of (seed, i); keys == NULL is or_run_playouts. */
+int or_run_playouts_keyed(const or_pos *pos, uint32_t npos, const uint32_t *counts,
+                          uint32_t per_pos, const uint32_t *keys, uint32_t seed, uint32_t epoch,
+                          int policy, int8_t *winners, or_trace *traces, int threads);
 
 /* ---- start positions ----
  * k = kmin + (r mod (kmax-kmin+1)) random plies from the start position, stopping early at

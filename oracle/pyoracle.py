@@ -105,14 +105,25 @@ class _Lib:
 
     def run_playouts(self, pos, per_pos: int | None = None, counts=None, seed: int = 12345,
                      epoch: int = 0, policy: int = POLICY_RANDOM, threads: int = 0,
-                     traces: bool = True):
+                     traces: bool = True, keys=None):
         pos = np.ascontiguousarray(pos, dtype=POS_DTYPE)
         c = None if counts is None else np.ascontiguousarray(counts, dtype=np.uint32)
         total = int(c.sum()) if c is not None else len(pos) * int(per_pos)
         winners = np.empty(total, dtype=np.int8)
         tr = np.empty(total, dtype=TRACE_DTYPE) if traces else None
-        rc = self._run(_ptr(pos), len(pos), _ptr(c), 0 if per_pos is None else int(per_pos), seed,
-                       epoch, policy, _ptr(winners), _ptr(tr), threads)
+        pp = 0 if per_pos is None else int(per_pos)
+        if keys is not None:  # stream keys (restatement only)
+            k = np.ascontiguousarray(keys, dtype=np.uint32)
+            if k.shape != (len(pos),):
+                raise ValueError("one key per position")
+            fn = self.lib.or_run_playouts_keyed
+            fn.argtypes = [_P, _U32, _P, _U32, _P, _U32, _U32, ctypes.c_int, _P, _P, ctypes.c_int]
+            fn.restype = ctypes.c_int
+            rc = fn(_ptr(pos), len(pos), _ptr(c), pp, _ptr(k), seed, epoch, policy, _ptr(winners),
+                    _ptr(tr), threads)
+        else:
+            rc = self._run(_ptr(pos), len(pos), _ptr(c), pp, seed, epoch, policy, _ptr(winners),
+                           _ptr(tr), threads)
         if rc != 0:
             raise RuntimeError(f"{self.prefix}_run_playouts failed")
         return winners, tr

@@ -26,6 +26,7 @@ def test_multisearch_graph_equals_plain_batched():
     announce("multi-search per-search launches")
     p = host_api.multisearch(mode="per_search", **kw)
     assert p["playouts"] == g["playouts"]
+    assert p["root_digest"] == g["root_digest"]  # keyed streams: the same trees
 
 
 def test_multisearch_pipelined_is_deterministic():
@@ -35,7 +36,7 @@ def test_multisearch_pipelined_is_deterministic():
     a = host_api.multisearch(mode="pipelined", **kw)
     b = host_api.multisearch(mode="pipelined", **kw)
     g = host_api.multisearch(mode="graph", **kw)
-    assert a["root_digest"] == b["root_digest"]
+    assert a["root_digest"] == b["root_digest"] == g["root_digest"]
     assert a["playouts"] == b["playouts"] == g["playouts"] > 0  # every budget is spent
 
 
@@ -138,3 +139,127 @@ def test_net_match_small():
         assert r["score_ms"][0] > 0.0 and r["score_ms"][1] > 0.0
     with pytest.raises(RuntimeError):
         host_api.net_match(params, games=2, move_ms=5.0, max_plies=10, net_exploration=0.0)
+
+
+# ---------------------------------------------------------------------------------------------
+# stream keys and the searches' batches against the oracle
+# ---------------------------------------------------------------------------------------------
+
+def _group_runs(states):
+    """Runs of equal consecutive states -> (positions, counts), as runPlayouts groups them."""
+    keyf = lambda p: (int(p["p1"]), int(p["p2"]), int(p["kings"]), int(p["turn"]),
+                      int(p["counter"]))
+    pos, counts = [], []
+    for i, p in enumerate(states):
+        if i > 0 and keyf(p) == keyf(states[i - 1]):
+            counts[-1] += 1
+        else:
+            pos.append(p)
+            counts.append(1)
+    return np.array(pos, dtype=states.dtype), np.array(counts, dtype=np.uint32)
+
+
+def test_keyed_playouts_match_oracle_and_ignore_batch_layout(engine, oracle, golden):
+    """mi_run_playouts_keyed and keyed graph replays against the oracle; a keyed position plays
+    the same games whatever shares its batch."""
+    pos = golden["positions"]["mid1024"][:400]
+    rng = np.random.default_rng(12)
+    counts = rng.integers(0, 60, len(pos)).astype(np.uint32)
+    keys = rng.integers(0, 2**32, len(pos), dtype=np.uint64).astype(np.uint32)
+    for policy in (0, 1):
+        announce(f"keyed playouts, policy {policy}")
+        got = engine.run_playouts(pos, counts=counts, seed=71, epoch=9, policy=policy, keys=keys)
+        w, _ = oracle.run_playouts(pos, counts=counts, seed=71, epoch=9, policy=policy,
+                                   traces=False, keys=keys)
+        assert np.array_equal(got.winners, w)
+        # positions 100..199 alone, with their keys: the same winners as inside the batch
+        offs = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
+        part = engine.run_playouts(pos[100:200], counts=counts[100:200], seed=71, epoch=9,
+                                   policy=policy, keys=keys[100:200])
+        assert np.array_equal(part.winners, got.winners[offs[100]:offs[200]])
+        with engine.graph(len(pos), int(counts.sum()), policy=policy) as g:
+            rep = g.run(pos, counts, seed=71, epoch=9, keys=keys)
+            assert np.array_equal(rep.winners, w)
+            plain = g.run(pos, counts, seed=71, epoch=9)  # unkeyed replay: key = index
+            eager = engine.run_playouts(pos, counts=counts, seed=71, epoch=9, policy=policy)
+            assert np.array_equal(plain.winners, eager.winners)
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_driver_run_playouts_matches_oracle(golden, oracle, policy):
+    """DeviceMI355XPlayoutDriver::runPlayouts on a leaf vector with repeated consecutive states
+    (including finished games, as terminal tree nodes hand out copies): each call's winners equal
+    the oracle's on the grouped batch with the call's epoch."""
+    from gpu_ai_amd import host_api
+    mid = golden["positions"]["mid1024"][:60]
+    odd = golden["movelists"]["positions"]
+    finished = odd[odd["counter"] >= 50][:5]
+    rng = np.random.default_rng(30 + policy)
+    pool = np.concatenate([mid, finished])
+    picks = rng.integers(0, len(pool), 150)
+    reps = rng.integers(1, 6, 150)
+    states = np.repeat(pool[picks], reps)  # runs of 1-5 equal states, equal runs may recur
+    announce(f"driver runPlayouts on {len(states)} states, policy {policy}")
+    got = host_api.driver_run_playouts(states, policy=policy, seed=4321, calls=2)
+    gpos, gcounts = _group_runs(states)
+    for c in range(2):
+        w, _ = oracle.run_playouts(gpos, counts=gcounts, seed=4321, epoch=c, policy=policy,
+                                   traces=False)
+        assert np.array_equal(got[c], w), c
+
+
+def test_search_leaves_match_oracle(golden, oracle):
+    """A GameTree search driven by runPlayouts (MCTSPlayer::worker's loop): every round's leaves,
+    grouped as the driver groups them, get the oracle's winners.  The second root is two plies
+    from the draw limit, so its tree reaches finished games and hands out repeated states."""
+    from gpu_ai_amd import host_api
+    odd = golden["movelists"]["positions"]
+    near_draw = next(p for p in odd[odd["counter"] == 48] if not oracle.is_game_over(p))
+    roots = [golden["positions"]["mid1024"][7], near_draw]
+    repeats = 0
+    for k, root in enumerate(roots):
+        announce(f"search rounds from root {k}")
+        rounds = host_api.search_record(root, iterations=6, initial=300, scale=0.0, policy=0,
+                                        seed=777)
+        assert len(rounds) == 6
+        for r in rounds:
+            gpos, gcounts = _group_runs(r["pos"])
+            repeats += int((gcounts > 1).sum())
+            w, _ = oracle.run_playouts(gpos, counts=gcounts, seed=777, epoch=r["iteration"],
+                                       policy=0, traces=False)
+            assert np.array_equal(r["winners"], w), (k, r["iteration"])
+    assert repeats > 0  # the grouping of repeated leaves was exercised
+
+
+def test_multisearch_256_matches_oracle_and_modes_agree(oracle):
+    """The batched multi-search at its configured size: 256 searches from seeded distinct
+    positions, budgets uniform in [1000, 64000], graph replays of the persistent refill kernel.
+    Every batch's winners equal the oracle's on the same positions, counts, keys and epoch; every
+    budget is spent exactly; batched, pipelined and per-search launches grow the same trees."""
+    from gpu_ai_amd import host_api
+    stats = {"batches": 0, "playouts": 0, "bad": 0}
+    per_search = np.zeros(256, dtype=np.int64)
+
+    def check(b):
+        w, _ = oracle.run_playouts(b["pos"], counts=b["counts"], seed=b["seed"],
+                                   epoch=b["iteration"], policy=b["policy"], traces=False,
+                                   keys=b["keys"], threads=16)
+        stats["bad"] += int(np.sum(w != b["winners"]))
+        stats["batches"] += 1
+        stats["playouts"] += len(w)
+        np.add.at(per_search, (b["keys"] >> 16).astype(np.int64), b["counts"].astype(np.int64))
+
+    announce("multisearch-256 graph mode, every batch against the oracle")
+    g = host_api.multisearch(mode="graph", on_batch=check)
+    announce(f"{stats['batches']} batches, {stats['playouts']} playouts checked")
+    assert stats["bad"] == 0
+    assert stats["playouts"] == g["playouts"] == int(g["budgets"].sum())
+    assert np.all(g["budgets"] >= 1000) and np.all(g["budgets"] <= 64000)
+    assert np.array_equal(per_search, g["budgets"].astype(np.int64))
+    assert np.array_equal(g["root_trials"], g["budgets"])
+    for mode in ("batched", "pipelined", "per_search"):
+        announce(f"multisearch-256 {mode} mode")
+        r = host_api.multisearch(mode=mode, autotune=False)
+        assert r["playouts"] == g["playouts"], mode
+        assert np.array_equal(r["root_trials"], g["budgets"]), mode
+        assert r["root_digest"] == g["root_digest"], mode

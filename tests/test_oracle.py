@@ -292,3 +292,24 @@ def test_oracle_full_configs_match_reference_golden(oracle, golden, name):
     assert np.array_equal(w, expect["winners"])
     plies = tr["plies"].reshape(1024, 1024).sum(axis=1, dtype=np.uint64)
     assert np.array_equal(plies, expect["plies"])
+
+
+def test_oracle_stream_keys(oracle, golden):
+    """Keyed playouts: keys equal to the position indices are the unkeyed playouts; a keyed
+    position plays the same games alone as inside a batch."""
+    pos = golden["positions"]["open1024"][:64]
+    counts = (np.arange(64, dtype=np.uint32) * 5) % 17
+    plain, _ = oracle.run_playouts(pos, counts=counts, seed=3, epoch=2, traces=False)
+    same, _ = oracle.run_playouts(pos, counts=counts, seed=3, epoch=2, traces=False,
+                                  keys=np.arange(64, dtype=np.uint32))
+    assert np.array_equal(plain, same)
+    keys = (np.arange(64, dtype=np.uint32) * 2654435761) & 0xFFFFFFFF
+    allk, _ = oracle.run_playouts(pos, counts=counts, seed=3, epoch=2, traces=False, keys=keys)
+    offs = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
+    for i in (5, 40):
+        one, _ = oracle.run_playouts(pos[i:i + 1], counts=counts[i:i + 1], seed=3, epoch=2,
+                                     traces=False, keys=keys[i:i + 1])
+        assert np.array_equal(one, allk[offs[i]:offs[i + 1]])
+    # a different key is a different stream
+    other, _ = oracle.run_playouts(pos, counts=counts, seed=3, epoch=2, traces=False, keys=keys + 1)
+    assert not np.array_equal(other, allk)
