@@ -251,23 +251,50 @@ CK_HD uint32_t slot_dir(const Lane &L, const Gen &g, uint32_t sq, uint32_t sl) {
   return (g.capture && !king && sl < 2u) ? (sl ^ 1u) : sl;
 }
 
-// k-th move (0-based) of the slot masks in square-major, slot-minor order.
+// Byte permutation (v_perm_b32): byte i of the result is byte sel_i of the 8-byte value
+// {hi:lo} (0-3 from lo, 4-7 from hi), or 0 for a selector of 12.
+CK_HD uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+  const uint64_t v = ((uint64_t)hi << 32) | lo;
+  uint32_t r = 0;
+  for (int i = 0; i < 4; i++) {
+    const uint32_t b = (sel >> (8 * i)) & 0xFFu;
+    const uint32_t byte = b < 8u ? (uint32_t)(v >> (8u * b)) & 0xFFu : 0u;
+    r |= byte << (8 * i);
+  }
+  return r;
+#endif
+}
+
+// k-th move (0-based) of the slot masks in square-major, slot-minor order.  The four masks are
+// transposed by bytes, so that word x_b holds squares 8b..8b+7 of slot t in its byte t; the
+// group of eight squares holding the k-th move is found from the four words' popcounts, and the
+// square inside it by a 3-step search that counts the group's moves below a square with one
+// popcount (the square's low-bit mask copied into every byte).
 CK_HD void select_slot(const uint32_t s[4], uint32_t k, uint32_t &sq, uint32_t &sl) {
+  const uint32_t a_lo = perm(s[1], s[0], 0x05010400u), a_hi = perm(s[1], s[0], 0x07030602u);
+  const uint32_t b_lo = perm(s[3], s[2], 0x05010400u), b_hi = perm(s[3], s[2], 0x07030602u);
+  const uint32_t x0 = perm(b_lo, a_lo, 0x05040100u), x1 = perm(b_lo, a_lo, 0x07060302u);
+  const uint32_t x2 = perm(b_hi, a_hi, 0x05040100u), x3 = perm(b_hi, a_hi, 0x07060302u);
+  const uint32_t p1 = popc(x0), p2 = p1 + popc(x1), p3 = p2 + popc(x2);
+  const bool g1 = k >= p1, g2 = k >= p2, g3 = k >= p3;
+  const uint32_t x = g3 ? x3 : g2 ? x2 : g1 ? x1 : x0;
+  const uint32_t base = g3 ? p3 : g2 ? p2 : g1 ? p1 : 0u;
+  const uint32_t kk = k - base;
+  auto below = [x](uint32_t j) { return popc(x & perm(0u, bit(j) - 1u, 0u)); };
   uint32_t lo = 0;
 #pragma unroll
-  for (uint32_t stride = 16; stride >= 1; stride >>= 1) {
-    uint32_t below = bit(lo + stride) - 1u;
-    uint32_t c = popc(s[0] & below) + popc(s[1] & below) + popc(s[2] & below) + popc(s[3] & below);
-    if (c <= k) lo += stride;
-  }
-  uint32_t below = bit(lo) - 1u;
-  uint32_t j = k - (popc(s[0] & below) + popc(s[1] & below) + popc(s[2] & below) + popc(s[3] & below));
-  uint32_t here = getbit(s[0], lo) | (getbit(s[1], lo) << 1) | (getbit(s[2], lo) << 2) |
-                  (getbit(s[3], lo) << 3);
+  for (uint32_t stride = 4; stride >= 1; stride >>= 1)
+    if (below(lo + stride) <= kk) lo += stride;
+  uint32_t j = kk - below(lo);
+  const uint32_t y = (x >> lo) & 0x01010101u;
+  uint32_t here = (y | (y >> 7) | (y >> 14) | (y >> 21)) & 15u;
   if (j >= 1u) here &= here - 1u;
   if (j >= 2u) here &= here - 1u;
   if (j >= 3u) here &= here - 1u;
-  sq = lo;
+  sq = 8u * (uint32_t)(g1 + g2 + g3) + lo;
   sl = ctz(here);
 }
 
@@ -488,18 +515,17 @@ CK_HD Choice select_walk(const Lane &L, const Gen &g, uint32_t k) {
   return c;
 }
 
-// Move of slot choice (square sq, slot sl).
+// Move of slot choice (square sq, slot sl).  A step in direction d from square x moves
+// base_d - (x's row parity) squares, base = {5, 4, -3, -4}; a jump is two steps from rows of
+// opposite parity, 2 base_d - 1 squares, over the square one step away.
 CK_HD Choice slot_choice(const Lane &L, const Gen &g, uint32_t sq, uint32_t sl) {
+  const uint32_t d = slot_dir(L, g, sq, sl);
+  const int base = (int)((int32_t)(0xFCFD0405u << (24u - 8u * d)) >> 24);
+  const int step = base - (int)((sq >> 2) & 1u);
   Choice c;
-  uint32_t d = slot_dir(L, g, sq, sl);
   c.from = sq;
-  if (g.capture) {
-    c.to = (uint32_t)((int)sq + jump_off(d));
-    c.removed = bit(mid_sq(sq, c.to));
-  } else {
-    c.to = step_to(sq, d);
-    c.removed = 0u;
-  }
+  c.to = (uint32_t)((int)sq + (g.capture ? 2 * base - 1 : step));
+  c.removed = g.capture ? bit((uint32_t)((int)sq + step)) : 0u;
   return c;
 }
 
@@ -531,8 +557,8 @@ CK_HD Choice select_in_walk(const Lane &L, const Gen &g, uint32_t s, uint32_t k)
   return c;
 }
 
-// The first four capture sequences of the multi-jump origins (in walk order): landing square
-// and removed pieces, held in registers (fixed slots, selected without dynamic indexing) so that
+// The first four capture sequences of the multi-jump kings (in walk order): landing square and
+// removed pieces, held in registers (fixed slots, selected without dynamic indexing) so that
 // choosing one of them needs no second walk.
 struct LeafCache {
   uint32_t to0, to1, to2, to3;
@@ -560,25 +586,226 @@ CK_HD void cache_get(const LeafCache &c, uint32_t i, uint32_t &to, uint32_t &rm)
   rm = (c.rm0 & m0) | (c.rm1 & m1) | (c.rm2 & m2) | (c.rm3 & m3);
 }
 
+// ------------------------------------------------------------------------------------------
+// a man's capture tree in closed form
+// ------------------------------------------------------------------------------------------
+// A man only jumps forward (+7 first in the reference's order, then +9, src/state.cu:292-302),
+// and every jump's validity is read on the unmodified board, so whether a jump from a square is
+// valid does not depend on the path that reached it; with two rows gained per jump a sequence
+// has at most three jumps.  The leaves of the tree therefore fall on eight fixed slots, one per
+// direction triple (d1, d2, d3) (d = 0 for +7, 1 for +9), slot 4*d1 + 2*d2 + d3; a sequence that
+// ends after one or two jumps takes the first slot of its subtree (trailing directions 0).  The
+// DFS order of the leaves is the slot order, so the tree is an 8-bit mask of present slots.
+// Bit r of `a` / `b` (the cap masks shifted right by the origin) says whether a +7 / +9 jump is
+// valid from the square r above the origin; squares past the board read as 0.
+CK_HD uint32_t man_tree(const Gen &g, uint32_t o) {
+  const uint32_t a = g.cap[1] >> o, b = g.cap[0] >> o, h = a | b;
+  auto at = [](uint32_t m, uint32_t r) { return (m >> r) & 1u; };
+  const uint32_t a0 = at(a, 0), b0 = at(b, 0);
+  const uint32_t h7 = at(h, 7), h9 = at(h, 9), h14 = at(h, 14), h16 = at(h, 16), h18 = at(h, 18);
+  // subtree of the landing square 7 (first jump +7) and of 9 (+9); a landing square with no
+  // further jump is a leaf in the subtree's first slot
+  const uint32_t t7 = a0 & h7, t9 = b0 & h9;
+  const uint32_t s00 = t7 & at(a, 7), s01 = t7 & at(b, 7);  // 7 -> 14, 7 -> 16
+  const uint32_t s10 = t9 & at(a, 9), s11 = t9 & at(b, 9);  // 9 -> 16, 9 -> 18
+  uint32_t p = 0;
+  p |= ((a0 & (h7 ^ 1u)) | (s00 & ((h14 ^ 1u) | at(a, 14)))) << 0;
+  p |= (s00 & h14 & at(b, 14)) << 1;
+  p |= (s01 & ((h16 ^ 1u) | at(a, 16))) << 2;
+  p |= (s01 & h16 & at(b, 16)) << 3;
+  p |= ((b0 & (h9 ^ 1u)) | (s10 & ((h16 ^ 1u) | at(a, 16)))) << 4;
+  p |= (s10 & h16 & at(b, 16)) << 5;
+  p |= (s11 & ((h18 ^ 1u) | at(a, 18))) << 6;
+  p |= (s11 & h18 & at(b, 18)) << 7;
+  return p;
+}
+
+// Leaf k (0-based, DFS order) of the tree man_tree gave for origin o: the k-th set bit of p
+// names the slot, and the tree's branching says where the sequence stops.
+CK_HD Choice man_leaf(const Gen &g, uint32_t o, uint32_t p, uint32_t k) {
+  // k-th set bit of an 8-bit mask
+  uint32_t slot = 0;
+  const uint32_t c4 = popc(p & 15u);
+  if (k >= c4) {
+    k -= c4;
+    slot = 4;
+  }
+  const uint32_t c2 = popc((p >> slot) & 3u);
+  if (k >= c2) {
+    k -= c2;
+    slot += 2;
+  }
+  const uint32_t c1 = (p >> slot) & 1u;
+  if (k >= c1) slot += 1;
+  const uint32_t d1 = slot >> 2, d2 = (slot >> 1) & 1u, d3 = slot & 1u;
+  const uint32_t h = g.cap[0] | g.cap[1];
+  Choice c;
+  c.from = o;
+  uint32_t x = o + 7u + 2u * d1;
+  uint32_t removed = bit(mid_sq(o, x));
+  if (getbit(h, x)) {  // a second jump (a landing square with a jump never ends a sequence)
+    const uint32_t y = x + 7u + 2u * d2;
+    removed |= bit(mid_sq(x, y));
+    x = y;
+    if (getbit(h, x)) {
+      const uint32_t z = x + 7u + 2u * d3;
+      removed |= bit(mid_sq(x, z));
+      x = z;
+    }
+  }
+  c.to = x;
+  c.removed = removed;
+  return c;
+}
+
+// ------------------------------------------------------------------------------------------
+// a king's capture tree to depth two in closed form
+// ------------------------------------------------------------------------------------------
+// A king jumps in directions 0..3 (+9, +7, -7, -9; src/state.cu:411-417) and may not land on a
+// square it landed on before (src/state.cu:134-139).  Up to two jumps that rule never binds: the
+// only way back is the straight jump onto the origin, which is occupied.  The tree to depth two
+// therefore falls on sixteen slots 4*d1 + d2 (a sequence of one jump takes slot 4*d1), in DFS
+// order.  `deep` is set when some second landing square allows a third jump other than the one
+// straight back (which the rule forbids); such trees are left to the walk.
+CK_HD uint32_t king_tree(const Gen &g, uint32_t o, uint32_t &deep) {
+  const uint32_t any = g.cap[0] | g.cap[1] | g.cap[2] | g.cap[3];
+  // third jumps from a second landing square reached in direction d (not straight back), seen
+  // from the first landing square: squares x with a valid jump in d to a square that continues
+  uint32_t d3 = 0;
+#pragma unroll
+  for (uint32_t d = 0; d < 4u; d++) {
+    // a jump in any direction but 3 - d, the one straight back
+    const uint32_t cont = (d == 3u ? 0u : g.cap[0]) | (d == 2u ? 0u : g.cap[1]) |
+                          (d == 1u ? 0u : g.cap[2]) | (d == 0u ? 0u : g.cap[3]);
+    const int off = jump_off(d);
+    d3 |= g.cap[d] & (off > 0 ? cont >> off : cont << -off);
+  }
+  // windows around the origin: bit 32 + r of w(m) is bit o + r of m (0 off the board)
+  auto w = [o](uint32_t m) { return ((uint64_t)m << 32) >> o; };
+  auto at = [](uint64_t x, int r) { return (uint32_t)(x >> (32 + r)) & 1u; };
+  const uint64_t w0 = w(g.cap[0]), w1 = w(g.cap[1]), w2 = w(g.cap[2]), w3 = w(g.cap[3]);
+  const uint64_t wa = w(any), wd = w(d3);
+  uint32_t t = 0, dp = 0;
+#pragma unroll
+  for (uint32_t d1 = 0; d1 < 4u; d1++) {
+    const int r1 = jump_off(d1);
+    const uint32_t v1 = at(d1 == 0 ? w0 : d1 == 1 ? w1 : d1 == 2 ? w2 : w3, 0);
+    const uint32_t c1 = at(wa, r1);  // the jump straight back reads as invalid: origin occupied
+    t |= (v1 & (c1 ^ 1u)) << (4u * d1);
+    const uint32_t go = v1 & c1;
+    t |= ((go & at(w0, r1)) | ((go & at(w1, r1)) << 1) | ((go & at(w2, r1)) << 2) |
+          ((go & at(w3, r1)) << 3)) << (4u * d1);
+    dp |= go & at(wd, r1);
+  }
+  deep = dp;
+  return t;
+}
+
+// Leaf k (0-based, DFS order) of the tree king_tree gave for origin o.
+CK_HD Choice king_leaf(const Gen &g, uint32_t o, uint32_t t, uint32_t k) {
+  uint32_t slot = 0;
+  const uint32_t c8 = popc(t & 255u);
+  if (k >= c8) {
+    k -= c8;
+    slot = 8;
+  }
+  const uint32_t c4 = popc((t >> slot) & 15u);
+  if (k >= c4) {
+    k -= c4;
+    slot += 4;
+  }
+  const uint32_t c2 = popc((t >> slot) & 3u);
+  if (k >= c2) {
+    k -= c2;
+    slot += 2;
+  }
+  if (k >= ((t >> slot) & 1u)) slot += 1;
+  const uint32_t any = g.cap[0] | g.cap[1] | g.cap[2] | g.cap[3];
+  Choice c;
+  c.from = o;
+  const uint32_t x = (uint32_t)((int)o + jump_off(slot >> 2));
+  uint32_t removed = bit(mid_sq(o, x));
+  c.to = x;
+  if (getbit(any, x)) {
+    const uint32_t y = (uint32_t)((int)x + jump_off(slot & 3u));
+    removed |= bit(mid_sq(x, y));
+    c.to = y;
+  }
+  c.removed = removed;
+  return c;
+}
+
+// A capture ply's moves as capture trees, origin by origin in square order: men's from
+// man_tree, kings' from king_tree.  Up to four origins' masks are kept (16 bits each, two per
+// word); `deep` is set when a king's tree goes past two jumps or more than four origins can
+// capture (then the walk decides the ply).  Returns the move count.
+CK_HD uint32_t tree_count(const Lane &L, const Gen &g, uint32_t &m01, uint32_t &m23,
+                          uint32_t &deep) {
+  uint32_t o = g.origins, n = 0, i = 0;
+  m01 = 0;
+  m23 = 0;
+  deep = popc(o) > 4u;
+  while (o && !deep) {
+    const uint32_t s = ctz(o);
+    o &= o - 1u;
+    uint32_t dp = 0;
+    const uint32_t t = getbit(L.kings, s) ? king_tree(g, s, dp) : man_tree(g, s);
+    deep |= dp;
+    if (i < 2u) m01 |= t << (16u * i);
+    else m23 |= t << (16u * (i - 2u));
+    n += popc(t);
+    i++;
+  }
+  return n;
+}
+
+// k-th move (mover-frame order) from the masks of tree_count.
+CK_HD Choice tree_select(const Lane &L, const Gen &g, uint32_t k, uint32_t m01, uint32_t m23) {
+  uint32_t o = g.origins, i = 0;
+  Choice c = {0u, 0u, 0u};
+  while (o) {
+    const uint32_t s = ctz(o);
+    o &= o - 1u;
+    const uint32_t t = ((i < 2u ? m01 : m23) >> (16u * (i & 1u))) & 0xFFFFu;
+    const uint32_t n = popc(t);
+    if (k < n) {
+      c = getbit(L.kings, s) ? king_leaf(g, s, t, k) : man_leaf(g, s, t, k);
+      break;
+    }
+    k -= n;
+    i++;
+  }
+  return c;
+}
+
 // Move count of a capture ply with multi-jump origins: single-jump origins contribute their slot
-// bits, each multi-jump origin its walked leaf count.  The first four such counts are returned
-// packed one per byte (in square order) for the selection pass, the first four leaves in `lc`.
-CK_HD uint32_t count_mixed(const Lane &L, const Gen &g, uint32_t &packed, LeafCache &lc) {
+// bits, each multi-jump origin its tree's leaf count (men: man_tree; kings: the walk).  The first
+// four such counts are returned packed one per byte (in square order) for the selection pass,
+// with the men's tree masks likewise in `trees` and the first four king leaves in `lc`.
+CK_HD uint32_t count_mixed(const Lane &L, const Gen &g, uint32_t &packed, uint32_t &trees,
+                           LeafCache &lc) {
   uint32_t n = popc(g.slot[0] & ~g.multi) + popc(g.slot[1] & ~g.multi) +
                popc(g.slot[2] & ~g.multi) + popc(g.slot[3] & ~g.multi);
   uint32_t o = g.multi, i = 0, m = 0;
   packed = 0;
+  trees = 0;
   while (o) {
     const uint32_t s = ctz(o);
     o &= o - 1u;
     uint32_t c = 0;
-    auto leaf = [&](uint32_t to, uint32_t removed, uint32_t, uint32_t) {
-      if (m < 4u) cache_put(lc, m, to, removed);
-      m++;
-      c++;
-      return false;
-    };
-    walk(s, getbit(L.kings, s), g.cap, leaf);
+    if (getbit(L.kings, s)) {
+      auto leaf = [&](uint32_t to, uint32_t removed, uint32_t, uint32_t) {
+        if (m < 4u) cache_put(lc, m, to, removed);
+        m++;
+        c++;
+        return false;
+      };
+      walk(s, 1u, g.cap, leaf);
+    } else {
+      const uint32_t t = man_tree(g, s);
+      c = popc(t);
+      if (i < 4u) trees |= t << (8u * i);
+    }
     if (i < 4u) packed |= (c < 255u ? c : 255u) << (8u * i);
     i++;
     n += c;
@@ -588,10 +815,10 @@ CK_HD uint32_t count_mixed(const Lane &L, const Gen &g, uint32_t &packed, LeafCa
 
 // k-th move (mover-frame order) of a capture ply with multi-jump origins.
 CK_HD Choice select_mixed(const Lane &L, const Gen &g, uint32_t k, uint32_t packed,
-                          const LeafCache &lc) {
+                          uint32_t trees, const LeafCache &lc) {
   uint32_t sm[4] = {g.slot[0] & ~g.multi, g.slot[1] & ~g.multi, g.slot[2] & ~g.multi,
                     g.slot[3] & ~g.multi};
-  uint32_t o = g.multi, acc = 0, i = 0;
+  uint32_t o = g.multi, acc = 0, kacc = 0, i = 0;
   while (o) {
     const uint32_t s = ctz(o);
     o &= o - 1u;
@@ -599,9 +826,13 @@ CK_HD Choice select_mixed(const Lane &L, const Gen &g, uint32_t k, uint32_t pack
     const uint32_t start = acc + popc(sm[0] & below) + popc(sm[1] & below) +
                            popc(sm[2] & below) + popc(sm[3] & below);
     if (k < start) break;
-    const uint32_t c = i < 4u ? (packed >> (8u * i)) & 255u : count_walk(L, g, s);
+    const uint32_t king = getbit(L.kings, s);
+    const uint32_t t = king ? 0u : (i < 4u ? (trees >> (8u * i)) & 255u : man_tree(g, s));
+    const uint32_t c = i < 4u ? (packed >> (8u * i)) & 255u
+                              : (king ? count_walk(L, g, s) : popc(t));
     if (k < start + c) {
-      const uint32_t ci = acc + (k - start);  // index among all multi-jump leaves
+      if (!king) return man_leaf(g, s, t, k - start);
+      const uint32_t ci = kacc + (k - start);  // index among all king leaves
       if (ci < 4u) {
         Choice ch;
         ch.from = s;
@@ -611,6 +842,7 @@ CK_HD Choice select_mixed(const Lane &L, const Gen &g, uint32_t k, uint32_t pack
       return select_in_walk(L, g, s, k - start);
     }
     acc += c;
+    kacc += king ? c : 0u;
     i++;
   }
   uint32_t sq, sl;
@@ -638,9 +870,9 @@ struct PlyResult {
 template <bool TRACE>
 CK_HD PlyResult ply_random_given(Lane &L, const Gen &g, uint32_t r, uint32_t *list_dig) {
   PlyResult res;
-  uint32_t packed = 0;
+  uint32_t packed = 0, trees = 0;
   LeafCache lc = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-  uint32_t n = g.multi ? count_mixed(L, g, packed, lc) : slot_count(g.slot);
+  uint32_t n = g.multi ? count_mixed(L, g, packed, trees, lc) : slot_count(g.slot);
   res.n = n;
   if (TRACE) *list_dig = list_digest(L, g, n);
   if (n == 0u || L.counter >= 50u) {  // src/state.cpp:16-23
@@ -656,12 +888,41 @@ CK_HD PlyResult ply_random_given(Lane &L, const Gen &g, uint32_t r, uint32_t *li
   uint32_t km = L.turn ? n - 1u - k : k;
   Choice c;
   if (g.multi) {
-    c = select_mixed(L, g, km, packed, lc);
+    c = select_mixed(L, g, km, packed, trees, lc);
   } else {
     uint32_t sq, sl;
     select_slot(g.slot, km, sq, sl);
     c = slot_choice(L, g, sq, sl);
   }
+  apply(L, c.from, c.to, c.removed);
+  return res;
+}
+
+// The deferred random ply (a capture ply with a multi-jump origin) from capture trees in
+// closed form, with word r and the fast ply's analysis g.  Sets `deep` and leaves the position
+// untouched when the trees cannot decide it (ply_random_given then plays it by the walk).
+template <bool TRACE>
+CK_HD PlyResult ply_random_trees(Lane &L, const Gen &g, uint32_t r, uint32_t *list_dig,
+                                 bool &deep) {
+  PlyResult res;
+  uint32_t m01, m23, dp;
+  const uint32_t n = tree_count(L, g, m01, m23, dp);
+  deep = dp != 0u;
+  res.n = n;
+  res.chosen = 0u;
+  res.over = 0u;
+  res.winner = 0;
+  if (deep) return res;
+  if (TRACE) *list_dig = list_digest(L, g, n);
+  if (L.counter >= 50u) {  // src/state.cpp:16-23 (a capture ply always has moves)
+    res.over = 1u;
+    res.chosen = 0xFFFFu;
+    res.winner = -1;
+    return res;
+  }
+  const uint32_t k = mod_u32(r, n);
+  res.chosen = k;
+  const Choice c = tree_select(L, g, L.turn ? n - 1u - k : k, m01, m23);
   apply(L, c.from, c.to, c.removed);
   return res;
 }

@@ -209,7 +209,12 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
           since_walk = 0;
           if (deferred) {
             const uint32_t r = take_word(w, ply);
-            if (POLICY == MI_POLICY_RANDOM) res = ck::ply_random_given<TRACE>(L, gd, r, &ld);
+            if (POLICY == MI_POLICY_RANDOM) {
+              // capture trees in closed form; the walk only for king trees past two jumps
+              bool deep = false;
+              res = ck::ply_random_trees<TRACE>(L, gd, r, &ld, deep);
+              if (deep) res = ck::ply_random_given<TRACE>(L, gd, r, &ld);
+            }
             else res = ck::ply_heuristic_given<TRACE>(L, gd, r, &ld);
             deferred = false;
             play = true;

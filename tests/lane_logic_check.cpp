@@ -37,8 +37,14 @@ int play(const or_pos &p, uint32_t seed, uint32_t pi, uint32_t j, uint32_t epoch
     if (d) {
       deferred++;
       // the kernel's walk phase: the full ply with the same word and the fast ply's analysis
-      res = policy == OR_POLICY_RANDOM ? ply_random_given<true>(L, g, r, &ld)
-                                       : ply_heuristic_given<true>(L, g, r, &ld);
+      // (random policy: capture trees in closed form, the walk for deep king trees)
+      if (policy == OR_POLICY_RANDOM) {
+        bool deep = false;
+        res = ply_random_trees<true>(L, g, r, &ld, deep);
+        if (deep) res = ply_random_given<true>(L, g, r, &ld);
+      } else {
+        res = ply_heuristic_given<true>(L, g, r, &ld);
+      }
     }
     h = hash_step(h, ply_record(ld, res.n, res.chosen));
     if (res.n > maxn) maxn = res.n;
@@ -88,12 +94,136 @@ int lane_moves(const or_pos &p, or_move *out, int cap) {
   return bad ? -1 : (int)n;
 }
 
+// The deferred ply's move choice (count_mixed / select_mixed: men's trees in closed form,
+// kings' by the walk) against the walk's enumeration, for every move index of a capture
+// position with a multi-jump origin.  Returns the number of disagreeing indices.
+long deep_trees = 0;  // multi-jump positions whose trees the closed forms leave to the walk
+
+long check_mixed(const or_pos &p, long &checked) {
+  const Lane L = lane_from_abs(p.p1, p.p2, p.kings, p.turn, p.counter);
+  Gen g;
+  analyse(L, g);
+  if (!g.capture || !g.multi) return 0;
+  uint32_t packed = 0, trees = 0;
+  LeafCache lc = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  const uint32_t n = count_mixed(L, g, packed, trees, lc);
+  long bad = n != count_walks(L, g);
+  uint32_t from[OR_MAX_MOVES], to[OR_MAX_MOVES], rm[OR_MAX_MOVES];
+  uint32_t m = 0;
+  auto visit = [&](uint32_t idx, uint32_t f, uint32_t t, uint32_t r, uint32_t, uint32_t,
+                   uint32_t) {
+    if (idx < OR_MAX_MOVES) {
+      from[idx] = f;
+      to[idx] = t;
+      rm[idx] = r;
+    }
+    m++;
+  };
+  enumerate(L, g, visit);
+  for (uint32_t k = 0; k < n && k < m && k < OR_MAX_MOVES; k++) {
+    const Choice c = select_mixed(L, g, k, packed, trees, lc);
+    checked++;
+    bad += c.from != from[k] || c.to != to[k] || c.removed != rm[k];
+  }
+  // the same through the capture trees of every origin, where they decide the ply
+  uint32_t m01, m23, deep;
+  const uint32_t nt = tree_count(L, g, m01, m23, deep);
+  // the closed forms give up exactly when a king has a sequence of three or more jumps (or more
+  // than four origins can capture)
+  uint32_t maxd = 0;
+  for (uint32_t o = g.origins & L.kings; o; o &= o - 1u) {
+    auto leaf = [&](uint32_t, uint32_t, uint32_t d, uint32_t) {
+      maxd = d > maxd ? d : maxd;
+      return false;
+    };
+    walk(ctz(o), 1u, g.cap, leaf);
+  }
+  bad += (deep != 0u) != (maxd >= 3u || popc(g.origins) > 4u);
+  if (!deep) {
+    bad += nt != n;
+    for (uint32_t k = 0; k < nt && k < m && k < OR_MAX_MOVES; k++) {
+      const Choice c = tree_select(L, g, k, m01, m23);
+      checked++;
+      bad += c.from != from[k] || c.to != to[k] || c.removed != rm[k];
+    }
+  } else {
+    deep_trees++;
+  }
+  return bad;
+}
+
+// The k-th move of the slot masks by a plain binary search over squares (four masked popcounts
+// per step), against which select_slot's byte-transposed search is checked.
+void select_slot_plain(const uint32_t s[4], uint32_t k, uint32_t &sq, uint32_t &sl) {
+  uint32_t lo = 0;
+  for (uint32_t stride = 16; stride >= 1; stride >>= 1) {
+    const uint32_t below = bit(lo + stride) - 1u;
+    if (popc(s[0] & below) + popc(s[1] & below) + popc(s[2] & below) + popc(s[3] & below) <= k)
+      lo += stride;
+  }
+  const uint32_t below = bit(lo) - 1u;
+  uint32_t j = k - (popc(s[0] & below) + popc(s[1] & below) + popc(s[2] & below) +
+                    popc(s[3] & below));
+  for (sl = 0; sl < 4u; sl++)
+    if (getbit(s[sl], lo) && j-- == 0u) break;
+  sq = lo;
+}
+
+// select_slot against the plain search for every k of random slot masks.
+long check_select_slot(int masks, long &checked) {
+  long bad = 0;
+  uint64_t x = 12345;
+  auto next = [&]() {
+    x = x * 6364136223846793005ull + 1442695040888963407ull;
+    return (uint32_t)(x >> 32);
+  };
+  for (int i = 0; i < masks; i++) {
+    const uint32_t dens = next() % 4u;
+    uint32_t s[4];
+    for (int t = 0; t < 4; t++) {
+      s[t] = next();
+      for (uint32_t d = 0; d < dens; d++) s[t] &= next();
+    }
+    const uint32_t n = popc(s[0]) + popc(s[1]) + popc(s[2]) + popc(s[3]);
+    for (uint32_t k = 0; k < n; k++) {
+      uint32_t a, b, c, d;
+      select_slot(s, k, a, b);
+      select_slot_plain(s, k, c, d);
+      bad += a != c || b != d;
+      checked++;
+    }
+  }
+  return bad;
+}
+
+// Random placements of men and kings (dense enough for long capture trees), from a fixed LCG.
+std::vector<or_pos> placements(int n) {
+  std::vector<or_pos> v((size_t)n);
+  uint64_t x = 0x9E3779B97F4A7C15ull;
+  auto next = [&]() {
+    x = x * 6364136223846793005ull + 1442695040888963407ull;
+    return (uint32_t)(x >> 32);
+  };
+  for (int i = 0; i < n; i++) {
+    const uint32_t occ = next() & next() | (next() & next() & next());  // about 44% occupied
+    const uint32_t side = next();
+    const uint32_t king = next() & next() & next();  // about 12% kings
+    v[i].p1 = occ & side;
+    v[i].p2 = occ & ~side;
+    v[i].kings = occ & king;
+    v[i].turn = (uint8_t)(next() & 1u);
+    v[i].counter = 0;
+    v[i].pad[0] = v[i].pad[1] = 0;
+  }
+  return v;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   const int N = argc > 1 ? std::atoi(argv[1]) : 1024;
   const int P = argc > 2 ? std::atoi(argv[2]) : 8;
-  long bad = 0, playouts = 0, deferred = 0, lists = 0;
+  long bad = 0, playouts = 0, deferred = 0, lists = 0, mixed = 0;
   for (int set = 0; set < 3; set++) {
     std::vector<or_pos> pos(N);
     if (set == 0) or_gen_start_positions(2016, N, 0, 30, 0, pos.data());
@@ -130,12 +260,29 @@ int main(int argc, char **argv) {
           if (bad < 5) std::printf("move list mismatch: set %d position %d ply %d\n", set, i, ply);
           bad++;
         }
+        const long mb = check_mixed(p, mixed);
+        if (mb && bad < 5) std::printf("deferred-ply choice mismatch: set %d position %d ply %d\n",
+                                       set, i, ply);
+        bad += mb;
         if (n <= 0 || p.counter >= OR_DRAW_MOVES) break;
         or_apply(&p, &om[(ply * 7 + i * 3) % n]);
       }
     }
   }
-  std::printf("playouts %ld (deferred plies %ld), move lists %ld, mismatches %ld\n", playouts,
-              deferred, lists, bad);
+  for (const or_pos &p : placements(400 * N)) {
+    const long mb = check_mixed(p, mixed);
+    if (mb && bad < 5)
+      std::printf("deferred-ply choice mismatch: placement %08x %08x %08x turn %d\n", p.p1, p.p2,
+                  p.kings, p.turn);
+    bad += mb;
+  }
+  long slots = 0;
+  const long sb = check_select_slot(20000, slots);
+  if (sb) std::printf("select_slot disagrees with the plain search on %ld of %ld choices\n", sb,
+                      slots);
+  bad += sb;
+  std::printf("playouts %ld (deferred plies %ld), move lists %ld, deferred choices %ld "
+              "(deep trees %ld), mismatches %ld\n", playouts, deferred, lists, mixed, deep_trees,
+              bad);
   return bad != 0;
 }

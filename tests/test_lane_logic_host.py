@@ -57,6 +57,9 @@ def test_lane_logic_matches_oracle(checker):
     # the sample reaches the deferred multi-jump path
     deferred = int(proc.stdout.split("deferred plies ")[1].split(")")[0])
     assert deferred > 1000
+    # every move index of the multi-jump positions met, chosen the deferred ply's way
+    choices = int(proc.stdout.split("deferred choices ")[1].split(" ")[0])
+    assert choices > 10000
 
 
 @pytest.mark.parametrize("k", range(1, 7))
