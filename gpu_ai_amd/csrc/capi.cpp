@@ -206,7 +206,7 @@ uint32_t grid_for(const mi_ctx *ctx, uint64_t total) {
   const uint64_t block = ctx->tuning.block_size;
   const uint64_t need = (total + block - 1) / block;
   const uint64_t resident = (uint64_t)ctx->cus * ctx->tuning.blocks_per_cu;
-  return (uint32_t)std::max<uint64_t>(1, std::min(need, resident));
+  return gpuai::grid_size(need, resident);
 }
 
 // Launches the playout kernel on the staged device buffers and reads back the statistics and,
@@ -223,6 +223,7 @@ int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
   a.offs = with_offs ? (const uint32_t *)ctx->offs.ptr : nullptr;
   a.npos = npos;
   a.per_pos = per_pos;
+  a.per_pos_shift = per_pos && !(per_pos & (per_pos - 1u)) ? (uint32_t)__builtin_ctz(per_pos) : 32u;
   a.total = (uint32_t)total;
   a.seed = seed;
   a.epoch = epoch;
@@ -800,6 +801,7 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
   gpuai::WorkWords *work = (gpuai::WorkWords *)g->d_out;
   gpuai::PlayoutArgs &a = g->args;
   std::memset(&a, 0, sizeof(a));
+  a.per_pos_shift = 32u;  // graph batches always carry offsets
   // positions, offsets and keys are found through the description's section offsets
   a.pos = (const mi_pos *)((char *)g->d_blob + sizeof(gpuai::LaunchDesc));
   a.offs = nullptr;
@@ -882,7 +884,7 @@ int mi_graph_launch(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_
                                                   hipMemcpyDeviceToHost));
     // both refill policies are correct for any grid: static refill strides by the grid's threads
     const uint64_t need = (total + g->block - 1u) / g->block;
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(need, g->resident));
+    const uint32_t grid = gpuai::grid_size(need, g->resident);
     if (grid != g->grid) {
       void *play_args[] = {&g->args};
       hipKernelNodeParams pp;

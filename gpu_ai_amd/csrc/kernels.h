@@ -8,15 +8,39 @@
 
 namespace gpuai {
 
-// Device-side per-launch work words (one 64-byte line, zeroed by launch_reset before use).
+// The queue's playout indices are dealt to slabs in blocks of kSlabBlock: with S = min(kSlabs,
+// grid) slabs, block b (indices 128b .. 128b+127) belongs to slab b % S, and the waves of
+// workgroup w claim from slab w % S only.  Workgroups are dispatched round-robin over the chip's
+// eight XCDs, so a slab's blocks -- and the 128-byte lines of winners they fill -- stay with one
+// XCD's L2 (16 slabs: two per XCD); each slab's counter sits alone in a 128-byte line, so claims
+// of different slabs do not contend for one line (a single shared counter throttled the claims:
+// 4-6% of a launch).  Grids
+// are a multiple of S (grid_size), so every slab is served by as many workgroups as any other
+// and holds the same share of the work, give or take one block; a wave whose slab is dry is
+// done.
+constexpr uint32_t kSlabs = 16;
+constexpr uint32_t kSlabBlock = 128;
+
+// Device-side per-launch work words (zeroed by launch_reset before use).
 struct WorkWords {
-  uint32_t queue;        // next unclaimed playout index (wave-level claims)
-  uint32_t max_moves;    // longest move list seen
-  uint32_t overflow;     // a move list exceeded a capacity
-  uint32_t pad0;
   unsigned long long plies;  // plies applied over the launch
-  uint32_t pad1[10];
+  uint32_t max_moves;        // longest move list seen
+  uint32_t overflow;         // a move list exceeded a capacity
+  uint32_t pad0[28];
+  struct alignas(128) QueueLine {
+    uint32_t next;  // next unclaimed slab-local index (wave-level claims)
+    uint32_t pad[31];
+  } queue[kSlabs];
 };
+static_assert(sizeof(WorkWords) == 128 * (1 + kSlabs), "header line + one line per slab");
+
+// Workgroups of a persistent launch for `need` workgroups' worth of work when `resident` fit on
+// the chip: a multiple of the slab count above it.
+inline uint32_t grid_size(uint64_t need, uint64_t resident) {
+  uint64_t g = need < resident ? need : resident;
+  if (g > kSlabs) g = (g + kSlabs - 1) / kSlabs * kSlabs;
+  return (uint32_t)(g ? g : 1);
+}
 
 // Batch description read by the kernel from device memory (graph replays change it between
 // launches without touching the captured kernel arguments).
@@ -48,6 +72,7 @@ struct PlayoutArgs {
   const uint32_t *keys;
   uint32_t npos;
   uint32_t per_pos;
+  uint32_t per_pos_shift;  // log2(per_pos) when per_pos is a power of two, else 32
   uint32_t total;          // total playouts (< 2^31)
   uint32_t seed;
   uint32_t epoch;

@@ -40,7 +40,7 @@ struct Scratch {
   uint32_t order[kMaxPos];         // hand-out rank -> position index
   mi_pos ord_pos[kMaxPos];         // positions in hand-out order
 };
-static_assert(sizeof(WorkWords) == 64 && sizeof(LaunchDesc) == 64, "64-byte header words");
+static_assert(sizeof(LaunchDesc) == 64, "64-byte batch description");
 
 static __device__ __align__(256) unsigned char scratch[sizeof(Scratch)];
 
@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(kPrepThreads)
     }
   }
   if (t == 0u) {
-    S->work.queue = 0u;
+    for (uint32_t q = 0; q < kSlabs; q++) S->work.queue[q].next = 0u;
     S->work.max_moves = 0u;
     S->work.overflow = 0u;
     S->work.plies = 0ull;
@@ -219,6 +219,7 @@ extern "C" int mi_run_playouts_noalloc(mi_ctx *ctx, const mi_pos *pos, uint32_t 
   a.keys = nullptr;
   a.npos = npos;
   a.per_pos = per_pos;
+  a.per_pos_shift = per_pos && !(per_pos & (per_pos - 1u)) ? (uint32_t)__builtin_ctz(per_pos) : 32u;
   a.total = 0;  // read from the description
   a.seed = seed;
   a.epoch = epoch;
@@ -233,8 +234,8 @@ extern "C" int mi_run_playouts_noalloc(mi_ctx *ctx, const mi_pos *pos, uint32_t 
   const uint64_t block = ctx->tuning.block_size;
   const uint64_t resident = (uint64_t)ctx->cus * ctx->tuning.blocks_per_cu;
   const uint64_t need = counts ? resident : (uniform_total + block - 1) / block;
-  const uint32_t grid = (uint32_t)(need < resident ? need : resident);
-  MI_HIP(ctx, gpuai::launch_playouts(a, policy, false, ctx->tuning.refill, grid ? grid : 1u,
-                                     (uint32_t)block, s));
+  const uint32_t grid = gpuai::grid_size(need, resident);
+  MI_HIP(ctx, gpuai::launch_playouts(a, policy, false, ctx->tuning.refill, grid, (uint32_t)block,
+                                     s));
   return MI_OK;
 }
