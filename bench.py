@@ -23,7 +23,8 @@ recorded in profiles/ (null when absent); roofline.hbm sets those bytes over the
 the 8 TB/s HBM peak (the path is not memory bound).
 cpu_baseline: the reference's own host code (oracle/_ref/libref.so: State::genMoves twice per ply
 as src/playout.cpp:17-32 does, OpenMP over playouts, Philox draws) on a bounded sample of the same
-workload, on this host's cores.
+workload, on the job's whole CPU share (OMP_NUM_THREADS, else every CPU it may use) and on one
+thread, with the host's nproc, affinity and CPU model.
 """
 from __future__ import annotations
 
@@ -65,25 +66,50 @@ def read_traffic(workload: str):
     return float(m.group(1)) if m else None
 
 
+def host_cpu() -> dict:
+    """This host's processors: nproc, the CPUs this process may run on, the model name, and the
+    thread share of the job (OMP_NUM_THREADS, which the GPU hosts set to one GPU's share)."""
+    model = ""
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                     if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or affinity
+    return {"nproc": os.cpu_count(), "affinity": affinity, "share": min(share, affinity),
+            "model": model}
+
+
 def cpu_baseline(cfg: dict, pos, threads: int) -> dict:
-    """Times the reference build (or the C restatement when that is absent) on a sample."""
+    """Times the reference build (or the C restatement when that is absent) on a bounded sample
+    of the workload: on `threads` threads (the job's whole CPU share) and on one thread."""
     from oracle import pyoracle
     kind, lib = "reference", None
     if os.path.exists(pyoracle.REF_SO):
         lib = pyoracle.Reference(pyoracle.REF_SO)
     else:
         kind, lib = "port", pyoracle.Oracle()
-    # about 10-30 s of CPU work: the whole random batch, a quarter of the heuristic one
+
+    def timed(npos: int, per: int, th: int) -> tuple[float, str]:
+        t0 = time.perf_counter()
+        lib.run_playouts(pos[:npos], per_pos=per, seed=PLAY_SEED, epoch=0, policy=cfg["policy"],
+                         threads=th, traces=False)
+        dt = time.perf_counter() - t0
+        return npos * per / dt, (f"first {npos} positions x {per} playouts of the workload "
+                                 f"({npos * per} playouts, {dt:.2f} s wall on {th} threads)")
+
+    # about 10-30 s of CPU work on all threads, a few seconds on one
     npos = min(len(pos), 1024 if cfg["policy"] == 0 else 256)
     per = min(cfg["per_pos"], 1024)
-    t0 = time.perf_counter()
-    lib.run_playouts(pos[:npos], per_pos=per, seed=PLAY_SEED, epoch=0, policy=cfg["policy"],
-                     threads=threads, traces=False)
-    dt = time.perf_counter() - t0
-    return {"value": round(npos * per / dt, 1), "unit": "playouts/s", "cores": threads,
-            "kind": kind,
-            "sample": f"first {npos} positions x {per} playouts of the workload "
-                      f"({npos * per} playouts, {dt:.2f} s wall on {threads} threads)"}
+    rate, sample = timed(npos, per, threads)
+    rate1, sample1 = timed(min(len(pos), 64), min(cfg["per_pos"], 1024), 1)
+    return {"value": round(rate, 1), "unit": "playouts/s", "cores": threads, "kind": kind,
+            "sample": sample, "one_thread": {"value": round(rate1, 1), "sample": sample1},
+            "host": host_cpu()}
 
 
 def bench_cpu(args) -> int:
@@ -98,7 +124,7 @@ def bench_cpu(args) -> int:
     cfg = WORKLOADS["small"]
     orc = pyoracle.Oracle()
     pos = orc.start_positions(POS_SEED, cfg["n"], cfg["kmin"], cfg["kmax"])
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or host_cpu()["share"]
     codes = [("restatement (oracle/liboracle.so)", orc, [("philox", 0)])]
     if os.path.exists(pyoracle.REF_SO):
         codes.insert(0, ("reference host code (oracle/_ref/libref.so)",
@@ -119,16 +145,9 @@ def bench_cpu(args) -> int:
                              "playouts_per_s": round(len(pos) * per / dt, 1)})
                 print(f"[cpu] {code}, {choice}, {th} threads: {rows[-1]['playouts_per_s']:.0f} "
                       f"playouts/s", file=sys.stderr, flush=True)
-    model = ""
-    try:
-        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
-                     if l.startswith("model name"))
-    except (OSError, StopIteration):
-        pass
     print(json.dumps({"metric": "CPU playouts/s (random-64x4096 positions)", "unit": "playouts/s",
                       "higher_is_better": True, "data": "synthetic (seeded positions)",
-                      "host": {"cpu_count": os.cpu_count(), "model": model},
-                      "rows": rows}), flush=True)
+                      "host": host_cpu(), "rows": rows}), flush=True)
     return 0
 
 
@@ -349,7 +368,7 @@ def main() -> int:
         line["copy_calibration"] = {"bytes": args.copy_calibration, "avg_ms": round(ms, 4),
                                     "GB/s": round(2 * args.copy_calibration / ms / 1e6, 1)}
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cpu()["share"]
         try:
             line["cpu_baseline"] = cpu_baseline(cfg, pos, threads)
         except Exception as e:  # the baseline is reported, never required for the GPU figure

@@ -21,7 +21,11 @@ namespace {
 // Queue claims are made in multiples of kChunk playouts (a wave holds at most one partly used
 // claim).  With one counter per slab in a line of its own, claims of 16 beat 32 by about 1%
 // (less unplayed work held when a slab runs dry); on a single shared counter, 16 had lost 9% to
-// the contention of twice as many atomics on one line.
+// the contention of twice as many atomics on one line.  Each claim is a device atomic, which
+// travels to memory as a 32-byte request (random-1024x1024: about 65,500 per launch, twice the
+// winners' bytes in the write-request counters).  Workgroup pools in LDS, refilled by guided
+// claims from the slab counter, cut that to about 5,000 but cost 5-9% of the launch in
+// instructions on the refill path, which this VALU-bound loop cannot hide; claims stay per wave.
 constexpr uint32_t kChunk = 16;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
@@ -254,15 +258,28 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
       }
     }
   }
-  // per-wave reduction of the statistics, one atomic per wave
+  // statistics: per wave, then per workgroup in LDS, then one device atomic each (device atomics
+  // travel to memory as 32-byte requests)
   for (int off = 32; off >= 1; off >>= 1) {
     plies_acc += __shfl_xor(plies_acc, off);
     uint32_t m = __shfl_xor(maxn, off);
     maxn = m > maxn ? m : maxn;
   }
+  __shared__ unsigned long long wg_plies;
+  __shared__ uint32_t wg_maxn;
+  if (threadIdx.x == 0) {
+    wg_plies = 0ull;
+    wg_maxn = 0u;
+  }
+  __syncthreads();
   if (lane == 0u) {
-    atomicAdd(&a.work->plies, (unsigned long long)plies_acc);
-    atomicMax(&a.work->max_moves, maxn);
+    atomicAdd(&wg_plies, (unsigned long long)plies_acc);
+    atomicMax(&wg_maxn, maxn);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&a.work->plies, wg_plies);
+    atomicMax(&a.work->max_moves, wg_maxn);
   }
 }
 
