@@ -1,6 +1,7 @@
 """Throughput benchmark of the MI355X playout path (one JSON line on stdout).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
+                    [--no-match]
 
 A step is one launch of the persistent playout kernel over the staged batch of the workload:
   random (default): 1024 seeded positions (k in [0, 30] random plies from the start) x 1024
@@ -25,6 +26,11 @@ cpu_baseline: the reference's own host code (oracle/_ref/libref.so: State::genMo
 as src/playout.cpp:17-32 does, OpenMP over playouts, Philox draws) on a bounded sample of the same
 workload, on the job's whole CPU share (OMP_NUM_THREADS, else every CPU it may use) and on one
 thread, with the host's nproc, affinity and CPU model.
+net_leaf_win_rate: the win-rate half of the metric, measured in the run (random workload, unless
+--no-match or --tuning): the value network is trained on the device (2M positions labelled by 256
+random playouts, 24 epochs, seconds), then 400 games of network leaves against playout leaves at
+250 ms per move each (about 2.5 minutes; progress on stderr).  The network's score is
+(wins + draws / 2) / games with its standard error.
 """
 from __future__ import annotations
 
@@ -179,20 +185,56 @@ def bench_multisearch() -> int:
     return 0
 
 
-def recorded_match():
-    """The network-leaf vs playout-leaf match recorded by `python -m gpu_ai_amd.value` (400
-    games, 250 ms per move; minutes of GPU time, so it is read, not rerun, here): the first run
-    of the table in profiles/value_net_match_400x250ms.md."""
-    path = os.path.join(ROOT, "profiles", "value_net_match_400x250ms.md")
-    if not os.path.exists(path):
-        return None
-    m = re.search(r"^\| ([A-Z]) \| ([^|]*)\|.*\*\*([0-9.]+) \+- ([0-9.]+)\*\*", open(path).read(),
-                  flags=re.M)
-    if not m:
-        return None
-    return {"score": float(m.group(3)), "stderr": float(m.group(4)), "games": 400,
-            "move_ms": 250, "training": m.group(2).strip(),
-            "source": f"profiles/value_net_match_400x250ms.md, run {m.group(1)}"}
+# The match: each player's exploration factor k in sqrt(k ln N / n), both at the best value of
+# the sweeps at 250 ms per move (profiles/value_net_match_400x250ms.md): the strongest playout-only
+# player over k in {0.5, 1, ..., 1024} (the reference's own player uses 2, src/mcts.cpp:189-190)
+# and the network player's best k against it.
+MATCH_NET_K = 1024.0
+MATCH_PLAYOUT_K = 256.0
+
+
+def measured_match(eng, args) -> dict:
+    """Trains the value network in the run (2M positions labelled by 256 random playouts each,
+    k in [0, 150] plies, 24 epochs of batch 4096, Adam lr 1e-3) and plays the 400-game match of
+    network leaves against playout leaves at equal wall time per move (netmatch.hpp): the
+    measured half of the metric.  Progress lines go to stderr."""
+    from gpu_ai_amd import host_api, value
+    from gpu_ai_amd.engine import ValueNet
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    pos, t = value.make_dataset(eng, 2_000_000, POS_SEED, kmax=150, playouts=256)
+    label_s = time.perf_counter() - t0
+    log(f"labelled {len(pos)} positions with 256 playouts each in {label_s:.1f} s")
+    with ValueNet(eng, 4096, seed=POS_SEED) as net:
+        t1 = time.perf_counter()
+        hist = value.train(net, pos, t, 24, 4096, 1e-3)
+        train_s = time.perf_counter() - t1
+        del pos, t
+        ev = value.evaluate(eng, net, 16384, ks=(1, 16, 256))
+        params = net.get_params()
+    log(f"trained 24 epochs in {train_s:.1f} s (loss {hist[-1]['loss']:.4f}, value MSE "
+        f"{ev['mse_net']:.4f}); playing {args.match_games} games at {args.match_ms:g} ms per move")
+    t2 = time.perf_counter()
+    r = host_api.net_match(params, games=args.match_games, seed=POS_SEED, move_ms=args.match_ms,
+                           progress_every=25, net_exploration=MATCH_NET_K,
+                           playout_exploration=args.match_playout_k)
+    match_s = time.perf_counter() - t2
+    log(f"match: network {r['net_wins']} wins, playouts {r['playout_wins']} wins, "
+        f"{r['draws']} draws, score {r['score']:.3f} in {match_s:.0f} s")
+    return {"score": round(r["score"], 4), "stderr": round(r["score_stderr"], 4),
+            "games": r["games"], "net_wins": r["net_wins"], "playout_wins": r["playout_wins"],
+            "draws": r["draws"], "move_ms": args.match_ms, "openings_seed": POS_SEED,
+            "net_exploration": MATCH_NET_K, "playout_exploration": args.match_playout_k,
+            "leaves": {"network": r["net_leaves"], "playouts": r["playout_leaves"]},
+            "training": {"positions": 2_000_000, "labels": "256 random playouts",
+                         "epochs": 24, "label_s": round(label_s, 2),
+                         "train_s": round(train_s, 2), "final_loss": round(hist[-1]["loss"], 5),
+                         "value_mse": round(ev["mse_net"], 5),
+                         "mse_playouts": {k: round(v, 5) for k, v in ev["mse_playouts"].items()}},
+            "seconds": round(match_s, 1)}
 
 
 # FLOPs of one training step of the 128-256-256-1 network on a batch of B positions: forward
@@ -224,7 +266,6 @@ def bench_value(args) -> int:
     device_s = sum(h["device_ms"] for h in timed) / 1e3
     flops = mlp_step_flops(4096)
     achieved = flops * steps / device_s
-    match = recorded_match()
     line = {
         "metric": "value-network training throughput (positions/s) on 1 MI355X",
         "value": round(4096 * steps / train_s, 1), "unit": "positions/s", "n_gpus": 1,
@@ -240,7 +281,6 @@ def bench_value(args) -> int:
                         "mse_playouts": {k: round(v, 5) for k, v in ev["mse_playouts"].items()},
                         "equivalent_playouts": round(ev["equivalent_playouts"], 1),
                         "after_epochs": epochs + args.warmup},
-        "match_recorded": match,
         "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 3), "peak": 2500.0,
                      "unit": "TFLOP/s", "frac": round(achieved / 2.5e15, 5), "traffic": None,
                      "kernel": "training step (k_encode, k_gemm x5, k_head_train, k_colsum, k_adam)",
@@ -266,6 +306,12 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a plain copy kernel of BYTES (traffic-counter calibration)")
+    ap.add_argument("--no-match", action="store_true",
+                    help="skip the in-run training and network-vs-playout match (the random "
+                         "workload runs them by default)")
+    ap.add_argument("--match-games", type=int, default=400)
+    ap.add_argument("--match-ms", type=float, default=250.0)
+    ap.add_argument("--match-playout-k", type=float, default=MATCH_PLAYOUT_K)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the playout path runs on a single MI355X here "
@@ -362,7 +408,6 @@ def main() -> int:
     valu_rate, valu_ms = eng.valu_rate()
     line["roofline"]["peak_measured"] = round(valu_rate / 1e12, 3)
     line["roofline"]["frac_of_measured"] = round(achieved / valu_rate, 4) if valu_rate else None
-    line["net_leaf_win_rate_recorded"] = recorded_match()
     if args.copy_calibration:
         ms = eng.copy_calibration(args.copy_calibration, 5)
         line["copy_calibration"] = {"bytes": args.copy_calibration, "avg_ms": round(ms, 4),
@@ -373,6 +418,10 @@ def main() -> int:
             line["cpu_baseline"] = cpu_baseline(cfg, pos, threads)
         except Exception as e:  # the baseline is reported, never required for the GPU figure
             line["cpu_baseline"] = {"value": None, "error": str(e)}
+    if args.workload == "random" and not args.no_match and not args.tuning:
+        line["net_leaf_win_rate"] = measured_match(eng, args)
+    else:
+        line["net_leaf_win_rate"] = None  # skipped: --no-match, a profiled run, or another workload
     eng.close()
     print(json.dumps(line), flush=True)
     return 0

@@ -254,8 +254,10 @@ int gpuai_net_match_run(const gpuai_net_match_config *cfg, const float *params,
     c.netBatch = cfg->net_batch;
     c.progressEvery = cfg->progress_every;
     c.netExploration = cfg->net_exploration;
+    c.playoutExploration = cfg->playout_exploration;
+    c.moverWins = cfg->mover_wins != 0;
     if (c.openingMax < c.openingMin || c.leafBatch == 0 || c.netBatch == 0 || !(c.moveMs >= 0) ||
-        !(c.netExploration > 0))
+        !(c.netExploration > 0) || !(c.playoutExploration > 0))
       throw std::runtime_error("invalid match configuration");
     NetMatchResult r = runNetMatch(c, params);
     out->games = r.games;

@@ -33,6 +33,7 @@ std::shared_ptr<GameTree> GameTree::move(const Move &m) {
   next.move(m);
   auto t = std::make_shared<GameTree>(next);
   t->explore = explore;
+  t->moverWins = moverWins;
   return t;
 }
 
@@ -82,6 +83,7 @@ void GameTree::selectInto(unsigned trials, std::vector<State> &out) {
         next.move(m);
         children.push_back(std::make_shared<GameTree>(next, this));
         children.back()->explore = explore;
+        children.back()->moverWins = moverWins;
       }
       expanded = true;
     } else {
@@ -98,7 +100,7 @@ void GameTree::selectInto(unsigned trials, std::vector<State> &out) {
   unsigned untried = 0;
   const double lnN = std::log((double)totalTrials);  // the parent's term, once per node
   for (size_t i = 0; i < n; i++) {
-    weight[i] = children[i]->ucb1(lnN, explore);
+    weight[i] = children[i]->ucb1(lnN, explore, moverWins ? state.turn : children[i]->state.turn);
     weightSum += weight[i];
     if (children[i]->totalTrials == 0) untried++;
     share[i] = 0;
@@ -206,12 +208,13 @@ void GameTree::updateValues(const float *p1WinProb, size_t n) {
 // The reference's arithmetic, including its long-double exploration term: the win rate of the
 // side to move in this child (src/mcts.cpp:189) plus sqrt(2 ln N / n).
 double GameTree::ucb1() const {
-  return ucb1(std::log((double)parent->totalTrials), parent->explore);
+  return ucb1(std::log((double)parent->totalTrials), parent->explore,
+              parent->moverWins ? parent->state.turn : state.turn);
 }
 
 // lnParent = log(parent's trials) in double precision, as log(unsigned) evaluates in the
 // reference; the rest in long double as there.
-double GameTree::ucb1(double lnParent, double k) const {
+double GameTree::ucb1(double lnParent, double k, PlayerId side) const {
   if (totalTrials == 0) return INFINITY;
-  return (double)wins[state.turn] / totalTrials + std::sqrt((long double)k * lnParent / totalTrials);
+  return (double)wins[side] / totalTrials + std::sqrt((long double)k * lnParent / totalTrials);
 }

@@ -43,6 +43,12 @@ class GameTree {
   // The factor k of the exploration term sqrt(k ln N / n); the reference's UCB1 has k = 2
   // (src/mcts.cpp:189-190), the default.  Nodes created later inherit it.
   void setExploration(double k) { explore = k; }
+  // Whose win fraction the exploitation term counts.  The reference's UCB1 counts the wins of the
+  // side to move in the child (src/mcts.cpp:189), that is, of the opponent of the player who
+  // chooses among the children, so its trials lean towards moves that are good for the opponent;
+  // on, the term counts the chooser's own wins.  Off by default (the reference's arithmetic);
+  // nodes created later inherit it.
+  void setMoverWins(bool on) { moverWins = on; }
 
   const State state;
 
@@ -55,7 +61,7 @@ class GameTree {
 
  private:
   double ucb1() const;  // src/mcts.cpp:182-191
-  double ucb1(double lnParent, double k) const;
+  double ucb1(double lnParent, double k, PlayerId side) const;
   template <class R, class Add>
   void updateImpl(const R *results, size_t n, Add add, double sums[NUM_PLAYERS]);
 
@@ -64,6 +70,7 @@ class GameTree {
   bool over = false;       // isGameOver() of the node's state, fixed at construction
   bool parallel = false;
   double explore = 2.0;
+  bool moverWins = false;
   mutable std::vector<Move> moves;
   mutable bool movesReady = false;
   std::vector<std::shared_ptr<GameTree>> children;

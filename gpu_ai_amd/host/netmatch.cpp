@@ -89,6 +89,9 @@ NetMatchResult runNetMatch(const NetMatchConfig &cfg, const float *params) {
       G.tree[s]->setParallel(false);
     }
     G.tree[0]->setExploration(cfg.netExploration);
+    G.tree[1]->setExploration(cfg.playoutExploration);
+    G.tree[0]->setMoverWins(cfg.moverWins);
+    G.tree[1]->setMoverWins(cfg.moverWins);
   }
 
   // leaf scorers: player 1's win probability per leaf, or playout winners

@@ -26,9 +26,13 @@ struct NetMatchConfig {
   uint32_t playoutSeed = 12345;
   uint32_t netBatch = 4096;      // network batch (max_batch)
   uint32_t progressEvery = 0;    // print a status line to stderr every this many plies (0: never)
-  // exploration factor k of the network player's tree, sqrt(k ln N / n) (the playout player
-  // keeps the reference's k = 2); the default is the run_ai network player's
+  // exploration factor k of each player's tree, sqrt(k ln N / n): the network player's default is
+  // the run_ai network player's, the playout player's the reference's (src/mcts.cpp:189-190)
   double netExploration = 64.0;
+  double playoutExploration = 2.0;
+  // both trees count the chooser's own wins in UCB1 (GameTree::setMoverWins), not the
+  // reference's child side to move
+  bool moverWins = false;
   int device = 0;
 };
 

@@ -64,7 +64,8 @@ class NetMatchConfig(ctypes.Structure):
     _fields_ = [("games", _U32), ("seed", _U32), ("opening_min", _U32), ("opening_max", _U32),
                 ("move_ms", ctypes.c_double), ("leaf_batch", _U32), ("max_plies", _U32),
                 ("playout_seed", _U32), ("net_batch", _U32), ("progress_every", _U32),
-                ("net_exploration", ctypes.c_double)]
+                ("net_exploration", ctypes.c_double), ("playout_exploration", ctypes.c_double),
+                ("mover_wins", _U32)]
 
 
 class NetMatchResult(ctypes.Structure):
@@ -213,12 +214,16 @@ def search_record(root, iterations: int, initial: int = 256, scale: float = 0.0,
 def net_match(params, games: int = 400, seed: int = 2016, opening_min: int = 0,
               opening_max: int = 6, move_ms: float = 100.0, leaf_batch: int = 4096,
               max_plies: int = 300, playout_seed: int = 12345, net_batch: int = 4096,
-              progress_every: int = 0, net_exploration: float = 64.0) -> dict:
+              progress_every: int = 0, net_exploration: float = 64.0,
+              playout_exploration: float = 2.0, mover_wins: bool = False) -> dict:
     """Value-network leaves against playout leaves at equal wall time per move (netmatch.hpp);
-    returns the tally, the network's score and the per-game outcomes (1 / -1 / 0)."""
+    returns the tally, the network's score and the per-game outcomes (1 / -1 / 0).  The
+    exploration factors are each player's k in sqrt(k ln N / n); mover_wins makes both trees
+    count the chooser's own wins in UCB1 instead of the reference's child side to move."""
     w = np.ascontiguousarray(params, dtype=np.float32)
     cfg = NetMatchConfig(games, seed, opening_min, opening_max, move_ms, leaf_batch, max_plies,
-                         playout_seed, net_batch, progress_every, net_exploration)
+                         playout_seed, net_batch, progress_every, net_exploration,
+                         playout_exploration, int(mover_wins))
     res = NetMatchResult()
     outcome = np.zeros(games, dtype=np.int8)
     err = ctypes.create_string_buffer(512)

@@ -109,9 +109,14 @@ def main(argv=None) -> int:
     ap.add_argument("--move-ms", type=float, default=100.0)
     ap.add_argument("--leaf-batch", type=int, default=4096)
     ap.add_argument("--net-exploration", type=float, default=64.0,
-                    help="k of the network player's UCB1 term sqrt(k ln N / n) (the playout "
-                         "player keeps the reference's 2)")
+                    help="k of the network player's UCB1 term sqrt(k ln N / n)")
+    ap.add_argument("--playout-exploration", default="2",
+                    help="k of the playout player's UCB1 term (the reference's is 2); a comma "
+                         "list plays one match per value against the same network")
     ap.add_argument("--match-seed", type=int, default=2016, help="seed of the match openings")
+    ap.add_argument("--mover-wins", action="store_true",
+                    help="both trees count the chooser's own wins in UCB1 (the reference counts "
+                         "the child's side to move)")
     args = ap.parse_args(argv)
     eng = Engine(0)
     with ValueNet(eng, args.batch, seed=args.seed) as net:
@@ -135,12 +140,17 @@ def main(argv=None) -> int:
     eng.close()
     if args.match_games:
         from . import host_api
-        r = host_api.net_match(params, games=args.match_games, seed=args.match_seed,
-                               move_ms=args.move_ms,
-                               leaf_batch=args.leaf_batch, net_batch=args.batch,
-                               progress_every=25, net_exploration=args.net_exploration)
-        r["outcome"] = "".join({1: "W", -1: "L", 0: "D"}[int(x)] for x in r["outcome"])
-        print(json.dumps({"match": r}), flush=True)
+        for kp in (float(x) for x in args.playout_exploration.split(",")):
+            r = host_api.net_match(params, games=args.match_games, seed=args.match_seed,
+                                   move_ms=args.move_ms, leaf_batch=args.leaf_batch,
+                                   net_batch=args.batch, progress_every=25,
+                                   net_exploration=args.net_exploration, playout_exploration=kp,
+                                   mover_wins=args.mover_wins)
+            r["outcome"] = "".join({1: "W", -1: "L", 0: "D"}[int(x)] for x in r["outcome"])
+            r["net_exploration"] = args.net_exploration
+            r["playout_exploration"] = kp
+            r["mover_wins"] = args.mover_wins
+            print(json.dumps({"match": r}), flush=True)
     return 0
 
 

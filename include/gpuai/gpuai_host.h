@@ -116,8 +116,12 @@ typedef struct gpuai_net_match_config {
   uint32_t playout_seed;   /* (12345) */
   uint32_t net_batch;      /* network batch (4096) */
   uint32_t progress_every; /* status line on stderr every this many plies (0: none) */
-  double net_exploration;  /* k of the network player's sqrt(k ln N / n) (64; the playout
-                            * player keeps the reference's 2) */
+  double net_exploration;  /* k of the network player's sqrt(k ln N / n) (64) */
+  double playout_exploration; /* k of the playout player's sqrt(k ln N / n) (the reference's 2,
+                               * src/mcts.cpp:189-190) */
+  uint32_t mover_wins;     /* 0: both trees count the child's side to move in UCB1's win
+                            * fraction, as the reference does (src/mcts.cpp:189); 1: the
+                            * chooser's own wins (GameTree::setMoverWins) */
 } gpuai_net_match_config;
 
 typedef struct gpuai_net_match_result {

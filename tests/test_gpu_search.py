@@ -121,14 +121,15 @@ def test_run_ai_net_leaf_player(tmp_path):
 
 def test_net_match_small():
     """The net-leaf vs playout-leaf match harness on a few short games: consistent tallies and
-    per-part times, both exploration settings, and a rejected configuration."""
+    per-part times, several exploration settings of both players, and rejected configurations."""
     from gpu_ai_amd import host_api
     from gpu_ai_amd.engine import Engine, ValueNet
     with Engine(0) as eng, ValueNet(eng, 4096, seed=7) as net:
         params = net.get_params()
-    for k in (2.0, 0.5):
-        announce(f"net match: 4 games, 10 ms per move, 40 plies, exploration {k}")
-        r = host_api.net_match(params, games=4, move_ms=10.0, max_plies=40, net_exploration=k)
+    for k, kp in ((2.0, 2.0), (0.5, 8.0), (64.0, 0.5)):
+        announce(f"net match: 4 games, 10 ms per move, 40 plies, exploration {k} / {kp}")
+        r = host_api.net_match(params, games=4, move_ms=10.0, max_plies=40, net_exploration=k,
+                               playout_exploration=kp)
         assert r["games"] == 4
         assert r["net_wins"] + r["playout_wins"] + r["draws"] == 4
         out = np.asarray(r["outcome"])
@@ -139,6 +140,8 @@ def test_net_match_small():
         assert r["score_ms"][0] > 0.0 and r["score_ms"][1] > 0.0
     with pytest.raises(RuntimeError):
         host_api.net_match(params, games=2, move_ms=5.0, max_plies=10, net_exploration=0.0)
+    with pytest.raises(RuntimeError):
+        host_api.net_match(params, games=2, move_ms=5.0, max_plies=10, playout_exploration=-1.0)
 
 
 # ---------------------------------------------------------------------------------------------
