@@ -202,11 +202,34 @@ int upload_batch(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *
   return MI_OK;
 }
 
-uint32_t grid_for(const mi_ctx *ctx, uint64_t total) {
-  const uint64_t block = ctx->tuning.block_size;
+uint32_t grid_for(const mi_ctx *ctx, const mi_tuning &t, uint64_t total) {
+  const uint64_t block = t.block_size;
   const uint64_t need = (total + block - 1) / block;
-  const uint64_t resident = (uint64_t)ctx->cus * ctx->tuning.blocks_per_cu;
+  const uint64_t resident = (uint64_t)ctx->cus * t.blocks_per_cu;
   return gpuai::grid_size(need, resident);
+}
+
+uint32_t tune_bucket(uint64_t total) {
+  const uint32_t b = total ? 63u - (uint32_t)__builtin_clzll(total) : 0u;
+  return b < kTuneBuckets ? b : kTuneBuckets - 1u;
+}
+
+// The launch shape of a batch of `total` playouts: the table's entry for its policy and bucket
+// (the faster refill, or refill `only` when it is 0 or 1), else the context's tuning.
+mi_tuning shape_for(const mi_ctx *ctx, uint32_t policy, uint64_t total, int only = -1) {
+  const TuneEntry *e = ctx->tune_table[policy == MI_POLICY_HEURISTIC ? 1 : 0][tune_bucket(total)];
+  const TuneEntry *best = nullptr;
+  for (int r = 0; r < 2; r++) {
+    if (e[r].kernel_ms < 0.f || (only >= 0 && r != only)) continue;
+    if (!best || e[r].kernel_ms < best->kernel_ms) best = &e[r];
+  }
+  return best ? best->tuning : ctx->tuning;
+}
+
+void clear_tune_table(mi_ctx *ctx) {
+  for (auto &pol : ctx->tune_table)
+    for (auto &bucket : pol)
+      for (TuneEntry &e : bucket) e = TuneEntry();
 }
 
 // Launches the playout kernel on the staged device buffers and reads back the statistics and,
@@ -214,7 +237,9 @@ uint32_t grid_for(const mi_ctx *ctx, uint64_t total) {
 // one synchronisation).
 int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, bool with_offs,
                uint32_t seed, uint32_t epoch, uint32_t policy, bool trace, mi_stats *stats,
-               int8_t *winners = nullptr, bool keyed = false) {
+               int8_t *winners = nullptr, bool keyed = false, const mi_tuning *shape = nullptr) {
+  // the autotuner passes the candidate it measures; every other launch takes its batch's shape
+  const mi_tuning t = shape ? *shape : shape_for(ctx, policy, total);
   int rc;
   if ((rc = ensure_host(ctx, ctx->h_back, sizeof(gpuai::WorkWords) + (winners ? total : 0))))
     return rc;
@@ -231,17 +256,16 @@ int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
   a.plies = trace ? (uint32_t *)ctx->plies.ptr : nullptr;
   a.hashes = trace ? (uint32_t *)ctx->hashes.ptr : nullptr;
   a.work = (gpuai::WorkWords *)ctx->work.ptr;
-  a.defer_min = ctx->tuning.defer_min;
+  a.defer_min = t.defer_min;
   a.desc = nullptr;
   a.ord_pos = ctx->ordered ? (const mi_pos *)ctx->ord_pos.ptr : nullptr;
   a.order = ctx->ordered ? (const uint32_t *)ctx->order.ptr : nullptr;
   a.ord_offs = ctx->ordered && with_offs ? (const uint32_t *)ctx->ord_offs.ptr : nullptr;
   a.keys = keyed ? (const uint32_t *)ctx->keys.ptr : nullptr;
-  const uint32_t grid = grid_for(ctx, total);
+  const uint32_t grid = grid_for(ctx, t, total);
   MI_HIP(ctx, gpuai::launch_reset(a.work, ctx->stream));
   MI_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
-  MI_HIP(ctx, gpuai::launch_playouts(a, policy, trace, ctx->tuning.refill, grid,
-                                     ctx->tuning.block_size, ctx->stream));
+  MI_HIP(ctx, gpuai::launch_playouts(a, policy, trace, t.refill, grid, t.block_size, ctx->stream));
   MI_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
   gpuai::WorkWords *hw = (gpuai::WorkWords *)ctx->h_back.ptr;
   int8_t *hwin = (int8_t *)ctx->h_back.ptr + sizeof(gpuai::WorkWords);
@@ -259,8 +283,8 @@ int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
     stats->kernel_ms = ms;
     stats->max_moves = w.max_moves;
     stats->grid_blocks = grid;
-    stats->block_size = ctx->tuning.block_size;
-    stats->refill = ctx->tuning.refill;
+    stats->block_size = t.block_size;
+    stats->refill = t.refill;
   }
   return MI_OK;
 }
@@ -414,6 +438,7 @@ int mi_set_tuning(mi_ctx *ctx, const mi_tuning *tuning) {
   int rc = check_tuning(ctx, t);
   if (rc) return rc;
   ctx->tuning = t;
+  clear_tune_table(ctx);  // an explicit shape holds for every batch size
   return MI_OK;
 }
 
@@ -522,6 +547,70 @@ int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin
   return MI_OK;
 }
 
+namespace {
+
+// Measures every candidate launch shape on the uploaded batch -- workgroup sizes 64..1024, 1..8
+// resident waves per SIMD, both refill policies, deferral thresholds 1, 8 and 16 -- `reps`
+// launches each (median kernel time), records the fastest of each refill policy in the tuning
+// table's entry for the batch's policy and size bucket, and returns the fastest overall.
+int tune_sweep(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, bool with_offs,
+               uint32_t policy, uint32_t reps, mi_tune_result *results, uint32_t max_results,
+               uint32_t *nresults, mi_tuning &best_t, float &best_ms) {
+  const uint32_t blocks[] = {64u, 128u, 192u, 256u, 512u, 768u, 1024u};
+  // 3 and 4 waves per SIMD are the fast ones on random-1024x1024 (more lanes in flight lengthen
+  // the end of a batch, fewer leave the SIMDs short of waves to issue from)
+  const uint32_t waves_per_simd[] = {1u, 2u, 3u, 4u, 8u};
+  const uint32_t defers[] = {1u, 8u, 16u};
+  mi_tuning refill_best[2] = {};
+  float refill_ms[2] = {3.0e38f, 3.0e38f};
+  uint32_t n = 0;
+  for (uint32_t refill = 0; refill < 2; refill++) {
+    for (uint32_t b : blocks) {
+      for (uint32_t w : waves_per_simd) {
+        for (uint32_t dm : defers) {
+          const uint32_t waves_per_cu = 4u * w;   // four SIMDs per CU
+          if ((b / 64u) > waves_per_cu) continue;  // a block larger than the residency
+          if ((waves_per_cu * 64u) % b) continue;  // whole blocks only
+          const mi_tuning t = {b, waves_per_cu * 64u / b, refill, dm};
+          std::vector<float> ms;
+          for (uint32_t r = 0; r < reps; r++) {
+            mi_stats st;
+            const int rc = run_kernel(ctx, npos, per_pos, total, with_offs, 0x7A5Eu,
+                                      0xFFFF0000u + r, policy, false, &st, nullptr, false, &t);
+            if (rc) return rc;
+            ms.push_back(st.kernel_ms);
+          }
+          std::sort(ms.begin(), ms.end());
+          const float med = ms[ms.size() / 2];
+          if (results && n < max_results) {
+            results[n].tuning = t;
+            results[n].kernel_ms = med;
+            results[n].playouts_per_s = (float)((double)total / (med * 1e-3));
+          }
+          n++;
+          if (med < refill_ms[refill]) {
+            refill_ms[refill] = med;
+            refill_best[refill] = t;
+          }
+        }
+      }
+    }
+  }
+  TuneEntry *e = ctx->tune_table[policy == MI_POLICY_HEURISTIC ? 1 : 0][tune_bucket(total)];
+  for (int r = 0; r < 2; r++) {
+    e[r].tuning = refill_best[r];
+    e[r].kernel_ms = refill_ms[r];
+    e[r].playouts = (uint32_t)total;
+  }
+  const int r = refill_ms[1] < refill_ms[0] ? 1 : 0;
+  best_t = refill_best[r];
+  best_ms = refill_ms[r];
+  if (nresults) *nresults = n;
+  return MI_OK;
+}
+
+}  // namespace
+
 int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
                 uint32_t per_pos, uint32_t policy, uint32_t reps, mi_tune_result *results,
                 uint32_t max_results, uint32_t *nresults, mi_tuning *best) {
@@ -537,54 +626,69 @@ int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *c
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
   if ((rc = upload_batch(ctx, pos, npos, counts, total))) return rc;
   ctx->staged = false;
-  const mi_tuning saved = ctx->tuning;
-  const uint32_t blocks[] = {64u, 128u, 192u, 256u, 512u, 768u, 1024u};
-  // 3 and 4 waves per SIMD are the fast ones on random-1024x1024 (more lanes in flight lengthen
-  // the end of a batch, fewer leave the SIMDs short of waves to issue from)
-  const uint32_t waves_per_simd[] = {1u, 2u, 3u, 4u, 8u};
-  mi_tuning best_t = saved;
-  float best_ms = 3.0e38f;
-  uint32_t n = 0;
-  const uint32_t defers[] = {1u, 8u, 16u};
-  for (uint32_t refill = 0; refill < 2; refill++) {
-    for (uint32_t b : blocks) {
-      for (uint32_t w : waves_per_simd) {
-       for (uint32_t dm : defers) {
-        const uint32_t waves_per_cu = 4u * w;           // four SIMDs per CU
-        if ((b / 64u) > waves_per_cu) continue;          // a block larger than the residency
-        if ((waves_per_cu * 64u) % b) continue;          // whole blocks only
-        mi_tuning t = {b, waves_per_cu * 64u / b, refill, dm};
-        ctx->tuning = t;
-        std::vector<float> ms;
-        for (uint32_t r = 0; r < reps; r++) {
-          mi_stats st;
-          rc = run_kernel(ctx, npos, per_pos, total, counts != nullptr, 0x7A5Eu, 0xFFFF0000u + r,
-                          policy, false, &st);
-          if (rc) {
-            ctx->tuning = saved;
-            return rc;
-          }
-          ms.push_back(st.kernel_ms);
-        }
-        std::sort(ms.begin(), ms.end());
-        const float med = ms[ms.size() / 2];
-        if (results && n < max_results) {
-          results[n].tuning = t;
-          results[n].kernel_ms = med;
-          results[n].playouts_per_s = (float)((double)total / (med * 1e-3));
-        }
-        n++;
-        if (med < best_ms) {
-          best_ms = med;
-          best_t = t;
-        }
-       }
-      }
+  mi_tuning best_t = ctx->tuning;
+  float best_ms = 0.f;
+  if ((rc = tune_sweep(ctx, npos, per_pos, total, counts != nullptr, policy, reps, results,
+                       max_results, nresults, best_t, best_ms)))
+    return rc;
+  ctx->tuning = best_t;
+  if (best) *best = best_t;
+  return MI_OK;
+}
+
+int mi_autotune_sizes(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, uint32_t policy,
+                      const uint32_t *sizes, uint32_t nsizes, uint32_t reps, mi_tune_entry *best) {
+  if (!ctx) return MI_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  int rc;
+  if ((rc = bad_policy(ctx, policy))) return rc;
+  if (npos == 0 || !pos || (nsizes && !sizes))
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "no positions or sizes");
+  if (reps == 0) reps = 3;
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
+  for (uint32_t i = 0; i < nsizes; i++) {
+    if (sizes[i] == 0) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "a batch size is 0");
+    // the first positions, each with the same playout count, making up the size (rounded down)
+    const uint32_t n = sizes[i] < npos ? sizes[i] : npos;
+    const uint32_t per = sizes[i] / n;
+    uint64_t total = 0;
+    if ((rc = plan_batch(ctx, pos, n, nullptr, per, total))) return rc;
+    if ((rc = upload_batch(ctx, pos, n, nullptr, total))) return rc;
+    ctx->staged = false;
+    mi_tuning t = ctx->tuning;
+    float ms = 0.f;
+    if ((rc = tune_sweep(ctx, n, per, total, false, policy, reps, nullptr, 0, nullptr, t, ms)))
+      return rc;
+    if (best) {
+      best[i].policy = policy;
+      best[i].bucket = tune_bucket(total);
+      best[i].tuning = t;
+      best[i].kernel_ms = ms;
+      best[i].playouts = (uint32_t)total;
     }
   }
-  ctx->tuning = best_t;
-  if (nresults) *nresults = n;
-  if (best) *best = best_t;
+  return MI_OK;
+}
+
+int mi_get_tuning_table(const mi_ctx *ctx, mi_tune_entry *out, uint32_t cap, uint32_t *count) {
+  if (!ctx || !count || (cap && !out)) return MI_ERR_INVALID_ARGUMENT;
+  uint32_t n = 0;
+  for (uint32_t pol = 0; pol < 2; pol++)
+    for (uint32_t b = 0; b < kTuneBuckets; b++)
+      for (uint32_t r = 0; r < 2; r++) {
+        const TuneEntry &e = ctx->tune_table[pol][b][r];
+        if (e.kernel_ms < 0.f) continue;
+        if (n < cap) {
+          out[n].policy = pol == 1 ? MI_POLICY_HEURISTIC : MI_POLICY_RANDOM;
+          out[n].bucket = b;
+          out[n].tuning = e.tuning;
+          out[n].kernel_ms = e.kernel_ms;
+          out[n].playouts = e.playouts;
+        }
+        n++;
+      }
+  *count = n;
   return MI_OK;
 }
 
@@ -808,12 +912,14 @@ int mi_graph_create(mi_ctx *ctx, uint32_t max_positions, uint32_t max_playouts, 
   a.keys = nullptr;
   a.winners = (int8_t *)((char *)g->d_out + sizeof(gpuai::WorkWords));
   a.work = work;
-  a.defer_min = ctx->tuning.defer_min;
+  // the shape of a full batch; each replay takes its own batch's shape with this refill policy
+  const mi_tuning t = shape_for(ctx, policy, max_playouts);
+  a.defer_min = t.defer_min;
   a.desc = (const gpuai::LaunchDesc *)g->d_blob;
-  g->block = ctx->tuning.block_size;
-  g->refill = ctx->tuning.refill;
-  g->resident = ctx->cus * ctx->tuning.blocks_per_cu;
-  g->grid = grid_for(ctx, max_playouts);
+  g->block = t.block_size;
+  g->refill = t.refill;
+  g->resident = ctx->cus * t.blocks_per_cu;
+  g->grid = grid_for(ctx, t, max_playouts);
   void *reset_args[] = {&work};
   void *play_args[] = {&a};
   hipKernelNodeParams rp;
@@ -882,19 +988,24 @@ int mi_graph_launch(mi_graph *g, const mi_pos *pos, uint32_t npos, const uint32_
     const size_t out_bytes = sizeof(gpuai::WorkWords) + (size_t)total;
     MI_HIP(ctx, hipGraphExecMemcpyNodeSetParams1D(g->exec, g->n_d2h, g->h_out, g->d_out, out_bytes,
                                                   hipMemcpyDeviceToHost));
-    // both refill policies are correct for any grid: static refill strides by the grid's threads
-    const uint64_t need = (total + g->block - 1u) / g->block;
-    const uint32_t grid = gpuai::grid_size(need, g->resident);
-    if (grid != g->grid) {
+    // the batch's shape from the tuning table (the graph's refill policy; both are correct for
+    // any grid: static refill strides by the grid's threads)
+    const mi_tuning t = shape_for(ctx, g->policy, total, (int)g->refill);
+    const uint64_t need = (total + t.block_size - 1u) / t.block_size;
+    const uint32_t grid = gpuai::grid_size(need, (uint64_t)ctx->cus * t.blocks_per_cu);
+    if (grid != g->grid || t.block_size != g->block || t.defer_min != g->args.defer_min) {
+      g->args.defer_min = t.defer_min;
       void *play_args[] = {&g->args};
       hipKernelNodeParams pp;
       std::memset(&pp, 0, sizeof(pp));
       pp.func = const_cast<void *>(gpuai::playout_kernel(g->policy, g->refill));
       pp.gridDim = dim3(grid);
-      pp.blockDim = dim3(g->block);
+      pp.blockDim = dim3(t.block_size);
       pp.kernelParams = play_args;
       MI_HIP(ctx, hipGraphExecKernelNodeSetParams(g->exec, g->n_play, &pp));
       g->grid = grid;
+      g->block = t.block_size;
+      g->resident = ctx->cus * t.blocks_per_cu;
     }
     MI_HIP(ctx, hipEventRecord(g->e0, ctx->stream));
     MI_HIP(ctx, hipMemcpyAsync(g->d_blob, g->h_blob, in_bytes, hipMemcpyHostToDevice, ctx->stream));

@@ -27,11 +27,21 @@ struct HostBuf {
 // instead of sharing a few SIMDs.
 constexpr mi_tuning kDefaultTuning = {64u, 12u, MI_REFILL_QUEUE, 16u};
 
+// Launch-shape table: the fastest shape the autotuner measured for a policy, a batch-size bucket
+// (floor(log2(playouts))) and a refill policy; kernel_ms < 0 marks an empty entry.
+constexpr uint32_t kTuneBuckets = 32;
+struct TuneEntry {
+  mi_tuning tuning = {0u, 0u, 0u, 0u};
+  float kernel_ms = -1.f;
+  uint32_t playouts = 0;  // size of the batch it was measured on
+};
+
 struct mi_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-  mi_tuning tuning = kDefaultTuning;
+  mi_tuning tuning = kDefaultTuning;  // the shape of batches whose bucket the table lacks
+  TuneEntry tune_table[2][kTuneBuckets][2];  // [policy][bucket][refill]
   uint32_t cus = 0, wave = 64, clock_khz = 0;
   gpuai::DevBuf pos, offs, winners, plies, hashes, work, tallies, moves, counts, keys;
   // longest-first order of the uploaded batch (see upload_batch): positions permuted into it,

@@ -45,6 +45,17 @@ class MiTuneResult(ctypes.Structure):
                 ("playouts_per_s", ctypes.c_float)]
 
 
+class MiTuneEntry(ctypes.Structure):
+    _fields_ = [("policy", ctypes.c_uint32), ("bucket", ctypes.c_uint32), ("tuning", MiTuning),
+                ("kernel_ms", ctypes.c_float), ("playouts", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        t = self.tuning
+        return {"policy": self.policy, "bucket": self.bucket, "block_size": t.block_size,
+                "blocks_per_cu": t.blocks_per_cu, "refill": t.refill, "defer_min": t.defer_min,
+                "kernel_ms": self.kernel_ms, "playouts": self.playouts}
+
+
 class MiStats(ctypes.Structure):
     _fields_ = [("playouts", ctypes.c_uint64), ("plies", ctypes.c_uint64),
                 ("kernel_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
@@ -89,6 +100,8 @@ _SIGS = {
     "mi_gen_start_positions": ([_P, _U32, _U32, _U32, _U32, _U32, _P], ctypes.c_int),
     "mi_autotune": ([_P, _P, _U32, _P, _U32, _U32, _U32, _P, _U32, _P, ctypes.POINTER(MiTuning)],
                     ctypes.c_int),
+    "mi_autotune_sizes": ([_P, _P, _U32, _U32, _P, _U32, _U32, _P], ctypes.c_int),
+    "mi_get_tuning_table": ([_P, _P, _U32, _P], ctypes.c_int),
     "mi_graph_create": ([_P, _U32, _U32, _U32, ctypes.POINTER(_P)], ctypes.c_int),
     "mi_graph_run": ([_P, _P, _U32, _P, _P, _U32, _U32, _P, ctypes.POINTER(MiStats)],
                      ctypes.c_int),
@@ -336,6 +349,24 @@ class Engine:
                 for r in res[:min(nres.value, 256)]]
         return ({"block_size": best.block_size, "blocks_per_cu": best.blocks_per_cu,
                  "refill": best.refill, "defer_min": best.defer_min}, rows)
+
+    def autotune_sizes(self, pos, sizes, policy: int = POLICY_RANDOM, reps: int = 3) -> list[dict]:
+        """Fills the launch-shape table for these batch sizes (the first min(size, len(pos))
+        positions, equal playout counts); returns the fastest shape of each size."""
+        pos = as_positions(pos)
+        sz = np.ascontiguousarray(sizes, dtype=np.uint32)
+        best = (MiTuneEntry * len(sz))()
+        self._check(self.lib.mi_autotune_sizes(self.ctx, _ptr(pos), len(pos), policy, _ptr(sz),
+                                               len(sz), reps, best))
+        return [b.as_dict() for b in best]
+
+    def tuning_table(self) -> list[dict]:
+        """The launch-shape table's entries (policy, bucket, refill order)."""
+        n = _U32()
+        self._check(self.lib.mi_get_tuning_table(self.ctx, None, 0, ctypes.byref(n)))
+        out = (MiTuneEntry * max(n.value, 1))()
+        self._check(self.lib.mi_get_tuning_table(self.ctx, out, n.value, ctypes.byref(n)))
+        return [e.as_dict() for e in out[:n.value]]
 
     def graph(self, max_positions: int, max_playouts: int,
               policy: int = POLICY_RANDOM) -> "GraphBatch":

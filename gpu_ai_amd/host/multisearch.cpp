@@ -52,14 +52,25 @@ void multiSearchSetup(const MultiSearchConfig &cfg, std::vector<mi_pos> &roots,
 
 namespace {
 
-// The launch shape for the searches' batches: mi_autotune on a batch of the same kind as an
-// iteration's (every root with `initial` playouts), or the context's default.
-mi_tuning tuneSearches(mi_ctx *ctx, const MultiSearchConfig &cfg, const std::vector<mi_pos> &roots) {
+// The launch shapes for the searches' batches: the context's launch-shape table filled for every
+// size an iteration's batch can take -- one search's `initial` trials, doubling up to `maxBatch`
+// (every search at its largest iteration) -- tuned on batches of the roots; or the context's
+// default shape.  Returns the shape of the largest batch (reported with the results).
+mi_tuning tuneSearches(mi_ctx *ctx, const MultiSearchConfig &cfg, const std::vector<mi_pos> &roots,
+                       uint64_t maxBatch) {
   mi_tuning t = {};
   if (cfg.autotune) {
-    checkMi(mi_autotune(ctx, roots.data(), (uint32_t)roots.size(), nullptr,
-                        std::max<uint32_t>(cfg.initial, 1u), cfg.policy, 1, nullptr, 0, nullptr, &t),
-            ctx, "mi_autotune");
+    const uint64_t top = std::min<uint64_t>(std::max<uint64_t>(maxBatch, 1), (1u << 31) - 1);
+    std::vector<uint32_t> sizes;
+    for (uint64_t s = std::max<uint32_t>(cfg.initial, 1u);; s *= 2) {
+      sizes.push_back((uint32_t)std::min(s, top));
+      if (s >= top) break;
+    }
+    std::vector<mi_tune_entry> best(sizes.size());
+    checkMi(mi_autotune_sizes(ctx, roots.data(), (uint32_t)roots.size(), cfg.policy, sizes.data(),
+                              (uint32_t)sizes.size(), 1, best.data()),
+            ctx, "mi_autotune_sizes");
+    t = best.back().tuning;
   } else {
     checkMi(mi_get_tuning(ctx, &t), ctx, "mi_get_tuning");
   }
@@ -218,16 +229,16 @@ MultiSearchResult runPipelined(const MultiSearchConfig &cfg,
     Gk.driver = std::make_unique<DeviceMI355XPlayoutDriver>(cfg.policy, cfg.device,
                                                             cfg.playoutSeed);
   }
-  // one launch shape for every group (a graph keeps the shape it was created with)
+  // each group's context gets its own launch-shape table (each batch then runs with its size's
+  // shape; a graph keeps the refill policy it was created with)
   MultiSearchResult res;
-  res.tuning = tuneSearches(groups[0]->driver->context(), cfg, roots);
   for (uint32_t k = 0; k < ngroups; k++) {
     SearchGroup &Gk = *groups[k];
-    checkMi(mi_set_tuning(Gk.driver->context(), &res.tuning), Gk.driver->context(),
-            "mi_set_tuning");
     uint64_t cap = 0;
     for (uint32_t s : Gk.ids) cap += iterationCap(cfg, remaining[s]);
     cap = std::max<uint64_t>(cap, 1);
+    const mi_tuning t = tuneSearches(Gk.driver->context(), cfg, roots, cap);
+    if (k == 0) res.tuning = t;
     checkMi(mi_graph_create(Gk.driver->context(), (uint32_t)std::min<uint64_t>(cap, 1u << 22),
                             (uint32_t)std::min<uint64_t>(cap, (1u << 31) - 1), cfg.policy,
                             &Gk.graph),
@@ -327,13 +338,16 @@ MultiSearchResult runMultiSearch(const MultiSearchConfig &cfg) {
     return res;
   }
   DeviceMI355XPlayoutDriver driver(cfg.policy, cfg.device, cfg.playoutSeed);
-  const mi_tuning tuning = tuneSearches(driver.context(), cfg, roots);
+  // the largest batch: every search's largest possible iteration
+  uint64_t cap = 0;
+  for (uint32_t b : budgets) cap += iterationCap(cfg, b);
+  cap = std::max<uint64_t>(cap, 1);
+  const mi_tuning tuning = tuneSearches(driver.context(), cfg, roots,
+                                        cfg.mode == MultiSearchMode::PER_SEARCH
+                                            ? iterationCap(cfg, cfg.maxBudget)
+                                            : cap);
   mi_graph *graph = nullptr;
   if (cfg.mode == MultiSearchMode::GRAPH) {
-    // the largest batch: every search's largest possible iteration
-    uint64_t cap = 0;
-    for (uint32_t b : budgets) cap += iterationCap(cfg, b);
-    cap = std::max<uint64_t>(cap, 1);
     checkMi(mi_graph_create(driver.context(), (uint32_t)std::min<uint64_t>(cap, 1u << 22),
                             (uint32_t)std::min<uint64_t>(cap, (1u << 31) - 1), cfg.policy, &graph),
             driver.context(), "mi_graph_create");

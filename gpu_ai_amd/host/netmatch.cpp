@@ -71,6 +71,13 @@ NetMatchResult runNetMatch(const NetMatchConfig &cfg, const float *params) {
   if (cfg.games == 0 || cfg.games % 2) throw std::invalid_argument("games must be even and > 0");
   DeviceMI355XPlayoutDriver driver(MI_POLICY_RANDOM, cfg.device, cfg.playoutSeed);
   mi_ctx *ctx = driver.context();
+  // the playout player's batches (leafBatch leaves over the games of a phase, fewer as games
+  // end) each run with their size's fastest launch shape
+  {
+    std::vector<uint32_t> sizes;
+    for (uint32_t s = std::max<uint32_t>(cfg.leafBatch, 1u); s >= 16u; s /= 4) sizes.push_back(s);
+    driver.tune(sizes);
+  }
   NetLeafEvaluator net(ctx, params, cfg.netBatch);
 
   // openings: one per pair of games

@@ -21,6 +21,16 @@ DeviceMI355XPlayoutDriver::DeviceMI355XPlayoutDriver(uint32_t policy, int device
 
 DeviceMI355XPlayoutDriver::~DeviceMI355XPlayoutDriver() { mi_destroy(ctx_); }
 
+void DeviceMI355XPlayoutDriver::tune(const std::vector<uint32_t> &sizes) {
+  if (sizes.empty()) return;
+  std::vector<mi_pos> pos(1024);
+  checkMi(mi_gen_start_positions(ctx_, 2016, (uint32_t)pos.size(), 0, 30, 0, pos.data()), ctx_,
+          "mi_gen_start_positions");
+  checkMi(mi_autotune_sizes(ctx_, pos.data(), (uint32_t)pos.size(), policy_, sizes.data(),
+                            (uint32_t)sizes.size(), 1, nullptr),
+          ctx_, "mi_autotune_sizes");
+}
+
 void DeviceMI355XPlayoutDriver::runGrouped(const std::vector<mi_pos> &pos,
                                            const std::vector<uint32_t> &counts,
                                            std::vector<int8_t> &winners, mi_stats *stats) {
@@ -71,9 +81,17 @@ std::unique_ptr<PlayoutDriver> getPlayoutDriver(std::string name) {
   static const char *heuristic_names[] = {"host_heuristic", "device_heuristic", "hybrid_heuristic",
                                           "hybrid_heruistic", "optimal_heuristic",
                                           "device_mi355x_heuristic"};
+  uint32_t policy = ~0u;
   for (const char *n : random_names)
-    if (name == n) return std::make_unique<DeviceMI355XPlayoutDriver>(MI_POLICY_RANDOM);
+    if (name == n) policy = MI_POLICY_RANDOM;
   for (const char *n : heuristic_names)
-    if (name == n) return std::make_unique<DeviceMI355XPlayoutDriver>(MI_POLICY_HEURISTIC);
-  throw std::runtime_error("Unknown playout type");
+    if (name == n) policy = MI_POLICY_HEURISTIC;
+  if (policy == ~0u) throw std::runtime_error("Unknown playout type");
+  auto d = std::make_unique<DeviceMI355XPlayoutDriver>(policy);
+  if (name == "optimal" || name == "optimal_heuristic") {
+    std::vector<uint32_t> sizes;
+    for (uint32_t s = 1u << 6; s <= (1u << 20); s <<= 2) sizes.push_back(s);
+    d->tune(sizes);
+  }
+  return d;
 }

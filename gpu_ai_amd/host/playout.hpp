@@ -47,6 +47,9 @@ class DeviceMI355XPlayoutDriver : public PlayoutDriver {
   void runKeyed(const std::vector<mi_pos> &pos, const std::vector<uint32_t> &counts,
                 const std::vector<uint32_t> &keys, uint32_t epoch, std::vector<int8_t> &winners,
                 mi_stats *stats = nullptr);
+  // Fills the context's launch-shape table for these batch sizes (mi_autotune_sizes on seeded
+  // positions: k in [0, 30] random plies), so that each batch runs with its size's fastest shape.
+  void tune(const std::vector<uint32_t> &sizes);
   mi_ctx *context() const { return ctx_; }
   uint32_t nextEpoch() { return epoch_++; }
   uint32_t seed() const { return seed_; }
@@ -62,7 +65,10 @@ class DeviceMI355XPlayoutDriver : public PlayoutDriver {
 // Name -> driver.  The reference's driver names (src/playout.cpp:189-223) all select the MI355X
 // engine: host, device_single, device_multiple, device_coarse, hybrid and optimal run random
 // playouts; host_heuristic, device_heuristic, hybrid_heuristic and optimal_heuristic run
-// heuristic playouts.  device_mi355x and device_mi355x_heuristic name them directly.
+// heuristic playouts.  device_mi355x and device_mi355x_heuristic name them directly.  The
+// optimal names, whose reference driver picks a driver by batch size (src/playout.cpp:80-166),
+// get a driver whose launch-shape table is tuned at construction for batches of 2^6 to 2^20
+// playouts (about a second).
 // Unknown names throw std::runtime_error("Unknown playout type").
 std::unique_ptr<PlayoutDriver> getPlayoutDriver(std::string name);
 

@@ -102,7 +102,8 @@ int mi_destroy(mi_ctx *ctx);
 const char *mi_strerror(int code);
 /* Text of the last error raised on this context (empty if none). */
 const char *mi_last_error(const mi_ctx *ctx);
-/* Sets the launch shape; zero fields take the defaults of mi_create. */
+/* Sets the launch shape, for every batch size: the launch-shape table (see mi_autotune) is
+ * emptied.  Zero fields take the defaults of mi_create. */
 int mi_set_tuning(mi_ctx *ctx, const mi_tuning *tuning);
 int mi_get_tuning(const mi_ctx *ctx, mi_tuning *tuning);
 /* Device facts: compute units, wave size, clock in kHz. */
@@ -161,19 +162,52 @@ int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin
                            uint32_t reject_terminal, mi_pos *out);
 
 /* Autotuner (no counterpart in the reference, whose OptimalPlayoutDriver picks between whole
- * drivers by a runtime model, src/playout.cpp:80-166).  Runs the given batch under every
- * candidate launch shape -- workgroup sizes 64..1024, 1..8 resident waves per SIMD (which sets the
- * playouts each lane runs), both refill policies and deferral thresholds 1, 8 and 16 -- `reps`
- * times each, and makes the one with the smallest median kernel time the context's tuning.  `results` (may be NULL) receives up to
- * `max_results` candidates; `nresults` the number measured; `best` (may be NULL) the winner. */
+ * drivers by a runtime model of the batch size, src/playout.cpp:80-166).  Runs the given batch
+ * under every candidate launch shape -- workgroup sizes 64..1024, 1..8 resident waves per SIMD
+ * (which sets the playouts each lane runs), both refill policies and deferral thresholds 1, 8 and
+ * 16 -- `reps` times each and takes the one with the smallest median kernel time.
+ *
+ * The context keeps a launch-shape table: per policy, batch-size bucket (batches of 2^b to
+ * 2^(b+1) - 1 playouts) and refill policy, the fastest shape measured.  Every tuning run records
+ * its batch's bucket there; mi_run_playouts (and the keyed, traced, tally and staged launches)
+ * and graph replays then launch each batch with its bucket's shape (graph replays: the fastest of
+ * the graph's refill policy), and batches whose bucket has no entry with the context's tuning.
+ * mi_set_tuning empties the table.  The allocation-free entry point always uses the context's
+ * tuning (its batch size is on the device).
+ *
+ * mi_autotune tunes on one batch and also makes the winner the context's tuning.  `results` (may
+ * be NULL) receives up to `max_results` candidates; `nresults` the number measured; `best` (may
+ * be NULL) the winner. */
 int mi_autotune(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, const uint32_t *counts,
                 uint32_t per_pos, uint32_t policy, uint32_t reps, mi_tune_result *results,
                 uint32_t max_results, uint32_t *nresults, mi_tuning *best);
 
+/* One entry of the launch-shape table. */
+typedef struct mi_tune_entry {
+  uint32_t policy;      /* MI_POLICY_* */
+  uint32_t bucket;      /* batches of 2^bucket .. 2^(bucket+1) - 1 playouts */
+  mi_tuning tuning;     /* the fastest shape measured (its refill field names the refill policy) */
+  float kernel_ms;      /* its median kernel time on the measured batch */
+  uint32_t playouts;    /* size of the measured batch */
+} mi_tune_entry;
+
+/* Fills the table for several batch sizes: for each size S, the first min(S, npos) positions
+ * with S / min(S, npos) playouts each (S rounded down to a multiple of the count) are tuned as
+ * mi_autotune tunes a batch.  The context's tuning is left as it was.  `best` (may be NULL)
+ * receives one entry per size: the fastest shape over both refill policies. */
+int mi_autotune_sizes(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, uint32_t policy,
+                      const uint32_t *sizes, uint32_t nsizes, uint32_t reps, mi_tune_entry *best);
+
+/* The table's entries (policy, bucket, refill order); `count` receives their number, of which
+ * up to `cap` are written. */
+int mi_get_tuning_table(const mi_ctx *ctx, mi_tune_entry *out, uint32_t cap, uint32_t *count);
+
 /* Graph-replayed batches for search loops (no counterpart in the reference, which launches,
  * copies and frees per call).  mi_graph_create fixes the capacities and builds one hipGraph for
- * the context's stream: a reset kernel, the persistent playout kernel with the context's tuning
- * (its grid sized to each batch), and the copy of the winners back to pinned memory.
+ * the context's stream: a reset kernel, the persistent playout kernel (the launch shape of a full
+ * batch, whose refill policy the graph keeps; each replay takes its batch's shape from the
+ * launch-shape table, see mi_autotune, and its grid is sized to the batch), and the copy of the
+ * winners back to pinned memory.
  * mi_graph_run packs the batch (seed, epoch, positions, playout offsets and, when keys is not
  * NULL, stream keys) into pinned memory, copies it to the device with one asynchronous copy on
  * the stream, replays the graph and returns the winners as mi_run_playouts_keyed would (same
