@@ -29,11 +29,12 @@ struct mi_mlp {
   // parameters, Adam moments, gradients (FP32) and BF16 operand copies
   float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
   uint16_t *W1t = nullptr, *W2t = nullptr, *W2r = nullptr;
-  // activations of one batch
-  uint16_t *X = nullptr, *H1 = nullptr, *H2 = nullptr, *dH2 = nullptr, *dH1 = nullptr;
-  float *y = nullptr, *dz = nullptr, *targets = nullptr, *loss = nullptr;
-  float *pW1 = nullptr, *pW2 = nullptr;  // split-K partials of dW1, dW2
-  float *pV = nullptr;                    // column-sum partials of b1, b2, w3, b3
+  // one batch: network outputs, targets, loss; the transposed activations of a training step
+  // ([feature][max_rows]) and its per 32-row block partials of the vector gradients
+  float *y = nullptr, *targets = nullptr, *loss = nullptr;
+  uint16_t *XT = nullptr, *H1T = nullptr, *dH2T = nullptr, *dH1T = nullptr;
+  float *P = nullptr;  // per 256-row split partials of dW1 and dW2
+  float *pV = nullptr;
   mi_pos *pos = nullptr;
   // pinned host staging of one batch: positions (max_rows), then values and targets (max_rows
   // floats each); copies from and to the caller's pageable buffers go through it, so the DMA
@@ -63,21 +64,25 @@ float *stage_targets(mi_mlp *n) { return stage_values(n) + n->max_rows; }
 
 // Rows per split of the weight-gradient GEMMs (K = rows): 128, or 64 when rows is an odd
 // multiple of 64, so a 4096-row batch gives 32 splits (512 and 256 workgroups for dW2, dW1).
-uint32_t split_len(uint32_t rows) { return rows % 128 == 0 ? 128 : 64; }
 
-K::AdamArgs adam_args(mi_mlp *n, uint32_t splits, uint32_t vsplits, float lr, float c1, float c2,
-                      int update, float *loss_out = nullptr) {
+K::AdamArgs adam_args(mi_mlp *n, uint32_t rows, float lr, float c1, float c2, int update,
+                      float *loss_out = nullptr) {
   K::AdamArgs a = {};
   a.loss_out = loss_out;
   a.w = n->w;
   a.m = n->m;
   a.v = n->v;
   a.grads = n->g;
-  a.pW1 = n->pW1;
-  a.pW2 = n->pW2;
-  a.splits = splits;
+  a.XT = n->XT;
+  a.H1T = n->H1T;
+  a.dH2T = n->dH2T;
+  a.dH1T = n->dH1T;
+  a.rows = rows;
+  a.ld = n->max_rows;
+  a.P = n->P;
+  a.splits = (rows + 255) / 256;
   a.pV = n->pV;
-  a.vsplits = vsplits;
+  a.blocks = rows / 64;
   a.W1t = n->W1t;
   a.W2t = n->W2t;
   a.W2r = n->W2r;
@@ -128,57 +133,39 @@ int upload_params(mi_mlp *n, const float *w) {
   MI_HIP(ctx, K::zero(n->v, MI_MLP_PARAMS, ctx->stream));
   MI_HIP(ctx, K::zero(n->g, MI_MLP_PARAMS, ctx->stream));
   // refresh the BF16 copies without updating (no partials: every gradient reads as 0)
-  MI_HIP(ctx, K::adam(adam_args(n, 0, 0, 0.f, 1.f, 1.f, 0), ctx->stream));
+  MI_HIP(ctx, K::wgrad_adam(adam_args(n, 0, 0.f, 1.f, 1.f, 0), ctx->stream));
   MI_HIP(ctx, hipStreamSynchronize(ctx->stream));
   n->step = 0;
   return MI_OK;
 }
 
-// Forward pass of `rows` encoded rows (X already filled); H1, H2 and y.
-int forward_rows(mi_mlp *n, uint32_t valid, uint32_t rows) {
+// One training step over `rows` rows (count valid): k_train_rows' forward and backward pass to
+// the transposed activations and vector partials, then the weight-gradient partials and Adam
+// (k_wgrad_part, k_adam; when update == 0, only the gradient sums); loss_out (device, may be
+// null) receives the batch loss.  Row r is
+// pos[idx ? idx[r] : r] with target targets[idx ? idx[r] : r].
+int train_rows(mi_mlp *n, const mi_pos *pos, const uint32_t *idx, const float *targets,
+               uint32_t count, uint32_t rows, float lr, int update, float *loss_out) {
   mi_ctx *ctx = n->ctx;
-  K::GemmArgs l1 = {};
-  l1.A = n->X; l1.sAm = 128; l1.sAk = 1;
-  l1.B = n->W1t; l1.sBk = 1; l1.sBn = 128;
-  l1.M = (int)rows; l1.N = 256; l1.k_per_split = 128;
-  l1.ldc = 256; l1.Cbf = n->H1; l1.bias = n->w + MI_MLP_OFF_B1; l1.relu = 1;
-  MI_HIP(ctx, K::gemm(l1, 1, true, true, ctx->stream));
-  K::GemmArgs l2 = {};
-  l2.A = n->H1; l2.sAm = 256; l2.sAk = 1;
-  l2.B = n->W2t; l2.sBk = 1; l2.sBn = 256;
-  l2.M = (int)rows; l2.N = 256; l2.k_per_split = 256;
-  l2.ldc = 256; l2.Cbf = n->H2; l2.bias = n->w + MI_MLP_OFF_B2; l2.relu = 1;
-  MI_HIP(ctx, K::gemm(l2, 1, true, true, ctx->stream));
-  (void)valid;
-  return MI_OK;
-}
-
-// Backward pass after the forward pass and the training head (which left dz, dH2 and the b2,
-// w3, b3 and loss partials): every other gradient, then Adam (or only the gradient sums when
-// update == 0); loss_out (device, may be null) receives the batch loss.
-int backward_rows(mi_mlp *n, uint32_t rows, float lr, int update, float *loss_out) {
-  mi_ctx *ctx = n->ctx;
-  hipStream_t s = ctx->stream;
-  const uint32_t kps = split_len(rows), splits = rows / kps;
-  K::GemmArgs w2 = {};  // dW2[i][o] = sum_b H1[b][i] dH2[b][o]
-  w2.A = n->H1; w2.sAm = 1; w2.sAk = 256;
-  w2.B = n->dH2; w2.sBk = 256; w2.sBn = 1;
-  w2.M = 256; w2.N = 256; w2.k_per_split = (int)kps;
-  w2.C = n->pW2; w2.split_stride = 65536; w2.ldc = 256;
-  MI_HIP(ctx, K::gemm(w2, (int)splits, true, true, s));
-  K::GemmArgs h1 = {};  // dH1[b][i] = sum_o dH2[b][o] W2[i][o], masked by H1 > 0
-  h1.A = n->dH2; h1.sAm = 256; h1.sAk = 1;
-  h1.B = n->W2r; h1.sBk = 1; h1.sBn = 256;
-  h1.M = (int)rows; h1.N = 256; h1.k_per_split = 256;
-  h1.ldc = 256; h1.Cbf = n->dH1; h1.mask = n->H1; h1.ldm = 256;
-  MI_HIP(ctx, K::gemm(h1, 1, true, true, s));
-  MI_HIP(ctx, K::colsum_bf16(n->dH1, rows, 256, nullptr, n->pV, s));       // b1
-  K::GemmArgs w1 = {};  // dW1[j][i] = sum_b X[b][j] dH1[b][i]
-  w1.A = n->X; w1.sAm = 1; w1.sAk = 128;
-  w1.B = n->dH1; w1.sBk = 256; w1.sBn = 1;
-  w1.M = 128; w1.N = 256; w1.k_per_split = (int)kps;
-  w1.C = n->pW1; w1.split_stride = 32768; w1.ldc = 256;
-  MI_HIP(ctx, K::gemm(w1, (int)splits, true, true, s));
+  K::TrainArgs ta = {};
+  ta.pos = pos;
+  ta.idx = idx;
+  ta.targets = targets;
+  ta.n = count;
+  ta.ld = n->max_rows;
+  ta.W1t = n->W1t;
+  ta.W2t = n->W2t;
+  ta.W2r = n->W2r;
+  ta.b1 = n->w + MI_MLP_OFF_B1;
+  ta.b2 = n->w + MI_MLP_OFF_B2;
+  ta.w3 = n->w + MI_MLP_OFF_W3;
+  ta.b3 = n->w + MI_MLP_OFF_B3;
+  ta.XT = n->XT;
+  ta.H1T = n->H1T;
+  ta.dH2T = n->dH2T;
+  ta.dH1T = n->dH1T;
+  ta.pV = n->pV;
+  MI_HIP(ctx, K::train_rows(ta, rows, ctx->stream));
   // the bias corrections are formed in double precision, as Adam's usual implementations do
   float c1 = 1.f, c2 = 1.f;
   if (update) {
@@ -186,7 +173,7 @@ int backward_rows(mi_mlp *n, uint32_t rows, float lr, int update, float *loss_ou
     c1 = (float)(1.0 / (1.0 - std::pow(n->beta1, (double)n->step)));
     c2 = (float)(1.0 / (1.0 - std::pow(n->beta2, (double)n->step)));
   }
-  MI_HIP(ctx, K::adam(adam_args(n, splits, rows / 64, lr, c1, c2, update, loss_out), s));
+  MI_HIP(ctx, K::wgrad_adam(adam_args(n, rows, lr, c1, c2, update, loss_out), ctx->stream));
   return MI_OK;
 }
 
@@ -207,7 +194,7 @@ int upload_positions(mi_mlp *n, const mi_pos *pos, uint32_t count) {
   return MI_OK;
 }
 
-// Uploads a batch (positions, optional targets) and encodes it; returns the padded row count.
+// Uploads a batch (positions, optional targets); returns the padded row count.
 int stage_batch(mi_mlp *n, const mi_pos *pos, const float *targets, uint32_t count,
                 uint32_t &rows) {
   mi_ctx *ctx = n->ctx;
@@ -220,7 +207,6 @@ int stage_batch(mi_mlp *n, const mi_pos *pos, const float *targets, uint32_t cou
     MI_HIP(ctx, hipMemcpyAsync(n->targets, ht, sizeof(float) * count, hipMemcpyHostToDevice,
                                ctx->stream));
   }
-  MI_HIP(ctx, K::encode(n->pos, nullptr, count, rows, n->X, ctx->stream));
   return MI_OK;
 }
 
@@ -246,18 +232,15 @@ int mi_mlp_create(mi_ctx *ctx, uint32_t max_batch, uint32_t seed, mi_mlp **out) 
   if (e == hipSuccess) e = dev_alloc(n, &n->W1t, (size_t)128 * 256);
   if (e == hipSuccess) e = dev_alloc(n, &n->W2t, (size_t)256 * 256);
   if (e == hipSuccess) e = dev_alloc(n, &n->W2r, (size_t)256 * 256);
-  if (e == hipSuccess) e = dev_alloc(n, &n->X, R * 128);
-  if (e == hipSuccess) e = dev_alloc(n, &n->H1, R * 256);
-  if (e == hipSuccess) e = dev_alloc(n, &n->H2, R * 256);
-  if (e == hipSuccess) e = dev_alloc(n, &n->dH2, R * 256);
-  if (e == hipSuccess) e = dev_alloc(n, &n->dH1, R * 256);
   if (e == hipSuccess) e = dev_alloc(n, &n->y, R);
-  if (e == hipSuccess) e = dev_alloc(n, &n->dz, R);
   if (e == hipSuccess) e = dev_alloc(n, &n->targets, R);
   if (e == hipSuccess) e = dev_alloc(n, &n->loss, 1);
-  if (e == hipSuccess) e = dev_alloc(n, &n->pW1, (R / 64) * 32768);
-  if (e == hipSuccess) e = dev_alloc(n, &n->pW2, (R / 64) * 65536);
-  if (e == hipSuccess) e = dev_alloc(n, &n->pV, (R / 64) * K::kVecSlots);
+  if (e == hipSuccess) e = dev_alloc(n, &n->XT, 128 * R);
+  if (e == hipSuccess) e = dev_alloc(n, &n->H1T, 256 * R);
+  if (e == hipSuccess) e = dev_alloc(n, &n->dH2T, 256 * R);
+  if (e == hipSuccess) e = dev_alloc(n, &n->dH1T, 256 * R);
+  if (e == hipSuccess) e = dev_alloc(n, &n->P, ((R + 255) / 256) * (size_t)(32768 + 65536));
+  if (e == hipSuccess) e = dev_alloc(n, &n->pV, (R / 32) * K::kVecSlots);
   if (e == hipSuccess) e = dev_alloc(n, &n->pos, R);
   if (e == hipSuccess)
     e = hipHostMalloc(&n->h_stage, R * (sizeof(mi_pos) + 2 * sizeof(float)), hipHostMallocDefault);
@@ -347,10 +330,7 @@ static int grad_common(mi_mlp *n, const mi_pos *pos, const float *targets, uint3
   if ((rc = use_device(n))) return rc;
   uint32_t rows;
   if ((rc = stage_batch(n, pos, targets, count, rows))) return rc;
-  if ((rc = forward_rows(n, count, rows))) return rc;
-  MI_HIP(ctx, K::head_train(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3, n->targets, nullptr,
-                            count, rows, n->dz, n->dH2, n->pV, ctx->stream));
-  if ((rc = backward_rows(n, rows, lr, update, n->loss))) return rc;
+  if ((rc = train_rows(n, n->pos, nullptr, n->targets, count, rows, lr, update, n->loss))) return rc;
   if (grads)
     MI_HIP(ctx, hipMemcpyAsync(grads, n->g, sizeof(float) * MI_MLP_PARAMS, hipMemcpyDeviceToHost,
                                ctx->stream));
@@ -427,12 +407,9 @@ int mi_mlp_train_epoch(mi_mlp *n, uint32_t batch, float lr, uint32_t shuffle_see
   MI_HIP(ctx, hipEventRecord(n->e0, ctx->stream));
   for (uint32_t b = 0; b < nb; b++) {
     const uint32_t *idx = (const uint32_t *)n->ds_perm.ptr + (size_t)b * batch;
-    MI_HIP(ctx, K::encode((const mi_pos *)n->ds_pos.ptr, idx, batch, batch, n->X, ctx->stream));
-    if ((rc = forward_rows(n, batch, batch))) return rc;
-    MI_HIP(ctx, K::head_train(n->H2, n->w + MI_MLP_OFF_W3, n->w + MI_MLP_OFF_B3,
-                              (const float *)n->ds_t.ptr, idx, batch, batch, n->dz, n->dH2, n->pV,
-                              ctx->stream));
-    if ((rc = backward_rows(n, batch, lr, 1, losses + b))) return rc;
+    if ((rc = train_rows(n, (const mi_pos *)n->ds_pos.ptr, idx, (const float *)n->ds_t.ptr, batch,
+                         batch, lr, 1, losses + b)))
+      return rc;
   }
   MI_HIP(ctx, hipEventRecord(n->e1, ctx->stream));
   std::vector<float> l(nb);

@@ -1,14 +1,16 @@
 // mlp_kernels.hip -- gfx950 kernels of the 128-256-256-1 value network.
 //
-// k_gemm: C[M][N] = A[M][K] B[K][N] on v_mfma_f32_16x16x32_bf16.  A 256-thread workgroup owns a
-// 64x64 tile of C; its four waves own 32x32 quadrants (2x2 MFMA tiles of 16x16).  Each 32-deep
-// slice of K is staged through LDS as BF16 with K contiguous (A as [m][k], B as [n][k]), so every
-// lane feeds the MFMA with one 16-byte LDS read per operand: lane l supplies row l%16 of A and
-// column l%16 of B for eight K values of its lane group l/16 (the same eight for A and B, which
-// is all the dot product needs), and receives D[4*(l/16)+v][l%16], v = 0..3.  A and B are read
-// from global memory through arbitrary (row, column) strides, which gives the transposed
-// operands of the backward pass without transposing in memory.  Epilogues fuse bias, ReLU, the
-// ReLU mask of the backward pass, the BF16 store of activations and split-K partial sums.
+// MFMA v_mfma_f32_16x16x32_bf16 throughout: lane l supplies row l%16 of A and column l%16 of B
+// for eight K values of its lane group l/16 (the same eight for A and B, which is all the dot
+// product needs: every operand is stored with K contiguous, so each is one 16-byte read), and
+// receives D[4*(l/16)+v][l%16], v = 0..3.
+//
+// A training step is three launches.  k_train_rows runs the forward and backward pass of 64 rows
+// per workgroup up to the activation gradients, with the activations in LDS, and writes what the
+// weight gradients need transposed (X^T, H1^T, dH2^T, dH1^T: [feature][row], rows as K) plus each
+// block's partial sums of the vector gradients.  k_wgrad_part forms 64x64 tiles of dW1 and dW2
+// over 256-row splits of the batch (split-K: 24 tiles x 16 splits for 4096 rows), and k_adam sums
+// every gradient's partials in a fixed order and applies Adam, refreshing the BF16 copies.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -22,8 +24,6 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 64, BN = 64, BK = 32, LDSK = 40;  // LDS rows padded to 40 elements
-
 __device__ __forceinline__ uint16_t to_bf(float f) {
   uint32_t u = __float_as_uint(f);
   u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
@@ -31,137 +31,32 @@ __device__ __forceinline__ uint16_t to_bf(float f) {
 }
 __device__ __forceinline__ float from_bf(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// One thread's share of a 64 x 32 slice of a strided BF16 operand X(r, k) = X[r*sr + k*sk]: eight
-// values, with K contiguous (sk == 1: row t/4, k 8(t%4)..+7) or rows contiguous (sr == 1: k t/8,
-// rows 8(t%8)..+7), both one 16-byte load; anything else element by element.
-__device__ __forceinline__ uint4 load_slice(const uint16_t *__restrict__ X, long sr, long sk,
-                                            int r0, int k0) {
-  const int t = threadIdx.x;
-  if (sk == 1) {
-    return *(const uint4 *)(X + (long)(r0 + (t >> 2)) * sr + (k0 + (t & 3) * 8));
-  } else if (sr == 1) {
-    return *(const uint4 *)(X + (long)(r0 + (t & 7) * 8) + (long)(k0 + (t >> 3)) * sk);
-  }
-  uint16_t e[8];
-  const int kk = t >> 3, rr = (t & 7) * 8;
-#pragma unroll
-  for (int j = 0; j < 8; j++) e[j] = X[(long)(r0 + rr + j) * sr + (long)(k0 + kk) * sk];
-  uint4 v;
-  v.x = e[0] | ((uint32_t)e[1] << 16);
-  v.y = e[2] | ((uint32_t)e[3] << 16);
-  v.z = e[4] | ((uint32_t)e[5] << 16);
-  v.w = e[6] | ((uint32_t)e[7] << 16);
-  return v;
+// Workgroup barrier for exchanges through LDS only: this wave's LDS accesses complete, then
+// s_barrier.  __syncthreads() also waits for the wave's global stores to complete, which costs a
+// memory round trip after every store of activations or partial sums; nothing here reads back
+// what the workgroup stored to global memory.  The memory clobber keeps the compiler from moving
+// memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Stores a thread's share into the LDS tile S[row][k] (K contiguous, rows padded to LDSK).
-__device__ __forceinline__ void store_slice(uint16_t (*S)[LDSK], long sk, const uint4 &v) {
-  const int t = threadIdx.x;
-  if (sk == 1) {
-    *(uint4 *)&S[t >> 2][(t & 3) * 8] = v;
-    return;
-  }
-  const int kk = t >> 3, rr = (t & 7) * 8;
-  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    S[rr + 2 * j][kk] = (uint16_t)(w[j] & 0xFFFFu);
-    S[rr + 2 * j + 1][kk] = (uint16_t)(w[j] >> 16);
-  }
-}
-
-// C = A B over one 64x64 tile and one K split; the next 32-deep slice is loaded into registers
-// while the current one is multiplied.
-__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[BM][LDSK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][LDSK];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  const int kbeg = blockIdx.z * g.k_per_split;
-  const int kend = kbeg + g.k_per_split;
-  const uint16_t *A = (const uint16_t *)g.A;
-  const uint16_t *B = (const uint16_t *)g.B;
-  f32x4 acc[2][2];
-#pragma unroll
-  for (int r = 0; r < 2; r++)
-#pragma unroll
-    for (int c = 0; c < 2; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-  uint4 ra = load_slice(A, g.sAm, g.sAk, m0, kbeg);
-  uint4 rb = load_slice(B, g.sBn, g.sBk, n0, kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    store_slice(As, g.sAk, ra);
-    store_slice(Bs, g.sBk, rb);
-    __syncthreads();
-    if (k0 + BK < kend) {  // in flight during the MFMAs below
-      ra = load_slice(A, g.sAm, g.sAk, m0, k0 + BK);
-      rb = load_slice(B, g.sBn, g.sBk, n0, k0 + BK);
-    }
-    bf16x8 a[2], b[2];
-#pragma unroll
-    for (int r = 0; r < 2; r++)
-      a[r] = *(const bf16x8 *)&As[wr * 32 + r * 16 + (lane & 15)][(lane >> 4) * 8];
-#pragma unroll
-    for (int c = 0; c < 2; c++)
-      b[c] = *(const bf16x8 *)&Bs[wc * 32 + c * 16 + (lane & 15)][(lane >> 4) * 8];
-#pragma unroll
-    for (int r = 0; r < 2; r++)
-#pragma unroll
-      for (int c = 0; c < 2; c++)
-        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b[c], acc[r][c], 0, 0, 0);
-    __syncthreads();
-  }
-  float *C = g.C ? g.C + (long)blockIdx.z * g.split_stride : nullptr;
-#pragma unroll
-  for (int r = 0; r < 2; r++) {
-#pragma unroll
-    for (int c = 0; c < 2; c++) {
-      const int n = n0 + wc * 32 + c * 16 + (lane & 15);
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int m = m0 + wr * 32 + r * 16 + 4 * (lane >> 4) + v;
-        float x = acc[r][c][v];
-        if (g.bias) x += g.bias[n];
-        if (g.relu) x = x > 0.f ? x : 0.f;
-        if (g.mask) x = from_bf(g.mask[(long)m * g.ldm + n]) > 0.f ? x : 0.f;
-        if (g.row_valid && m >= g.row_valid) x = 0.f;
-        if (C) C[(long)m * g.ldc + n] = x;
-        if (g.Cbf) g.Cbf[(long)m * g.ldc + n] = to_bf(x);
-      }
-    }
-  }
-}
-
-// Positions -> BF16 input planes in the mover's frame; rows at or past n (padding) are zero.
-__global__ void __launch_bounds__(256) k_encode(const mi_pos *__restrict__ pos,
-                                               const uint32_t *__restrict__ idx, uint32_t n,
-                                               uint32_t rows, uint16_t *__restrict__ X) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (row, plane)
-  const uint32_t row = t >> 2, plane = t & 3;
-  if (row >= rows) return;
-  uint32_t bits = 0;
-  if (row < n) {
-    const mi_pos p = pos[idx ? idx[row] : row];
-    uint32_t own = p.turn ? __builtin_bitreverse32(p.p2) : p.p1;
-    uint32_t opp = p.turn ? __builtin_bitreverse32(p.p1) : p.p2;
-    uint32_t kings = p.turn ? __builtin_bitreverse32(p.kings) : p.kings;
-    kings &= own | opp;
-    bits = plane == 0 ? (own & ~kings) : plane == 1 ? (own & kings)
-         : plane == 2 ? (opp & ~kings) : (opp & kings);
-  }
-  uint16_t *dst = X + (size_t)row * 128 + plane * 32;
-#pragma unroll
-  for (int s = 0; s < 32; s++) dst[s] = ((bits >> s) & 1u) ? (uint16_t)0x3F80u : (uint16_t)0u;
+// The four input planes of a position in the mover's frame (own men, own kings, opponent's men,
+// opponent's kings), 32 squares each.
+__device__ __forceinline__ uint32_t plane_bits(const mi_pos &p, int plane) {
+  uint32_t own = p.turn ? __builtin_bitreverse32(p.p2) : p.p1;
+  uint32_t opp = p.turn ? __builtin_bitreverse32(p.p1) : p.p2;
+  uint32_t kings = p.turn ? __builtin_bitreverse32(p.kings) : p.kings;
+  kings &= own | opp;
+  return plane == 0 ? (own & ~kings) : plane == 1 ? (own & kings)
+       : plane == 2 ? (opp & ~kings) : (opp & kings);
 }
 
 // Inference in one launch: encoding, both hidden layers and the output layer for FR rows per
-// workgroup, with the activations in LDS.  The arithmetic is that of k_encode and k_gemm (bias,
-// ReLU, BF16 rounding of H1 and H2; the same 32-deep K slices accumulated in the same order by
-// the same MFMA), and the output layer sums four columns per lane and then over the wave with a
-// butterfly, so the values equal those of the layer-by-layer forward bit for bit (checked at
-// batches of 1 to 4096 rows).  Wave w computes output columns 64w..64w+63 of
-// each layer as four 16x16 tiles; the B operands (W1t, W2t: [out][in], in contiguous) are read
-// from global memory (they stay in L2), one 16-byte load per lane per tile and slice.
+// workgroup, with the activations in LDS: H1 = bf(relu(X W1 + b1)), H2 = bf(relu(H1 W2 + b2)),
+// y = tanh(H2 . w3 + b3), the output layer summing four columns per lane and then over the wave
+// with a butterfly.  Wave w computes output columns 64w..64w+63 of each layer as four 16x16
+// tiles; the B operands (W1t, W2t: [out][in], in contiguous) are read from global memory (they
+// stay in L2), one 16-byte load per lane per tile and slice.
 constexpr int FR = 16, XP = 128 + 8, HP = 256 + 8;  // rows; LDS row pitches (BF16 elements)
 
 __global__ void __launch_bounds__(256) k_forward(const mi_pos *__restrict__ pos, uint32_t n,
@@ -178,15 +73,7 @@ __global__ void __launch_bounds__(256) k_forward(const mi_pos *__restrict__ pos,
   {  // encode: thread t -> row t/16, plane (t/4)%4, squares 8(t%4)..+7
     const int row = t >> 4, plane = (t >> 2) & 3, s0 = (t & 3) * 8;
     uint32_t bits = 0;
-    if (r0 + row < n) {
-      const mi_pos p = pos[r0 + row];
-      uint32_t own = p.turn ? __builtin_bitreverse32(p.p2) : p.p1;
-      uint32_t opp = p.turn ? __builtin_bitreverse32(p.p1) : p.p2;
-      uint32_t kings = p.turn ? __builtin_bitreverse32(p.kings) : p.kings;
-      kings &= own | opp;
-      bits = plane == 0 ? (own & ~kings) : plane == 1 ? (own & kings)
-           : plane == 2 ? (opp & ~kings) : (opp & kings);
-    }
+    if (r0 + row < n) bits = plane_bits(pos[r0 + row], plane);
     bits >>= s0;
     uint32_t w[4];
 #pragma unroll
@@ -252,133 +139,275 @@ __global__ void __launch_bounds__(256) k_forward(const mi_pos *__restrict__ pos,
   }
 }
 
-// Output layer with its loss and gradients, over 64-row blocks (four waves of 16 rows; lane l
-// holds columns 4l..4l+3).  Per row: y = tanh(H2 . w3 + b3), e = y - t, dz = 2 e (1 - y^2) / n
-// (0 on padding rows) and dH2 = dz w3 masked by H2 > 0 (BF16).  Per block, into the partial
-// slots of block b (part + b * kVecSlots): b2 (256 + c) = sum of dH2, w3 (512 + c) = sum of
-// dz H2, b3 (768) = sum of dz, and the loss (769) = sum of e^2 / n.  No atomics: the partials
-// are summed in a fixed order by k_adam.
-__global__ void __launch_bounds__(256) k_head_train(const uint16_t *__restrict__ H2,
-                                                   const float *__restrict__ w3, const float *b3,
-                                                   const float *__restrict__ targets,
-                                                   const uint32_t *__restrict__ tidx, uint32_t n,
-                                                   float *__restrict__ dz,
-                                                   uint16_t *__restrict__ dH2,
-                                                   float *__restrict__ part) {
-  __shared__ float red_b2[4][256], red_w3[4][256], red_s[4][2];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  float wl[4], acc_b2[4] = {0.f, 0.f, 0.f, 0.f}, acc_w3[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int c = 0; c < 4; c++) wl[c] = w3[lane * 4 + c];
-  const float bias = b3[0];
-  float acc_b3 = 0.f, acc_loss = 0.f;
-  const uint32_t row0 = blockIdx.x * 64u + wv * 16u;
-  // the wave's 16 targets, fetched together up front (lane i holds row i's) instead of one
-  // dependent pair of loads per row
-  float tgt = 0.f;
-  if (lane < 16 && row0 + lane < n) tgt = targets[tidx ? tidx[row0 + lane] : row0 + lane];
-  // the 16 rows' activations, loaded together (4 BF16 per lane and row, one 8-byte load each);
-  // the row loop is unrolled so that they stay in registers
-  uint2 hrow[16];
-#pragma unroll
-  for (uint32_t i = 0; i < 16u; i++)
-    hrow[i] = *(const uint2 *)(H2 + (size_t)(row0 + i) * 256 + lane * 4);
-#pragma unroll
-  for (uint32_t i = 0; i < 16u; i++) {
-    const uint32_t row = row0 + i;
-    const float target = __shfl(tgt, (int)i);
-    const uint2 h = hrow[i];
-    const float hv[4] = {from_bf((uint16_t)(h.x & 0xFFFFu)), from_bf((uint16_t)(h.x >> 16)),
-                         from_bf((uint16_t)(h.y & 0xFFFFu)), from_bf((uint16_t)(h.y >> 16))};
-    float dot = 0.f;
-#pragma unroll
-    for (int c = 0; c < 4; c++) dot += hv[c] * wl[c];
-    for (int off = 32; off >= 1; off >>= 1) dot += __shfl_xor(dot, off);
-    const float yy = tanhf(dot + bias);
-    float d = 0.f;
-    if (row < n) {
-      const float e = yy - target;
-      d = 2.f * e * (1.f - yy * yy) / (float)n;
-      acc_loss += e * e / (float)n;
-    }
-    if (lane == 0) dz[row] = d;
-    acc_b3 += d;
-    uint16_t gb[4];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      gb[c] = to_bf(hv[c] > 0.f ? d * wl[c] : 0.f);
-      acc_b2[c] += from_bf(gb[c]);
-      acc_w3[c] += d * hv[c];
-    }
-    *(uint2 *)(dH2 + (size_t)row * 256 + lane * 4) =
-        uint2{gb[0] | ((uint32_t)gb[1] << 16), gb[2] | ((uint32_t)gb[3] << 16)};
-  }
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    red_b2[wv][lane * 4 + c] = acc_b2[c];
-    red_w3[wv][lane * 4 + c] = acc_w3[c];
-  }
-  if (lane == 0) {
-    red_s[wv][0] = acc_b3;
-    red_s[wv][1] = acc_loss;
-  }
-  __syncthreads();
-  float *pb = part + (size_t)blockIdx.x * kVecSlots;
-  const int t = threadIdx.x;
-  pb[256 + t] = red_b2[0][t] + red_b2[1][t] + red_b2[2][t] + red_b2[3][t];
-  pb[512 + t] = red_w3[0][t] + red_w3[1][t] + red_w3[2][t] + red_w3[3][t];
-  if (t == 0) {
-    pb[768] = red_s[0][0] + red_s[1][0] + red_s[2][0] + red_s[3][0];
-    pb[769] = red_s[0][1] + red_s[1][1] + red_s[2][1] + red_s[3][1];
+// ---------------------------------------------------------------------------------------------
+// Training.  Row r of the batch is pos[idx ? idx[r] : r] with target targets[idx ? idx[r] : r];
+// rows n and up (to the multiple of 32) are padding: zero inputs and no loss.  The arithmetic:
+// H1 = bf(relu(X W1 + b1)), H2 = bf(relu(H1 W2 + b2)), y = tanh(H2 . w3 + b3),
+// dz = 2 (y - t)(1 - y^2) / n, dH2 = bf(dz w3) where H2 > 0, dH1 = bf(dH2 W2^T) where H1 > 0;
+// dW1 = X^T dH1, dW2 = H1^T dH2, b1 and b2 the column sums of dH1 and dH2, w3 = sum of dz H2,
+// b3 = sum of dz, loss = sum of (y - t)^2 / n.
+
+constexpr int KR = 64;  // rows per workgroup of k_train_rows
+constexpr int TP = KR + 8;  // pitch of the transposed staging tile (XP and HP as k_forward's)
+
+// Copies the transposed staging tile T ([f][row], f < features) to G ([f][rows_total], this
+// block's 64 rows at r0): 128 bytes per feature, eight 16-byte stores.
+__device__ __forceinline__ void store_transposed(const uint16_t (*T)[TP], int features,
+                                                 uint16_t *__restrict__ G, uint32_t ld,
+                                                 uint32_t r0) {
+  for (int i = threadIdx.x; i < features * 8; i += 512) {
+    const int f = i >> 3, u = i & 7;
+    *(uint4 *)(G + (size_t)f * ld + r0 + 8 * u) = *(const uint4 *)&T[f][8 * u];
   }
 }
 
-// Partial column sums: block (x, y) writes part[y * part_stride + col] = sum over rows
-// [64 y, 64 y + 64) of weight[row] * src[row][col] (weight 1 when null), for 64 columns of a BF16
-// or FP32 source with `cols` columns.  The partials are summed in a fixed order by k_adam, so
-// the reduction is deterministic.
-template <typename T>
-__global__ void __launch_bounds__(256) k_colsum(const T *__restrict__ src, uint32_t rows,
-                                               uint32_t cols, const float *__restrict__ weight,
-                                               float *__restrict__ part, uint32_t part_stride) {
-  __shared__ float red[4][64];
-  const int c = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  const uint32_t col = blockIdx.x * 64 + c;
-  const uint32_t r0 = blockIdx.y * 64, r1 = min(r0 + 64u, rows);
-  float s = 0.f;
-  if (col < cols) {
-#pragma unroll 4
-    for (uint32_t r = r0 + grp; r < r1; r += 4) {
-      float v;
-      if constexpr (sizeof(T) == 2) v = from_bf((uint16_t)src[(size_t)r * cols + col]);
-      else v = (float)src[(size_t)r * cols + col];
-      s += weight ? weight[r] * v : v;
+// The forward and backward pass of 64 rows: wave w (of eight) owns columns 32w..32w+31 of each
+// 256-wide layer (two 16x16 column tiles, four row tiles).  Each workgroup reads the whole of the
+// BF16 weights (320 KB), so the rows per workgroup set the weight traffic through L2.  A operands come from LDS ([row][k]),
+// B operands from the BF16 weight copies in global memory (W1t, W2t: [out][in]; W2r: [in][out]).
+// Every global load is issued at the start (a dependent load waits out the memory latency, and
+// a layer's panel loaded slice by slice would wait it out once per 32-deep slice); the pass then
+// only computes and stores.
+__global__ void __launch_bounds__(512) k_train_rows(TrainArgs a) {
+  __shared__ __attribute__((aligned(16))) uint16_t Xs[KR][XP];
+  __shared__ __attribute__((aligned(16))) uint16_t H1s[KR][HP];
+  __shared__ __attribute__((aligned(16))) uint16_t dH2s[KR][HP];
+  __shared__ __attribute__((aligned(16))) uint16_t T[256][TP];  // transposed staging
+  __shared__ float red[8][KR];
+  __shared__ float dzs[KR];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int lr = lane & 15, kg = (lane >> 4) * 8;
+  const uint32_t r0 = blockIdx.x * KR, ld = a.ld;
+  float *pv = a.pV + (size_t)blockIdx.x * kVecSlots;
+  // the wave's columns, rotated by the block index: all workgroups read the same weights at the
+  // same time, and the rotation spreads their requests over different lines of L2
+  const int cw = ((wave + (int)blockIdx.x) & 7) * 32;
+  // Every global load the pass needs, issued before any is used and unconditionally (padding rows
+  // load row n - 1 and ignore it), in an order that lets each wait cover only what it needs --
+  // loads complete in issue order as far as the wait counter goes: the wave's W1t panel with the
+  // rows' dataset indices, then the positions and targets they point to, then the W2t and W2r
+  // panels and the biases.  The empty asm statements keep the compiler from reordering the
+  // groups (it would hoist the index-dependent address arithmetic, and its wait, to the top).
+  const int erow = t >> 3;
+  const bool evalid = r0 + erow < a.n;
+  const uint32_t nlast = a.n - 1u;
+  bf16x8 w1p[4][2], w2p[8][2], w2rp[8][2];  // this wave's weight panels: [slice][column tile]
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+      w1p[s][c] = *(const bf16x8 *)(a.W1t + (cw + c * 16 + lr) * 128 + s * 32 + kg);
+  __asm__ volatile("" ::: "memory");
+  uint32_t esrc = min(r0 + erow, nlast), tsrc = min(r0 + lane, nlast);
+  if (a.idx) {
+    esrc = a.idx[esrc];
+    tsrc = a.idx[tsrc];
+  }
+  __asm__ volatile("" ::: "memory");
+  const mi_pos epos = a.pos[esrc];
+  const float tgt = a.targets[tsrc];
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int s = 0; s < 8; s++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      w2p[s][c] = *(const bf16x8 *)(a.W2t + (cw + c * 16 + lr) * 256 + s * 32 + kg);
+      w2rp[s][c] = *(const bf16x8 *)(a.W2r + (cw + c * 16 + lr) * 256 + s * 32 + kg);
+    }
+  float b1c[2], b2c[2], w3c[2];
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    b1c[c] = a.b1[cw + c * 16 + lr];
+    b2c[c] = a.b2[cw + c * 16 + lr];
+    w3c[c] = a.w3[cw + c * 16 + lr];
+  }
+  const float b3 = a.b3[0];
+  {  // encode: thread t -> row t/8, plane (t/2)%4, squares 16(t%2)..+15
+    const int row = erow, plane = (t >> 1) & 3, s0 = (t & 1) * 16;
+    uint32_t bits = 0;
+    if (evalid) bits = plane_bits(epos, plane) >> s0;
+    uint32_t w[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      w[j] = ((bits >> (2 * j)) & 1u ? 0x3F80u : 0u) | ((bits >> (2 * j + 1)) & 1u ? 0x3F800000u : 0u);
+    *(uint4 *)&Xs[row][plane * 32 + s0] = uint4{w[0], w[1], w[2], w[3]};
+    *(uint4 *)&Xs[row][plane * 32 + s0 + 8] = uint4{w[4], w[5], w[6], w[7]};
+#pragma unroll
+    for (int s = 0; s < 16; s++) T[plane * 32 + s0 + s][row] = ((bits >> s) & 1u) ? 0x3F80u : 0u;
+  }
+  lds_barrier();
+  store_transposed(T, 128, a.XT, ld, r0);
+  f32x4 acc[4][2];
+  // ---- layer 1: X[64][128] . W1 -> H1 (row-major and transposed)
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const bf16x8 x = *(const bf16x8 *)&Xs[r * 16 + lr][s * 32 + kg];
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, w1p[s][c], acc[r][c], 0, 0, 0);
     }
   }
-  red[grp][c] = s;
-  __syncthreads();
-  if (grp == 0 && col < cols)
-    part[(size_t)blockIdx.y * part_stride + col] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+  lds_barrier();  // X^T has left the staging tile
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int col = cw + c * 16 + lr;
+    const float bias = b1c[c];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = r * 16 + 4 * (lane >> 4) + v;
+        const float x = acc[r][c][v] + bias;
+        const uint16_t h = to_bf(x > 0.f ? x : 0.f);
+        H1s[row][col] = h;
+        T[col][row] = h;
+      }
+  }
+  lds_barrier();
+  store_transposed(T, 256, a.H1T, ld, r0);
+  // ---- layer 2: H1[64][256] . W2 -> H2 (registers), the output layer, dz
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const bf16x8 h = *(const bf16x8 *)&H1s[r * 16 + lr][s * 32 + kg];
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h, w2p[s][c], acc[r][c], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const float bias = b2c[c];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const float x = acc[r][c][v] + bias;
+        acc[r][c][v] = from_bf(to_bf(x > 0.f ? x : 0.f));  // H2
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {  // this wave's share of the row's dot product with w3
+      float d = 0.f;
+#pragma unroll
+      for (int c = 0; c < 2; c++) d += acc[r][c][v] * w3c[c];
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) d += __shfl_xor(d, off);
+      if (lr == 0) red[wave][r * 16 + 4 * (lane >> 4) + v] = d;
+    }
+  lds_barrier();
+  if (wave == 0) {  // output, loss and dz per row (lane = row); the b3 and loss partials
+    const uint32_t row = r0 + lane;
+    float d = 0.f, l = 0.f;
+    {
+      float dot = 0.f;
+#pragma unroll
+      for (int w = 0; w < 8; w++) dot += red[w][lane];
+      const float y = tanhf(dot + b3);
+      if (row < a.n) {
+        const float e = y - tgt;
+        d = 2.f * e * (1.f - y * y) / (float)a.n;
+        l = e * e / (float)a.n;
+      }
+      dzs[lane] = d;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      d += __shfl_xor(d, off);
+      l += __shfl_xor(l, off);
+    }
+    if (lane == 0) {
+      pv[768] = d;
+      pv[769] = l;
+    }
+  }
+  lds_barrier();
+  {  // dH2 = bf(dz w3) where H2 > 0 (row-major and transposed); b2 and w3 partials
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int col = cw + c * 16 + lr;
+      float sb2 = 0.f, sw3 = 0.f;
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int row = r * 16 + 4 * (lane >> 4) + v;
+          const float d = dzs[row], h = acc[r][c][v];
+          const uint16_t g = to_bf(h > 0.f ? d * w3c[c] : 0.f);
+          dH2s[row][col] = g;
+          T[col][row] = g;
+          sb2 += from_bf(g);
+          sw3 += d * h;
+        }
+      sb2 += __shfl_xor(sb2, 16);
+      sw3 += __shfl_xor(sw3, 16);
+      sb2 += __shfl_xor(sb2, 32);
+      sw3 += __shfl_xor(sw3, 32);
+      if (lane < 16) {
+        pv[256 + col] = sb2;
+        pv[512 + col] = sw3;
+      }
+    }
+  }
+  lds_barrier();
+  store_transposed(T, 256, a.dH2T, ld, r0);
+  // ---- dH1 = bf(dH2 . W2^T) where H1 > 0 (transposed); b1 partials
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const bf16x8 g = *(const bf16x8 *)&dH2s[r * 16 + lr][s * 32 + kg];
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(g, w2rp[s][c], acc[r][c], 0, 0, 0);
+    }
+  }
+  lds_barrier();  // dH2^T has left the staging tile
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int col = cw + c * 16 + lr;
+    float sb1 = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = r * 16 + 4 * (lane >> 4) + v;
+        const uint16_t g = to_bf(H1s[row][col] ? acc[r][c][v] : 0.f);  // H1 >= 0: nonzero is > 0
+        T[col][row] = g;
+        sb1 += from_bf(g);
+      }
+    sb1 += __shfl_xor(sb1, 16);
+    sb1 += __shfl_xor(sb1, 32);
+    if (lane < 16) pv[col] = sb1;
+  }
+  lds_barrier();
+  store_transposed(T, 256, a.dH1T, ld, r0);
 }
 
-// Adam over every parameter.  Gradients are sums of partials in a fixed order: W1 and W2 over
-// their split-K partials, b1, b2, w3 and b3 over the column-sum partials (slots 0-255 b1, then
-// b2, w3, b3 at 256-768, in parameter order).  The summed gradient is written to `grads` (for
-// inspection).  Refreshes the BF16 copies: W1t[o][i], W2t[o][i] (forward B operands) and
-// W2r[i][o] (backward).
-// Adam over 16x16 tiles of W1 ([128][256]) and W2 ([256][256]), then the 769 vector parameters
-// (b1, b2, w3, b3; slot v of the column-sum partials).  Each parameter's gradient is the sum of
-// its partials in split order; the BF16 copies are refreshed, the transposed ones (W1t, W2t:
-// [out][in]) through an LDS tile so that their stores are contiguous.
-constexpr uint32_t AT = 16, kAdamW1 = (128 / AT) * (256 / AT), kAdamW2 = (256 / AT) * (256 / AT);
-constexpr uint32_t kAdamVec = (769 + 255) / 256;
-
-__device__ __forceinline__ float adam_step(const AdamArgs &a, uint32_t p, float g) {
+// One Adam step of parameter p with gradient g, from its values w, m, v (loaded by the caller
+// beforehand, together with the gradient's partials); returns the new value.
+__device__ __forceinline__ float adam_step(const AdamArgs &a, uint32_t p, float g, float w, float m,
+                                           float v) {
   a.grads[p] = g;
-  float w = a.w[p];
   if (a.update) {
-    const float m = a.beta1 * a.m[p] + a.omb1 * g;
-    const float v = a.beta2 * a.v[p] + a.omb2 * g * g;
+    m = a.beta1 * m + a.omb1 * g;
+    v = a.beta2 * v + a.omb2 * g * g;
     a.m[p] = m;
     a.v[p] = v;
     w -= a.lr * (m * a.c1) / (sqrtf(v * a.c2) + a.eps);
@@ -386,6 +415,93 @@ __device__ __forceinline__ float adam_step(const AdamArgs &a, uint32_t p, float 
   }
   return w;
 }
+
+// The sum of n partials at p, p + stride, ... in order, their loads issued 16 at a time.
+__device__ __forceinline__ float sum_partials(const float *p, size_t stride, uint32_t n) {
+  float g = 0.f;
+  for (uint32_t s0 = 0; s0 < n; s0 += 16u) {
+    float x[16];
+#pragma unroll
+    for (uint32_t j = 0; j < 16u; j++) x[j] = s0 + j < n ? p[(s0 + j) * stride] : 0.f;
+#pragma unroll
+    for (uint32_t j = 0; j < 16u; j++) g += x[j];
+  }
+  return g;
+}
+
+// Weight-gradient partials: workgroup (tile, split) forms one 64x64 tile of dW2 (tiles 0..15,
+// [256 in][256 out]) or dW1 (16..23, [128][256]) over the split's rows [256 split, +256) of the
+// batch, dW1[j][i] = sum_r X^T[j][r] dH1^T[i][r], dW2[i][o] = sum_r H1^T[i][r] dH2^T[o][r], and
+// writes it to P + split * kWParams (W1 at 0, W2 at 32768, each [in][out]).  The split's operand
+// rows (64 features x 256 rows each, 64 KB) come in with one round of 16-byte loads (16 per
+// thread in flight) into LDS; rows past the batch read as zero.  Wave w computes the 32x32
+// quadrant (w / 2, w % 2) as 2 x 2 MFMA tiles over eight 32-deep slices.
+constexpr uint32_t kWSplit = 256, kWTiles = 24, kWParams = 32768 + 65536;
+constexpr int WP = 256 + 8;  // LDS pitch of the staged operand rows
+
+__global__ void __launch_bounds__(256) k_wgrad_part(AdamArgs a) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[64][WP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[64][WP];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t tile = blockIdx.x, split = blockIdx.y;
+  const bool w1 = tile >= 16u;
+  const uint32_t tb = w1 ? tile - 16u : tile;
+  const uint32_t m0 = (tb >> 2) * 64u, n0 = (tb & 3u) * 64u;  // output rows (in), columns (out)
+  const uint16_t *A = w1 ? a.XT : a.H1T;
+  const uint16_t *B = w1 ? a.dH1T : a.dH2T;
+  const uint32_t k0 = split * kWSplit;
+  uint4 va[8], vb[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const uint32_t u = t + 256u * j, row = u >> 5, k = k0 + (u & 31u) * 8u;
+    const bool in = k < a.rows;
+    va[j] = in ? *(const uint4 *)(A + (size_t)(m0 + row) * a.ld + k) : uint4{0u, 0u, 0u, 0u};
+    vb[j] = in ? *(const uint4 *)(B + (size_t)(n0 + row) * a.ld + k) : uint4{0u, 0u, 0u, 0u};
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const uint32_t u = t + 256u * j, row = u >> 5, c = (u & 31u) * 8u;
+    *(uint4 *)&As[row][c] = va[j];
+    *(uint4 *)&Bs[row][c] = vb[j];
+  }
+  __syncthreads();
+  const uint32_t qr = (wave >> 1) * 32u, qc = (wave & 1u) * 32u;
+  const uint32_t lr = lane & 15u, kg = (lane >> 4) * 8u;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    bf16x8 x[2], y[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) x[r] = *(const bf16x8 *)&As[qr + r * 16 + lr][s * 32 + kg];
+#pragma unroll
+    for (int c = 0; c < 2; c++) y[c] = *(const bf16x8 *)&Bs[qc + c * 16 + lr][s * 32 + kg];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x[r], y[c], acc[r][c], 0, 0, 0);
+  }
+  float *P = a.P + (size_t)split * kWParams + (w1 ? 0u : 32768u);
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        P[(m0 + qr + r * 16 + 4 * (lane >> 4) + v) * 256u + n0 + qc + c * 16 + lr] = acc[r][c][v];
+}
+
+// Adam over every parameter.  Each gradient is the sum of its partials in split order: W1 and W2
+// over the weight-gradient splits (P), b1, b2, w3 and b3 over the row pass's 64-row blocks (pV;
+// slot 769 holds the loss).  Workgroups take 16x16 tiles of W1 ([128][256]) and W2 ([256][256]),
+// then the 769 vector parameters; the BF16 copies are refreshed, the transposed ones (W1t, W2t:
+// [out][in]) through an LDS tile so that their stores are contiguous.
+constexpr uint32_t AT = 16, kAdamW1 = (128 / AT) * (256 / AT), kAdamW2 = (256 / AT) * (256 / AT);
+constexpr uint32_t kAdamVec = (769 + 255) / 256;
 
 __global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
   __shared__ uint16_t tile[AT][AT + 2];
@@ -396,37 +512,32 @@ __global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
     const uint32_t i0 = (tb / (256 / AT)) * AT, o0 = (tb % (256 / AT)) * AT;
     const uint32_t il = t / AT, ol = t % AT;
     const uint32_t q = (i0 + il) * 256 + o0 + ol;  // index within the matrix ([in][out])
-    const float *pW = w1 ? a.pW1 : a.pW2;
-    const size_t stride = w1 ? 32768 : 65536;
-    float g = 0.f;
-#pragma unroll 8
-    for (uint32_t s = 0; s < a.splits; s++) g += pW[(size_t)s * stride + q];
-    const uint16_t h = to_bf(adam_step(a, (w1 ? MI_MLP_OFF_W1 : MI_MLP_OFF_W2) + q, g));
+    const uint32_t p = (w1 ? MI_MLP_OFF_W1 : MI_MLP_OFF_W2) + q;
+    const float w = a.w[p], m = a.m[p], v = a.v[p];
+    const float g = sum_partials(a.P + (w1 ? 0u : 32768u) + q, kWParams, a.splits);
+    const uint16_t h = to_bf(adam_step(a, p, g, w, m, v));
     if (!w1) a.W2r[q] = h;
     tile[ol][il] = h;
-    __syncthreads();
+    lds_barrier();
     const uint32_t ro = t / AT, ri = t % AT;  // row (out) and column (in) of the transposed copy
     (w1 ? a.W1t : a.W2t)[(o0 + ro) * (w1 ? 128u : 256u) + i0 + ri] = tile[ro][ri];
     return;
   }
-  if (b == kAdamW1 + kAdamW2 && t < 64u && a.loss_out) {
-    // the batch loss: the sum of the head's per-block partials in block order; the first wave
-    // loads 64 partials at a time and every lane adds them in order from the lanes that hold
-    // them
-    float l = 0.f;
-    for (uint32_t base = 0; base < a.vsplits; base += 64u) {
-      const float part = base + t < a.vsplits ? a.pV[(size_t)(base + t) * kVecSlots + 769] : 0.f;
-      const uint32_t cnt = min(64u, a.vsplits - base);
-      for (uint32_t j = 0; j < cnt; j++) l += __shfl(part, (int)j);
-    }
-    if (t == 0u) *a.loss_out = l;
+  const uint32_t slot = (b - kAdamW1 - kAdamW2) * 256 + t;  // b1 0-255, b2 256-511, w3 512-767,
+  if (slot > 769u) return;                                   // b3 768, loss 769
+  const uint32_t p = slot < 256u ? MI_MLP_OFF_B1 + slot : MI_MLP_OFF_B2 + (slot - 256u);
+  float w = 0.f, m = 0.f, v = 0.f;
+  if (slot < 769u) {
+    w = a.w[p];
+    m = a.m[p];
+    v = a.v[p];
   }
-  const uint32_t v = (b - kAdamW1 - kAdamW2) * 256 + t;  // vector slot: b1 0-255, b2 256-511,
-  if (v >= 769u) return;                                 // w3 512-767, b3 768
-  float g = 0.f;
-#pragma unroll 8
-  for (uint32_t s = 0; s < a.vsplits; s++) g += a.pV[(size_t)s * kVecSlots + v];
-  adam_step(a, v < 256u ? MI_MLP_OFF_B1 + v : MI_MLP_OFF_B2 + (v - 256u), g);
+  const float g = sum_partials(a.pV + slot, kVecSlots, a.blocks);
+  if (slot == 769u) {
+    if (a.loss_out) *a.loss_out = g;
+    return;
+  }
+  adam_step(a, p, g, w, m, v);
 }
 
 __global__ void k_zero(float *p, uint32_t n) {
@@ -436,21 +547,6 @@ __global__ void k_zero(float *p, uint32_t n) {
 
 }  // namespace
 
-hipError_t gemm(const GemmArgs &g, int splits, bool a_bf16, bool b_bf16, hipStream_t stream) {
-  if (g.M % BM || g.N % BN || g.k_per_split % BK) return hipErrorInvalidValue;
-  if (!a_bf16 || !b_bf16) return hipErrorInvalidValue;  // BF16 operands only
-  dim3 grid(g.N / BN, g.M / BM, splits), block(256);
-  hipLaunchKernelGGL(k_gemm, grid, block, 0, stream, g);
-  return hipGetLastError();
-}
-
-hipError_t encode(const mi_pos *pos, const uint32_t *idx, uint32_t n, uint32_t rows, uint16_t *X,
-                  hipStream_t stream) {
-  const uint32_t threads = rows * 4;
-  hipLaunchKernelGGL(k_encode, dim3((threads + 255) / 256), dim3(256), 0, stream, pos, idx, n, rows, X);
-  return hipGetLastError();
-}
-
 hipError_t forward(const mi_pos *pos, uint32_t n, const uint16_t *W1t, const float *b1,
                    const uint16_t *W2t, const float *b2, const float *w3, const float *b3, float *y,
                    hipStream_t stream) {
@@ -459,30 +555,17 @@ hipError_t forward(const mi_pos *pos, uint32_t n, const uint16_t *W1t, const flo
   return hipGetLastError();
 }
 
-hipError_t head_train(const uint16_t *H2, const float *w3, const float *b3, const float *targets,
-                      const uint32_t *tidx, uint32_t n, uint32_t rows, float *dz, uint16_t *dH2,
-                      float *part, hipStream_t stream) {
-  if (rows % 64u) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_head_train, dim3(rows / 64), dim3(256), 0, stream, H2, w3, b3, targets,
-                     tidx, n, dz, dH2, part);
+hipError_t train_rows(const TrainArgs &a, uint32_t rows, hipStream_t stream) {
+  if (rows % KR || rows == 0 || rows > a.ld || a.n == 0 || a.n > rows) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_train_rows, dim3(rows / KR), dim3(512), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t colsum_bf16(const uint16_t *src, uint32_t rows, uint32_t cols, const float *weight,
-                       float *part, hipStream_t stream) {
-  hipLaunchKernelGGL(k_colsum<uint16_t>, dim3((cols + 63) / 64, (rows + 63) / 64), dim3(256), 0,
-                     stream, src, rows, cols, weight, part, (uint32_t)kVecSlots);
-  return hipGetLastError();
-}
-
-hipError_t colsum_f32(const float *src, uint32_t rows, uint32_t cols, const float *weight,
-                      float *part, hipStream_t stream) {
-  hipLaunchKernelGGL(k_colsum<float>, dim3((cols + 63) / 64, (rows + 63) / 64), dim3(256), 0,
-                     stream, src, rows, cols, weight, part, (uint32_t)kVecSlots);
-  return hipGetLastError();
-}
-
-hipError_t adam(const AdamArgs &a, hipStream_t stream) {
+hipError_t wgrad_adam(const AdamArgs &a, hipStream_t stream) {
+  if (a.rows % KR || a.rows > a.ld || a.blocks * KR != a.rows ||
+      a.splits != (a.rows + kWSplit - 1) / kWSplit)
+    return hipErrorInvalidValue;
+  if (a.splits) hipLaunchKernelGGL(k_wgrad_part, dim3(kWTiles, a.splits), dim3(256), 0, stream, a);
   hipLaunchKernelGGL(k_adam, dim3(kAdamW1 + kAdamW2 + kAdamVec), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
