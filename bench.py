@@ -7,8 +7,10 @@ A step is one launch of the persistent playout kernel over the staged batch of t
   random (default): 1024 seeded positions (k in [0, 30] random plies from the start) x 1024
                     random playouts = 1,048,576 bit-exact playouts
   heuristic       : 1024 seeded midgame positions (k in [20, 60], finished games redrawn) x 1024
-                    heuristic playouts
-  small           : 64 positions x 4096 random playouts (the host path's own size)
+                    heuristic playouts (largest weight, ties broken by the ply's word)
+  noisy           : the same positions x 1024 playouts of the reference's heuristic, each move's
+                    weight plus an N(0, 0.11) draw
+  small          : 64 positions x 4096 random playouts (the host path's own size)
   multisearch     : 256 concurrent MCTS searches sharing the device (one line, its own fields)
   value           : value-network training and evaluation (one line, its own fields)
   cpu             : CPU rates only (reference host code and restatement, 1 and all threads)
@@ -56,6 +58,10 @@ WORKLOADS = {
                       name="heuristic-1024x1024",
                       desc="heuristic-1024x1024: 1024 seeded midgame positions (k in [20,60], "
                            "finished games redrawn) x 1024 heuristic playouts"),
+    "noisy": dict(n=1024, per_pos=1024, kmin=20, kmax=60, reject=True, policy=2,
+                  name="noisy-1024x1024",
+                  desc="noisy-1024x1024: 1024 seeded midgame positions (k in [20,60], finished "
+                       "games redrawn) x 1024 heuristic playouts with N(0, 0.11) weight noise"),
     "small": dict(n=64, per_pos=4096, kmin=0, kmax=30, reject=False, policy=0,
                   name="random-64x4096",
                   desc="random-64x4096: 64 seeded positions x 4096 random playouts"),
@@ -376,7 +382,7 @@ def main() -> int:
         "data": "synthetic (seeded positions generated on the device)",
         "config": {"workload": cfg["desc"], "positions": cfg["n"],
                    "playouts_per_position": cfg["per_pos"], "playouts_per_step": total,
-                   "policy": "random" if cfg["policy"] == 0 else "heuristic",
+                   "policy": ("random", "heuristic", "heuristic_noisy")[cfg["policy"]],
                    "position_seed": POS_SEED, "playout_seed": PLAY_SEED,
                    "block_size": tuning["block_size"], "blocks_per_cu": tuning["blocks_per_cu"],
                    "refill": "queue" if tuning["refill"] == 1 else "static",

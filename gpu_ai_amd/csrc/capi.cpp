@@ -217,7 +217,7 @@ uint32_t tune_bucket(uint64_t total) {
 // The launch shape of a batch of `total` playouts: the table's entry for its policy and bucket
 // (the faster refill, or refill `only` when it is 0 or 1), else the context's tuning.
 mi_tuning shape_for(const mi_ctx *ctx, uint32_t policy, uint64_t total, int only = -1) {
-  const TuneEntry *e = ctx->tune_table[policy == MI_POLICY_HEURISTIC ? 1 : 0][tune_bucket(total)];
+  const TuneEntry *e = ctx->tune_table[policy][tune_bucket(total)];
   const TuneEntry *best = nullptr;
   for (int r = 0; r < 2; r++) {
     if (e[r].kernel_ms < 0.f || (only >= 0 && r != only)) continue;
@@ -290,8 +290,7 @@ int run_kernel(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
 }
 
 int bad_policy(mi_ctx *ctx, uint32_t policy) {
-  if (policy != MI_POLICY_RANDOM && policy != MI_POLICY_HEURISTIC)
-    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "unknown policy");
+  if (policy > MI_POLICY_HEURISTIC_NOISY) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "unknown policy");
   return MI_OK;
 }
 
@@ -596,7 +595,7 @@ int tune_sweep(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
       }
     }
   }
-  TuneEntry *e = ctx->tune_table[policy == MI_POLICY_HEURISTIC ? 1 : 0][tune_bucket(total)];
+  TuneEntry *e = ctx->tune_table[policy][tune_bucket(total)];
   for (int r = 0; r < 2; r++) {
     e[r].tuning = refill_best[r];
     e[r].kernel_ms = refill_ms[r];
@@ -674,13 +673,13 @@ int mi_autotune_sizes(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, uint32_t po
 int mi_get_tuning_table(const mi_ctx *ctx, mi_tune_entry *out, uint32_t cap, uint32_t *count) {
   if (!ctx || !count || (cap && !out)) return MI_ERR_INVALID_ARGUMENT;
   uint32_t n = 0;
-  for (uint32_t pol = 0; pol < 2; pol++)
+  for (uint32_t pol = 0; pol <= MI_POLICY_HEURISTIC_NOISY; pol++)
     for (uint32_t b = 0; b < kTuneBuckets; b++)
       for (uint32_t r = 0; r < 2; r++) {
         const TuneEntry &e = ctx->tune_table[pol][b][r];
         if (e.kernel_ms < 0.f) continue;
         if (n < cap) {
-          out[n].policy = pol == 1 ? MI_POLICY_HEURISTIC : MI_POLICY_RANDOM;
+          out[n].policy = pol;
           out[n].bucket = b;
           out[n].tuning = e.tuning;
           out[n].kernel_ms = e.kernel_ms;

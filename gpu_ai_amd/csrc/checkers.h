@@ -1159,6 +1159,311 @@ CK_HD PlyResult ply_heuristic_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool
   return heuristic_body<TRACE, false>(L, g, r, list_dig);
 }
 
+// ------------------------------------------------------------------------------------------
+// noisy heuristic (src/heuristicPlayout.cpp:16,26-36; src/heuristicPlayout.cu:50-62;
+// src/heuristic.hpp:5): each move's weight (float)(S_me + d_me) / (S_op + d_op) plus an
+// N(0, 0.11) draw; the move of largest sum wins, the first in reference order on equality.
+// When every move weighs the same, the draws alone decide, and the argmax of iid draws is
+// uniform over the moves: such a ply plays reference index r mod n without drawing (the first
+// move when the common weight is +inf, which no draw changes).  The draws are made only when
+// the weights differ.
+// ------------------------------------------------------------------------------------------
+constexpr float NOISE_SIGMA = 0.11f;
+constexpr uint32_t NOISE_STEP = 0x9E3779B9u;
+
+CK_HD float f32_from_bits(uint32_t b) { return __builtin_bit_cast(float, b); }
+CK_HD uint32_t f32_bits(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+// A pair of standard normal draws from two words by the Box-Muller transform (Box and Muller,
+// Ann. Math. Statist. 29 (1958) 610-611):
+//   radius = sqrt(-2 ln u), u = ((a >> 8) + 1) 2^-24 in (0, 1]
+//   angle  = q pi/2 + x, q = b >> 30, x = (t 2^-24 - 1/2) pi/2, t = (b >> 6) mod 2^24
+// ln u = (e - 24) ln 2 + ln m with (a >> 8) + 1 = m 2^e, m in (sqrt(1/2), sqrt(2)], and
+// ln m = 2 atanh(s), s = (m - 1)/(m + 1), by its series to s^9; sin and cos of x in
+// [-pi/4, pi/4) by their Taylor series to x^9 and x^10.  Only IEEE-754 single additions,
+// multiplications, one division and one square root, never contracted, so the host and the
+// device round every step alike.
+CK_HD void normal_pair(uint32_t a, uint32_t b, float &z0, float &z1) {
+#pragma clang fp contract(off)
+  const float uf = (float)((a >> 8) + 1u);  // exact: at most 2^24
+  const uint32_t ub = f32_bits(uf);
+  int e = (int)(ub >> 23) - 127;
+  float m = f32_from_bits((ub & 0x007FFFFFu) | 0x3F800000u);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  const float lnm =
+      s * (2.0f + s2 * (0.666666667f + s2 * (0.4f + s2 * (0.285714286f + s2 * 0.222222222f))));
+  const float lnu = (float)(e - 24) * 0.693147181f + lnm;
+  const float rad = __builtin_sqrtf(-2.0f * lnu);
+  const float x = ((float)((b >> 6) & 0x00FFFFFFu) * 5.96046448e-8f - 0.5f) * 1.57079633f;
+  const float x2 = x * x;
+  const float sn =
+      x * (1.0f + x2 * (-0.166666667f + x2 * (8.33333333e-3f + x2 * (-1.98412698e-4f +
+                                                                     x2 * 2.75573192e-6f))));
+  const float cs =
+      1.0f + x2 * (-0.5f + x2 * (4.16666667e-2f + x2 * (-1.38888889e-3f +
+                                                        x2 * (2.48015873e-5f +
+                                                              x2 * -2.75573192e-7f))));
+  const uint32_t q = b >> 30;
+  const float c = q == 0u ? cs : q == 1u ? -sn : q == 2u ? -cs : sn;
+  const float t = q == 0u ? sn : q == 1u ? cs : q == 2u ? -sn : -cs;
+  z0 = rad * c;
+  z1 = rad * t;
+}
+
+// Draws of move m (mover-frame order) in a ply of word r: moves 2j and 2j+1 take the two draws
+// of the pair from fmix(r + (2j+1) K) and fmix(r + (2j+2) K), K = 0x9E3779B9, scaled by sigma.
+CK_HD void noise_pair(uint32_t r, uint32_t j, float &e0, float &e1) {
+#pragma clang fp contract(off)
+  float z0, z1;
+  normal_pair(fmix(r + (2u * j + 1u) * NOISE_STEP), fmix(r + (2u * j + 2u) * NOISE_STEP), z0, z1);
+  e0 = z0 * NOISE_SIGMA;
+  e1 = z1 * NOISE_SIGMA;
+}
+
+// Running argmax of the noisy weights in mover-frame order.  Player 2's reference order is the
+// reverse, so its first maximum in reference order is the last one here (>=).
+struct NoisyBest {
+  float best;
+  float spare;  // second draw of the current pair
+  uint32_t m;   // moves seen
+  uint32_t pick;
+  uint32_t rev;
+};
+
+CK_HD bool noisy_take(NoisyBest &nb, uint32_t r, float w) {
+#pragma clang fp contract(off)
+  float e;
+  if ((nb.m & 1u) == 0u) {
+    noise_pair(r, nb.m >> 1, e, nb.spare);
+  } else {
+    e = nb.spare;
+  }
+  const float v = w + e;
+  const bool take = nb.rev ? v >= nb.best : v > nb.best;
+  if (take) {
+    nb.best = v;
+    nb.pick = nb.m;
+  }
+  nb.m++;
+  return take;
+}
+
+// Weight classes of a ply without longer capture sequences, as slot masks: kc[t] = the move of
+// slot t removes a king, prom = the men whose slot-0/1 move promotes.
+CK_HD void weight_classes(const Lane &L, const Gen &g, uint32_t kc[4], uint32_t &prom) {
+  const uint32_t men = L.own & ~L.kings;
+  kc[0] = kc[1] = kc[2] = kc[3] = 0u;
+  if (g.capture) {
+    const uint32_t kng = L.own & L.kings;
+    const uint32_t okings = L.opp & L.kings;
+    const uint32_t kd0 = g.cap[0] & step3(okings), kd1 = g.cap[1] & step2(okings);
+    kc[0] = g.slot[0] & ((men & kd1) | (kng & kd0));
+    kc[1] = g.slot[1] & ((men & kd0) | (kng & kd1));
+    kc[2] = g.slot[2] & g.cap[2] & step1(okings);
+    kc[3] = g.slot[3] & g.cap[3] & step0(okings);
+  }
+  prom = men & (g.capture ? ROW5 : ROW6);
+}
+
+// Whether every move of such a ply weighs the same; `inf` when that weight is +inf (the mover
+// takes the opponent's last piece, or the opponent has none).
+CK_HD bool single_weight(const Lane &L, const Gen &g, uint32_t sm, uint32_t so, bool &inf) {
+  uint32_t kc[4], prom;
+  weight_classes(L, g, kc, prom);
+  const uint32_t n = slot_count(g.slot);
+  const uint32_t np = popc(g.slot[0] & prom) + popc(g.slot[1] & prom);
+  if (!g.capture) {  // steps: sm / so and, for promotions, (sm + 3) / so
+    inf = so == 0u;
+    return inf || np == 0u || np == n;
+  }
+  // single jumps: (sm + 3p) / (so - (k ? 4 : 1)) over the classes that occur
+  const uint32_t nk = popc(kc[0]) + popc(kc[1]) + popc(kc[2]) + popc(kc[3]);
+  const uint32_t nkp = popc(kc[0] & prom) + popc(kc[1] & prom);
+  const uint32_t cnt[4] = {n - nk - np + nkp, nk - nkp, np - nkp, nkp};
+  bool have = false, same = true;
+  uint32_t bn = 0u, bd = 1u;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    if (!cnt[c]) continue;
+    const uint32_t num = sm + ((c & 2) ? 3u : 0u), den = so - ((c & 1) ? 4u : 1u);
+    if (!have) {
+      bn = num;
+      bd = den;
+      have = true;
+    } else if (cmp_weight(num, den, bn, bd) != 0) {
+      same = false;
+    }
+  }
+  inf = bd == 0u;
+  return same;
+}
+
+// The noisy heuristic ply after analyse (WALK as for heuristic_body).
+template <bool TRACE, bool WALK>
+CK_HD PlyResult noisy_body(Lane &L, const Gen &g, uint32_t r, uint32_t *list_dig) {
+  PlyResult res;
+  const uint32_t sm = material(L.own, L.kings);
+  const uint32_t so = material(L.opp, L.kings);
+  const bool walked = WALK && g.capture && g.multi;
+  NoisyBest nb = {-__builtin_inff(), 0.0f, 0u, 0u, L.turn};
+  Choice c = {0u, 0u, 0u};
+  uint32_t n, km;
+  bool single, inf = false;
+  if (!walked) {
+    n = slot_count(g.slot);
+    if (TRACE) *list_dig = list_digest(L, g, n);
+    if (n == 0u || L.counter >= 50u) {
+      res.n = n;
+      res.over = 1u;
+      res.chosen = 0xFFFFu;
+      res.winner = L.counter < 50u ? (int32_t)(L.turn ^ 1u) : -1;
+      return res;
+    }
+    single = single_weight(L, g, sm, so, inf);
+    if (!single) {
+      uint32_t kc[4], prom;
+      weight_classes(L, g, kc, prom);
+      float wc[4];  // by class: bit 0 = a king removed, bit 1 = a man promoted
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t num = sm + ((k & 2) ? 3u : 0u);
+        const uint32_t den = so - (g.capture ? ((k & 1) ? 4u : 1u) : 0u);
+        wc[k] = (float)num / (float)den;  // only classes that occur are read
+      }
+      uint32_t any = g.slot[0] | g.slot[1] | g.slot[2] | g.slot[3];
+      uint32_t bsq = 0u, bsl = 0u;
+      while (any) {
+        const uint32_t sq = ctz(any);
+        any &= any - 1u;
+        for (uint32_t sl = 0; sl < 4u; sl++) {
+          if (!getbit(g.slot[sl], sq)) continue;
+          const uint32_t k = getbit(kc[sl], sq) | ((sl < 2u ? getbit(prom, sq) : 0u) << 1);
+          const float w = k == 0u ? wc[0] : k == 1u ? wc[1] : k == 2u ? wc[2] : wc[3];
+          if (noisy_take(nb, r, w)) {
+            bsq = sq;
+            bsl = sl;
+          }
+        }
+      }
+      c = slot_choice(L, g, bsq, bsl);
+      km = nb.pick;
+    } else {
+      const uint32_t k = inf ? 0u : mod_u32(r, n);  // reference index
+      km = L.turn ? n - 1u - k : k;
+      uint32_t sq, sl;
+      select_slot(g.slot, km, sq, sl);
+      c = slot_choice(L, g, sq, sl);
+    }
+  } else {
+    if (L.counter >= 50u) {
+      n = count_walks(L, g);
+      if (TRACE) *list_dig = list_digest(L, g, n);
+      res.n = n;
+      res.over = 1u;
+      res.chosen = 0xFFFFu;
+      res.winner = -1;
+      return res;
+    }
+    // one walk draws for every sequence and notes whether the weights differ at all
+    uint32_t o = g.origins;
+    uint32_t king = 0, s = 0, fn = 0, fd = 0;
+    single = true;
+    auto leaf = [&](uint32_t to, uint32_t removed, uint32_t, uint32_t) {
+      const uint32_t num = sm + ((!king && to >= FAR_ROW_FIRST) ? 3u : 0u);
+      const uint32_t den = so - material(removed, L.kings);
+      if (nb.m == 0u) {
+        fn = num;
+        fd = den;
+      } else if (cmp_weight(num, den, fn, fd) != 0) {
+        single = false;
+      }
+      if (noisy_take(nb, r, (float)num / (float)den)) {
+        c.from = s;
+        c.to = to;
+        c.removed = removed;
+      }
+      return false;
+    };
+    while (o) {
+      s = ctz(o);
+      o &= o - 1u;
+      king = getbit(L.kings, s);
+      walk(s, king, g.cap, leaf);
+    }
+    n = nb.m;
+    km = nb.pick;
+    if (single) {
+      const uint32_t k = fd == 0u ? 0u : mod_u32(r, n);
+      km = L.turn ? n - 1u - k : k;
+      c = select_walk(L, g, km);
+    }
+    if (TRACE) *list_dig = list_digest(L, g, n);
+  }
+  res.n = n;
+  res.over = 0u;
+  res.winner = 0;
+  res.chosen = TRACE ? (L.turn ? n - 1u - km : km) : 0u;
+  apply(L, c.from, c.to, c.removed);
+  return res;
+}
+
+// Plays one ply of the noisy heuristic with word r.
+template <bool TRACE>
+CK_HD PlyResult ply_noisy(Lane &L, uint32_t r, uint32_t *list_dig) {
+  Gen g;
+  analyse(L, g);
+  return noisy_body<TRACE, true>(L, g, r, list_dig);
+}
+
+// The noisy heuristic ply given the position's analysis g.
+template <bool TRACE>
+CK_HD PlyResult ply_noisy_given(Lane &L, const Gen &g, uint32_t r, uint32_t *list_dig) {
+  return noisy_body<TRACE, true>(L, g, r, list_dig);
+}
+
+// The common part of a noisy heuristic ply: plays it when its moves all weigh the same (and no
+// capture sequence is longer than one jump), else leaves it untouched with `deferred` set, for
+// ply_noisy_given with the same word and the analysis left in g.
+template <bool TRACE>
+CK_HD PlyResult ply_noisy_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool &deferred, Gen &g) {
+  PlyResult res;
+  analyse(L, g);
+  const uint32_t n = slot_count(g.slot);
+  bool inf = false;
+  deferred = (g.capture && g.multi) ||
+             (n != 0u && L.counter < 50u &&
+              !single_weight(L, g, material(L.own, L.kings), material(L.opp, L.kings), inf));
+  if (deferred) {
+    res.n = 0u;
+    res.chosen = 0u;
+    res.over = 0u;
+    res.winner = 0;
+    return res;
+  }
+  res.n = n;
+  if (TRACE) *list_dig = list_digest(L, g, n);
+  if (n == 0u || L.counter >= 50u) {
+    res.over = 1u;
+    res.chosen = 0xFFFFu;
+    res.winner = L.counter < 50u ? (int32_t)(L.turn ^ 1u) : -1;
+    return res;
+  }
+  const uint32_t k = inf ? 0u : mod_u32(r, n);  // reference index
+  uint32_t sq, sl;
+  select_slot(g.slot, L.turn ? n - 1u - k : k, sq, sl);
+  const Choice c = slot_choice(L, g, sq, sl);
+  res.over = 0u;
+  res.winner = 0;
+  res.chosen = k;
+  apply(L, c.from, c.to, c.removed);
+  return res;
+}
+
 // True when the position is finished (no legal move, or the draw counter reached 50).
 CK_HD bool game_over(const Lane &L) {
   if (L.counter >= 50u) return true;

@@ -41,7 +41,7 @@ struct mi_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   mi_tuning tuning = kDefaultTuning;  // the shape of batches whose bucket the table lacks
-  TuneEntry tune_table[2][kTuneBuckets][2];  // [policy][bucket][refill]
+  TuneEntry tune_table[3][kTuneBuckets][2];  // [policy][bucket][refill]
   uint32_t cus = 0, wave = 64, clock_khz = 0;
   gpuai::DevBuf pos, offs, winners, plies, hashes, work, tallies, moves, counts, keys;
   // longest-first order of the uploaded batch (see upload_batch): positions permuted into it,

@@ -188,8 +188,7 @@ extern "C" int mi_run_playouts_noalloc(mi_ctx *ctx, const mi_pos *pos, uint32_t 
   if (!ctx) return MI_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->last_error.clear();
-  if (policy != MI_POLICY_RANDOM && policy != MI_POLICY_HEURISTIC)
-    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "unknown policy");
+  if (policy > MI_POLICY_HEURISTIC_NOISY) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "unknown policy");
   if (npos > gpuai::kMaxPos)
     return fail(ctx, MI_ERR_INVALID_ARGUMENT,
                 "mi_run_playouts_noalloc takes at most 65536 positions (its fixed scratch size)");

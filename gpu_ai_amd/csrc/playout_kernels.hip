@@ -214,7 +214,8 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
       // for a given ply)
       bool d = false;
       if (POLICY == MI_POLICY_RANDOM) res = ck::ply_random_fast<TRACE>(L, r, &ld, d, gd);
-      else res = ck::ply_heuristic_fast<TRACE>(L, r, &ld, d, gd);
+      else if (POLICY == MI_POLICY_HEURISTIC) res = ck::ply_heuristic_fast<TRACE>(L, r, &ld, d, gd);
+      else res = ck::ply_noisy_fast<TRACE>(L, r, &ld, d, gd);
       deferred = d;
       play = !d;
     }
@@ -233,7 +234,8 @@ __global__ void __launch_bounds__(1024) k_playouts(PlayoutArgs a) {
               res = ck::ply_random_trees<TRACE>(L, gd, r, &ld, deep);
               if (deep) res = ck::ply_random_given<TRACE>(L, gd, r, &ld);
             }
-            else res = ck::ply_heuristic_given<TRACE>(L, gd, r, &ld);
+            else if (POLICY == MI_POLICY_HEURISTIC) res = ck::ply_heuristic_given<TRACE>(L, gd, r, &ld);
+            else res = ck::ply_noisy_given<TRACE>(L, gd, r, &ld);
             deferred = false;
             play = true;
           }
@@ -450,9 +452,13 @@ const void *playout_kernel(uint32_t policy, uint32_t refill) {
     return refill == MI_REFILL_STATIC
                ? (const void *)k_playouts<MI_POLICY_RANDOM, false, MI_REFILL_STATIC>
                : (const void *)k_playouts<MI_POLICY_RANDOM, false, MI_REFILL_QUEUE>;
+  if (policy == MI_POLICY_HEURISTIC)
+    return refill == MI_REFILL_STATIC
+               ? (const void *)k_playouts<MI_POLICY_HEURISTIC, false, MI_REFILL_STATIC>
+               : (const void *)k_playouts<MI_POLICY_HEURISTIC, false, MI_REFILL_QUEUE>;
   return refill == MI_REFILL_STATIC
-             ? (const void *)k_playouts<MI_POLICY_HEURISTIC, false, MI_REFILL_STATIC>
-             : (const void *)k_playouts<MI_POLICY_HEURISTIC, false, MI_REFILL_QUEUE>;
+             ? (const void *)k_playouts<MI_POLICY_HEURISTIC_NOISY, false, MI_REFILL_STATIC>
+             : (const void *)k_playouts<MI_POLICY_HEURISTIC_NOISY, false, MI_REFILL_QUEUE>;
 }
 
 const void *reset_kernel() { return (const void *)k_reset; }
@@ -467,8 +473,11 @@ hipError_t launch_playouts(const PlayoutArgs &args, uint32_t policy, bool trace,
   if (policy == MI_POLICY_RANDOM)
     return trace ? launch_refill<MI_POLICY_RANDOM, true>(args, refill, grid, block, stream)
                  : launch_refill<MI_POLICY_RANDOM, false>(args, refill, grid, block, stream);
-  return trace ? launch_refill<MI_POLICY_HEURISTIC, true>(args, refill, grid, block, stream)
-               : launch_refill<MI_POLICY_HEURISTIC, false>(args, refill, grid, block, stream);
+  if (policy == MI_POLICY_HEURISTIC)
+    return trace ? launch_refill<MI_POLICY_HEURISTIC, true>(args, refill, grid, block, stream)
+                 : launch_refill<MI_POLICY_HEURISTIC, false>(args, refill, grid, block, stream);
+  return trace ? launch_refill<MI_POLICY_HEURISTIC_NOISY, true>(args, refill, grid, block, stream)
+               : launch_refill<MI_POLICY_HEURISTIC_NOISY, false>(args, refill, grid, block, stream);
 }
 
 hipError_t launch_gen_moves(const mi_pos *pos, uint32_t npos, mi_move *moves, uint32_t cap,

@@ -30,7 +30,8 @@ MI_ERR_CAPACITY = 5
 MI_ERR_NOT_STAGED = 6
 
 POLICY_RANDOM = 0
-POLICY_HEURISTIC = 1
+POLICY_HEURISTIC = 1        # the heuristic without noise, ties broken by the ply's word
+POLICY_HEURISTIC_NOISY = 2  # the reference's heuristic: an N(0, 0.11) draw on each weight
 REFILL_STATIC = 0
 REFILL_QUEUE = 1
 

@@ -172,6 +172,8 @@ int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch
   }
 }
 
+int gpuai_playout_policy(const char *name) { return name ? playoutPolicyOf(name) : -1; }
+
 int gpuai_driver_run_playouts(const mi_pos *states, uint32_t n, uint32_t policy, uint32_t seed,
                               uint32_t calls, int8_t *winners, char *err, uint32_t err_len) {
   try {

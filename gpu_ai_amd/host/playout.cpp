@@ -14,7 +14,8 @@ void checkMi(int rc, const mi_ctx *ctx, const char *what) {
 DeviceMI355XPlayoutDriver::DeviceMI355XPlayoutDriver(uint32_t policy, int device, uint32_t seed,
                                                      const mi_tuning *tuning)
     : policy_(policy), seed_(seed) {
-  if (policy != MI_POLICY_RANDOM && policy != MI_POLICY_HEURISTIC)
+  if (policy != MI_POLICY_RANDOM && policy != MI_POLICY_HEURISTIC &&
+      policy != MI_POLICY_HEURISTIC_NOISY)
     throw std::runtime_error("unknown playout policy");
   checkMi(mi_create(&ctx_, device, tuning), nullptr, "mi_create");
 }
@@ -75,19 +76,25 @@ std::vector<PlayerId> DeviceMI355XPlayoutDriver::runPlayouts(std::vector<State> 
   return results;
 }
 
-std::unique_ptr<PlayoutDriver> getPlayoutDriver(std::string name) {
+int playoutPolicyOf(const std::string &name) {
+  // hybrid_heuristic is a plain hybrid driver in the reference (src/playout.cpp:211-213)
   static const char *random_names[] = {"host", "device_single", "device_multiple", "device_coarse",
-                                       "hybrid", "optimal", "device_mi355x"};
-  static const char *heuristic_names[] = {"host_heuristic", "device_heuristic", "hybrid_heuristic",
-                                          "hybrid_heruistic", "optimal_heuristic",
-                                          "device_mi355x_heuristic"};
+                                       "hybrid", "hybrid_heuristic", "optimal", "device_mi355x"};
+  static const char *noisy_names[] = {"host_heuristic", "device_heuristic", "optimal_heuristic",
+                                      "device_mi355x_heuristic"};
   uint32_t policy = ~0u;
   for (const char *n : random_names)
     if (name == n) policy = MI_POLICY_RANDOM;
-  for (const char *n : heuristic_names)
-    if (name == n) policy = MI_POLICY_HEURISTIC;
-  if (policy == ~0u) throw std::runtime_error("Unknown playout type");
-  auto d = std::make_unique<DeviceMI355XPlayoutDriver>(policy);
+  for (const char *n : noisy_names)
+    if (name == n) policy = MI_POLICY_HEURISTIC_NOISY;
+  if (name == "device_mi355x_heuristic_argmax") policy = MI_POLICY_HEURISTIC;
+  return policy == ~0u ? -1 : (int)policy;
+}
+
+std::unique_ptr<PlayoutDriver> getPlayoutDriver(std::string name) {
+  const int policy = playoutPolicyOf(name);
+  if (policy < 0) throw std::runtime_error("Unknown playout type");
+  auto d = std::make_unique<DeviceMI355XPlayoutDriver>((uint32_t)policy);
   if (name == "optimal" || name == "optimal_heuristic") {
     std::vector<uint32_t> sizes;
     for (uint32_t s = 1u << 6; s <= (1u << 20); s <<= 2) sizes.push_back(s);

@@ -35,7 +35,9 @@ class DeviceMI355XPlayoutDriver : public PlayoutDriver {
 
   std::vector<PlayerId> runPlayouts(std::vector<State> states) override;
   std::string getName() const override {
-    return policy_ == MI_POLICY_RANDOM ? "device_mi355x" : "device_mi355x_heuristic";
+    return policy_ == MI_POLICY_RANDOM      ? "device_mi355x"
+           : policy_ == MI_POLICY_HEURISTIC ? "device_mi355x_heuristic_argmax"
+                                            : "device_mi355x_heuristic";
   }
 
   // Grouped form used by the batched searches: positions with playout counts, winners in
@@ -63,14 +65,19 @@ class DeviceMI355XPlayoutDriver : public PlayoutDriver {
 };
 
 // Name -> driver.  The reference's driver names (src/playout.cpp:189-223) all select the MI355X
-// engine: host, device_single, device_multiple, device_coarse, hybrid and optimal run random
-// playouts; host_heuristic, device_heuristic, hybrid_heuristic and optimal_heuristic run
-// heuristic playouts.  device_mi355x and device_mi355x_heuristic name them directly.  The
+// engine: host, device_single, device_multiple, device_coarse, hybrid, hybrid_heuristic (a plain
+// hybrid driver there, src/playout.cpp:211-213) and optimal run random playouts;
+// host_heuristic, device_heuristic and optimal_heuristic run the noisy heuristic
+// (MI_POLICY_HEURISTIC_NOISY, the reference's N(0, 0.11) weight noise).  device_mi355x and
+// device_mi355x_heuristic name those two directly, device_mi355x_heuristic_argmax the
+// noise-free heuristic with ties broken by the ply's word (MI_POLICY_HEURISTIC).  The
 // optimal names, whose reference driver picks a driver by batch size (src/playout.cpp:80-166),
 // get a driver whose launch-shape table is tuned at construction for batches of 2^6 to 2^20
 // playouts (about a second).
 // Unknown names throw std::runtime_error("Unknown playout type").
 std::unique_ptr<PlayoutDriver> getPlayoutDriver(std::string name);
+// The MI_POLICY_* that getPlayoutDriver(name) plays, or -1 for an unknown name.
+int playoutPolicyOf(const std::string &name);
 
 // Throws std::runtime_error with the library's message when rc != MI_OK.
 void checkMi(int rc, const mi_ctx *ctx, const char *what);

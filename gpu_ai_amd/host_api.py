@@ -95,6 +95,8 @@ def host_library() -> ctypes.CDLL:
                                               ctypes.POINTER(MultiSearchResult), ctypes.c_char_p,
                                               _U32]
         lib.gpuai_multisearch_run.restype = ctypes.c_int
+        lib.gpuai_playout_policy.argtypes = [ctypes.c_char_p]
+        lib.gpuai_playout_policy.restype = ctypes.c_int
         lib.gpuai_driver_run_playouts.argtypes = [_P, _U32, _U32, _U32, _U32, _P,
                                                   ctypes.c_char_p, _U32]
         lib.gpuai_driver_run_playouts.restype = ctypes.c_int
@@ -174,6 +176,11 @@ def multisearch(searches: int = 256, seed: int = 2016, playout_seed: int = 12345
     out["budgets"] = budgets
     out["root_trials"] = root_trials
     return out
+
+
+def playout_policy(name: str) -> int:
+    """The MI_POLICY_* that getPlayoutDriver(name) plays (-1: unknown name); needs no device."""
+    return int(host_library().gpuai_playout_policy(name.encode()))
 
 
 def driver_run_playouts(states, policy: int = 0, seed: int = 12345, calls: int = 1) -> np.ndarray:

@@ -96,6 +96,9 @@ int gpuai_multisearch_run(const gpuai_multisearch_config *cfg, gpuai_multisearch
  * trials), runPlayouts and update; `hook` receives each round's leaves (states in trial order,
  * ungrouped, in record->pos with npos = trials and counts = keys = NULL) and the winners the
  * driver returned.  Both return 0, or -1 with a message in `err`. */
+/* The MI_POLICY_* of the driver that getPlayoutDriver(name) (src/playout.cpp:189-223) returns,
+ * or -1 for a name it rejects; needs no device. */
+int gpuai_playout_policy(const char *name);
 int gpuai_driver_run_playouts(const mi_pos *states, uint32_t n, uint32_t policy, uint32_t seed,
                               uint32_t calls, int8_t *winners, char *err, uint32_t err_len);
 int gpuai_search_record(const mi_pos *root, uint32_t iterations, uint32_t initial, double scale,

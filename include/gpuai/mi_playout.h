@@ -39,11 +39,17 @@ extern "C" {
 
 /* ---- policies ---- */
 #define MI_POLICY_RANDOM 0     /* HostPlayoutDriver, src/playout.cpp:17-32 */
-#define MI_POLICY_HEURISTIC 1  /* HostHeuristicPlayoutDriver, src/heuristicPlayout.cpp:12-48 */
+/* The heuristic of src/heuristicPlayout.cpp:12-48 without its noise: the largest weight
+ * (S_me + d_me) / (S_op + d_op), ties broken by the ply's word. */
+#define MI_POLICY_HEURISTIC 1
+/* HostHeuristicPlayoutDriver and DeviceHeuristicPlayoutDriver (src/heuristicPlayout.cpp:12-48,
+ * src/heuristicPlayout.cu:50-62): the largest weight plus an N(0, 0.11) draw per move, the
+ * draws taken from the ply's word by a Box-Muller transform. */
+#define MI_POLICY_HEURISTIC_NOISY 2
 
 /* ---- refill policies of the persistent playout kernel ---- */
 #define MI_REFILL_STATIC 0     /* lane-strided assignment, no atomics */
-#define MI_REFILL_QUEUE 1      /* wave-level claims of 32 playouts from a device counter */
+#define MI_REFILL_QUEUE 1      /* wave-level claims, multiples of 16 playouts, from slab counters */
 
 /* Packed position (16 bytes).  Replaces the 776-byte State (src/state.hpp:119-122). */
 typedef struct mi_pos {

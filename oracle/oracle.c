@@ -15,11 +15,14 @@
  * golden checks detect it:
  *   1 men try the (f,+1) jump before (f,-1)      2 kings may land again on their first landing
  *   3 draw at 49 plies instead of 50              4 men keep their type on the far row in captures
- *   5 heuristic ties always take the first move   6 Philox uses 9 rounds */
+ *   5 heuristic ties always take the first move   6 Philox uses 9 rounds
+ *   7 the noisy heuristic's sigma is 0.22          8 its noise follows reference, not mover, order
+ *   9 it draws on plies whose weights are all equal */
 #ifndef OR_MUTATION
 #define OR_MUTATION 0
 #endif
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -476,6 +479,127 @@ static int heuristic_choice(const board_state *s, const board_move *list, int n,
   return pick;
 }
 
+/* ---- the noisy heuristic: src/heuristicPlayout.cpp:16,26-36, src/heuristic.hpp:5 ---- */
+
+static float f32_from_bits(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static uint32_t f32_bits(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+
+/* Two standard normal draws from the words a, b by the Box-Muller transform (Box and Muller,
+ * Ann. Math. Statist. 29 (1958) 610-611), in single precision with every step rounded
+ * (x86-64 SSE arithmetic, no contraction):
+ *   radius = sqrt(-2 ln u), u = ((a >> 8) + 1) 2^-24;  angle = q pi/2 + x, q = b >> 30,
+ *   x = (((b >> 6) mod 2^24) 2^-24 - 1/2) pi/2.
+ * ln u = (e - 24) ln 2 + 2 atanh((m - 1)/(m + 1)) for (a >> 8) + 1 = m 2^e, m in (sqrt(1/2),
+ * sqrt(2)], the atanh series to s^9; sin x and cos x by Taylor series to x^9 and x^10. */
+static void normal_pair(uint32_t a, uint32_t b, float *z0, float *z1) {
+  const float uf = (float)((a >> 8) + 1u);
+  const uint32_t ub = f32_bits(uf);
+  int e = (int)(ub >> 23) - 127;
+  float m = f32_from_bits((ub & 0x007FFFFFu) | 0x3F800000u);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  const float lnm =
+      s * (2.0f + s2 * (0.666666667f + s2 * (0.4f + s2 * (0.285714286f + s2 * 0.222222222f))));
+  const float lnu = (float)(e - 24) * 0.693147181f + lnm;
+  const float rad = sqrtf(-2.0f * lnu);
+  const float x = ((float)((b >> 6) & 0x00FFFFFFu) * 5.96046448e-8f - 0.5f) * 1.57079633f;
+  const float x2 = x * x;
+  const float sn = x * (1.0f + x2 * (-0.166666667f + x2 * (8.33333333e-3f +
+                                     x2 * (-1.98412698e-4f + x2 * 2.75573192e-6f))));
+  const float cs = 1.0f + x2 * (-0.5f + x2 * (4.16666667e-2f + x2 * (-1.38888889e-3f +
+                                 x2 * (2.48015873e-5f + x2 * -2.75573192e-7f))));
+  const uint32_t q = b >> 30;
+  const float c = q == 0u ? cs : q == 1u ? -sn : q == 2u ? -cs : sn;
+  const float t = q == 0u ? sn : q == 1u ? cs : q == 2u ? -sn : -cs;
+  *z0 = rad * c;
+  *z1 = rad * t;
+}
+
+#define NOISE_SIGMA (OR_MUTATION == 7 ? 0.22f : 0.11f)
+
+/* The noise of move m in mover order (player 2's order is the reverse of the reference's) for
+ * the ply word r: moves 2j and 2j+1 take the pair of fmix(r + (2j+1) K), fmix(r + (2j+2) K),
+ * K = 0x9E3779B9, times sigma. */
+float or_heuristic_noise(uint32_t r, uint32_t m) {
+  const uint32_t j = m >> 1;
+  float z0, z1;
+  normal_pair(or_fmix32(r + (2u * j + 1u) * 0x9E3779B9u), or_fmix32(r + (2u * j + 2u) * 0x9E3779B9u),
+              &z0, &z1);
+  return ((m & 1u) ? z1 : z0) * NOISE_SIGMA;
+}
+
+void or_normal_pairs(const uint32_t *a, const uint32_t *b, uint32_t n, float *z) {
+  for (uint32_t i = 0; i < n; i++) normal_pair(a[i], b[i], &z[2 * i], &z[2 * i + 1]);
+}
+
+/* Noisy choice, src/heuristicPlayout.cpp:26-36: weight (float)(S_me + d_me) / (S_op + d_op)
+ * (src/heuristic.cu:44-49) plus the move's draw; the first move of the largest sum in reference
+ * order (strict >, starting from -infinity).  When all n weights are equal the draws alone
+ * decide, and the argmax of iid draws is uniform: the ply then plays index r mod n without
+ * drawing (index 0 when the common weight is +inf, which no draw changes). */
+static int noisy_choice(const board_state *s, const board_move *list, int n, const uint32_t score[2],
+                        uint32_t r, int32_t chosen_delta[2]) {
+  int me = s->turn, op = next_turn(s->turn);
+  float w[OR_MAX_MOVES];
+  int32_t d[2];
+  int single = 1;
+  if (n <= 0) return 0;
+  for (int i = 0; i < n; i++) {
+    score_move(s, &list[i], d);
+    const uint32_t num = score[me] + (uint32_t)d[me];
+    const uint32_t den = score[op] + (uint32_t)d[op];
+    w[i] = (float)num / (float)den;
+    if (w[i] != w[0]) single = 0;
+  }
+  int pick = 0;
+  if (single && OR_MUTATION != 9) {
+    pick = isinf(w[0]) ? 0 : (int)(r % (uint32_t)n);
+  } else {
+    float best = -INFINITY;
+    for (int i = 0; i < n; i++) {
+      const uint32_t m =
+          (OR_MUTATION == 8 || me == OR_PLAYER_1) ? (uint32_t)i : (uint32_t)(n - 1 - i);
+      const float v = w[i] + or_heuristic_noise(r, m);
+      if (v > best) {
+        best = v;
+        pick = i;
+      }
+    }
+  }
+  score_move(s, &list[pick], chosen_delta);
+  return pick;
+}
+
+/* Choice frequencies of one ply of `policy` from p: trial t plays the ply with word
+ * or_stream_word(seed, 0, t, 0, OR_TAG_PLAYOUT, 0); counts[i] = times reference index i was
+ * chosen.  Returns n, or -1 if n > cap. */
+int or_choice_counts(const or_pos *p, int policy, uint32_t seed, uint32_t trials, uint32_t *counts,
+                     int cap) {
+  board_state s;
+  board_move list[LIST_CAP];
+  uint32_t score[2];
+  unpack(p, &s);
+  const int n = gen(&s, list);
+  if (n > cap) return -1;
+  score_state(&s, score);
+  for (int i = 0; i < n; i++) counts[i] = 0;
+  if (n == 0) return 0;
+  for (uint32_t t = 0; t < trials; t++) {
+    const uint32_t r = or_stream_word(seed, 0, t, 0, OR_TAG_PLAYOUT, 0);
+    int32_t d[2];
+    int k;
+    if (policy == OR_POLICY_HEURISTIC_NOISY) k = noisy_choice(&s, list, n, score, r, d);
+    else if (policy == OR_POLICY_HEURISTIC) k = heuristic_choice(&s, list, n, score, r, d);
+    else k = (int)(r % (uint32_t)n);
+    counts[k]++;
+  }
+  return n;
+}
+
 /* src/playout.cpp:17-32 (random) and src/heuristicPlayout.cpp:12-48 (heuristic): play until
  * isGameOver (src/state.cpp:16-18), then report getWinner (src/state.cpp:20-23). */
 int or_playout(or_pos p, uint32_t seed, uint32_t pos_index, uint32_t playout, uint32_t epoch,
@@ -486,7 +610,7 @@ int or_playout(or_pos p, uint32_t seed, uint32_t pos_index, uint32_t playout, ui
   uint32_t h = 0, plies = 0, captures = 0;
   int max_n = 0;
   unpack(&p, &s);
-  if (policy == OR_POLICY_HEURISTIC) score_state(&s, score);
+  if (policy == OR_POLICY_HEURISTIC || policy == OR_POLICY_HEURISTIC_NOISY) score_state(&s, score);
   uint32_t words[4] = {0, 0, 0, 0};
   for (uint32_t ply = 0;; ply++) {
     int n = gen(&s, list);
@@ -506,6 +630,11 @@ int or_playout(or_pos p, uint32_t seed, uint32_t pos_index, uint32_t playout, ui
     if (policy == OR_POLICY_HEURISTIC) {
       int32_t d[2];
       k = heuristic_choice(&s, list, n, score, r, d);
+      score[0] += (uint32_t)d[0];
+      score[1] += (uint32_t)d[1];
+    } else if (policy == OR_POLICY_HEURISTIC_NOISY) {
+      int32_t d[2];
+      k = noisy_choice(&s, list, n, score, r, d);
       score[0] += (uint32_t)d[0];
       score[1] += (uint32_t)d[1];
     } else {

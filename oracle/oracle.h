@@ -33,10 +33,11 @@ extern "C" {
 #define OR_PLAYER_NONE (-1)
 
 #define OR_POLICY_RANDOM 0
-#define OR_POLICY_HEURISTIC 1
+#define OR_POLICY_HEURISTIC 1          /* the heuristic without noise, ties by the ply's word */
+#define OR_POLICY_HEURISTIC_NOISY 2    /* the heuristic with the N(0, 0.11) draw per move */
 /* Reference harness only (timing): the random choice drawn with glibc rand() % n, as
  * RandomPlayer::getMove does (src/player.cpp:13-16).  Not reproducible; never used for parity. */
-#define OR_POLICY_RANDOM_LIBC 2
+#define OR_POLICY_RANDOM_LIBC 9
 
 /* Philox stream tags (third counter word). */
 #define OR_TAG_PLAYOUT 0u
@@ -95,6 +96,14 @@ uint32_t or_move_digest(const or_move *m);
 uint32_t or_list_digest(const or_move *list, int n);
 uint32_t or_ply_record(uint32_t list_digest, int n, int chosen);   /* chosen = -1 at game end */
 uint32_t or_hash_step(uint32_t h, uint32_t record);
+
+/* Noisy heuristic: the draw sigma * z of move m (mover order) in a ply of word r, and the
+ * standard normal pairs z[2i], z[2i+1] of the words a[i], b[i]. */
+float or_heuristic_noise(uint32_t r, uint32_t m);
+void or_normal_pairs(const uint32_t *a, const uint32_t *b, uint32_t n, float *z);
+/* Choice frequencies of one ply of `policy` from p over `trials` words; returns n (-1: n > cap). */
+int or_choice_counts(const or_pos *p, int policy, uint32_t seed, uint32_t trials, uint32_t *counts,
+                     int cap);
 
 /* ---- playouts ---- */
 int or_playout(or_pos p, uint32_t seed, uint32_t pos_index, uint32_t playout, uint32_t epoch,

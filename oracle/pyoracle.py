@@ -25,7 +25,8 @@ TRACE_DTYPE = np.dtype([("winner", "i1"), ("max_n", "u1"), ("pad", "<u2"), ("pli
                         ("hash", "<u4"), ("captures", "<u4")])
 
 POLICY_RANDOM = 0
-POLICY_HEURISTIC = 1
+POLICY_HEURISTIC = 1        # without noise, ties broken by the ply's word
+POLICY_HEURISTIC_NOISY = 2  # the reference's N(0, 0.11) draw on each move's weight
 
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -157,6 +158,34 @@ class Oracle(_Lib):
         L.or_move_digest.restype = _U32
         L.or_list_digest.argtypes = [_P, ctypes.c_int]
         L.or_list_digest.restype = _U32
+        L.or_heuristic_noise.argtypes = [_U32, _U32]
+        L.or_heuristic_noise.restype = ctypes.c_float
+        L.or_normal_pairs.argtypes = [_P, _P, _U32, _P]
+        L.or_normal_pairs.restype = None
+
+    def choice_counts(self, pos, policy: int, seed: int, trials: int) -> np.ndarray:
+        """How often each move (reference index) is chosen in `trials` plies from pos."""
+        fn = self.lib.or_choice_counts
+        fn.argtypes = [_P, ctypes.c_int, _U32, _U32, _P, ctypes.c_int]
+        fn.restype = ctypes.c_int
+        p = np.ascontiguousarray(pos, dtype=POS_DTYPE).reshape(1)
+        out = np.zeros(256, dtype=np.uint32)
+        n = fn(_ptr(p), policy, seed, trials, _ptr(out), 256)
+        if n < 0:
+            raise RuntimeError("move list longer than 256")
+        return out[:n].copy()
+
+    def heuristic_noise(self, r: int, m: int) -> float:
+        """The noisy heuristic's draw for move m (mover order) in a ply of word r."""
+        return float(self.lib.or_heuristic_noise(r, m))
+
+    def normal_pairs(self, a, b) -> np.ndarray:
+        """Box-Muller pairs of the word arrays a, b: shape (n, 2), float32."""
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        z = np.zeros((len(a), 2), dtype=np.float32)
+        self.lib.or_normal_pairs(_ptr(a), _ptr(b), len(a), _ptr(z))
+        return z
 
     def philox(self, ctr, key) -> np.ndarray:
         c = np.ascontiguousarray(ctr, dtype=np.uint32)
@@ -209,6 +238,31 @@ class Reference(_Lib):
         if n < 0:
             raise RuntimeError("ref_mcts_trace failed")
         return out[:n]
+
+    def host_heuristic_driver(self, pos, per_pos: int) -> np.ndarray:
+        """The reference's HostHeuristicPlayoutDriver::runPlayouts on per_pos copies of each
+        position (position-major), one thread, its own default-seeded noise stream."""
+        fn = self.lib.ref_host_heuristic_driver
+        fn.argtypes = [_P, _U32, _U32, _P]
+        fn.restype = ctypes.c_int
+        pos = np.ascontiguousarray(pos, dtype=POS_DTYPE)
+        out = np.zeros(len(pos) * per_pos, dtype=np.int8)
+        if fn(_ptr(pos), len(pos), per_pos, _ptr(out)) != 0:
+            raise RuntimeError("ref_host_heuristic_driver failed")
+        return out
+
+    def noisy_choice_counts(self, pos, trials: int, seed: int) -> np.ndarray:
+        """One noisy heuristic ply from pos, `trials` times, with the reference's weights and
+        its std::normal_distribution noise: how often each move (reference index) won."""
+        fn = self.lib.ref_noisy_choice_counts
+        fn.argtypes = [_P, _U32, _U32, _P, ctypes.c_int]
+        fn.restype = ctypes.c_int
+        p = np.ascontiguousarray(pos, dtype=POS_DTYPE).reshape(1)
+        out = np.zeros(256, dtype=np.uint32)
+        n = fn(_ptr(p), trials, seed, _ptr(out), 256)
+        if n < 0:
+            raise RuntimeError("move list longer than 256")
+        return out[:n].copy()
 
     def apply_index(self, pos, index: int) -> np.ndarray:
         p = np.array(pos, dtype=POS_DTYPE).reshape(1).copy()

@@ -1,16 +1,20 @@
 // ref_harness.cpp -- C entry points over the reference's own host code (test infrastructure).
 //
 // Built only in the development container by oracle/Makefile, against the unmodified
-// reference sources src/state.cu, src/state.cpp, src/heuristic.cu and src/mcts.cpp, which the
-// Makefile compiles where they lie.  The functions below drive that code with the same packed
-// positions, Philox streams and hash records as oracle.c, so that the restatement can be
-// compared with the reference on any input.  The driver loops restate the ten-line bodies of
-// HostPlayoutDriver::runPlayouts (src/playout.cpp:17-32) and
+// reference sources src/state.cu, src/state.cpp, src/heuristic.cu, src/heuristicPlayout.cpp and
+// src/mcts.cpp, which the Makefile compiles where they lie.  The functions below drive that
+// code with the same packed positions, Philox streams and hash records as oracle.c, so that the
+// restatement can be compared with the reference on any input.  The driver loops restate the
+// ten-line bodies of HostPlayoutDriver::runPlayouts (src/playout.cpp:17-32) and
 // HostHeuristicPlayoutDriver::runPlayouts (src/heuristicPlayout.cpp:12-48), replacing
-// rand() % n and the N(0, 0.11) noise by the Philox word of the ply.
+// rand() % n and the N(0, 0.11) noise by the Philox word of the ply (the noisy policy draws its
+// noise with or_heuristic_noise, and plays r mod n where all weights are equal, where the argmax
+// of iid draws is uniform); ref_host_heuristic_driver runs the reference's own
+// HostHeuristicPlayoutDriver, noise and all.
 #include "state.hpp"
 #include "heuristic.hpp"
 #include "mcts.hpp"
+#include "playout.hpp"
 
 #include "oracle.h"
 
@@ -18,6 +22,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <random>
 #include <vector>
 #include <omp.h>
 
@@ -98,7 +103,8 @@ int playout(State state, uint32_t seed, uint32_t posIndex, uint32_t playoutIndex
   uint32_t h = 0, plies = 0, captures = 0;
   int maxN = 0;
   unsigned stateScore[NUM_PLAYERS];
-  if (policy == OR_POLICY_HEURISTIC) scoreState(state, stateScore);
+  if (policy == OR_POLICY_HEURISTIC || policy == OR_POLICY_HEURISTIC_NOISY)
+    scoreState(state, stateScore);
   Words words;
   for (uint32_t ply = 0;; ply++) {
     bool over = state.isGameOver();            // first genMoves of the ply
@@ -134,6 +140,33 @@ int playout(State state, uint32_t seed, uint32_t posIndex, uint32_t playoutIndex
       for (size_t i = 0; i < moves.size(); i++)
         if (weights[i] == optWeight) ties.push_back((unsigned)i);
       k = ties[r % ties.size()];
+      for (int i = 0; i < NUM_PLAYERS; i++) stateScore[i] += moveScores[k][i];
+    } else if (policy == OR_POLICY_HEURISTIC_NOISY) {
+      // src/heuristicPlayout.cpp:26-36 with the draw of move i taken from the ply's word (mover
+      // order: player 2's list is generated in the reverse of its mover-frame order); a ply
+      // whose weights are all equal plays index r mod n (0 for +inf) without drawing
+      std::vector<std::array<int, NUM_PLAYERS>> moveScores(moves.size());
+      std::vector<float> weights(moves.size());
+      bool single = true;
+      for (size_t i = 0; i < moves.size(); i++) {
+        scoreMove(state, moves[i], moveScores[i].data());
+        weights[i] = getWeight(state, stateScore, moveScores[i].data());
+        if (weights[i] != weights[0]) single = false;
+      }
+      if (single) {
+        k = std::isinf(weights[0]) ? 0u : (unsigned)(r % moves.size());
+      } else {
+        float optWeight = -1 / 0.0;
+        k = (unsigned)-1;
+        for (size_t i = 0; i < moves.size(); i++) {
+          const uint32_t m = state.turn == PLAYER_1 ? (uint32_t)i : (uint32_t)(n - 1 - (int)i);
+          float weight = weights[i] + or_heuristic_noise(r, m);
+          if (weight > optWeight) {
+            k = (unsigned)i;
+            optWeight = weight;
+          }
+        }
+      }
       for (int i = 0; i < NUM_PLAYERS; i++) stateScore[i] += moveScores[k][i];
     } else {
       k = r % moves.size();
@@ -250,6 +283,61 @@ int ref_run_playouts(const or_pos *pos, uint32_t npos, const uint32_t *counts, u
     if (traces) traces[g] = t;
   }
   return 0;
+}
+
+// The reference's own HostHeuristicPlayoutDriver::runPlayouts (src/heuristicPlayout.cpp:12-48)
+// on per_pos copies of each position, position-major, on one OpenMP thread: its generator is a
+// default-seeded std::default_random_engine shared by the loop's threads, so one thread makes
+// the call reproducible, and the copies of a position draw successive noise from that stream.
+int ref_host_heuristic_driver(const or_pos *pos, uint32_t npos, uint32_t perPos, int8_t *winners) {
+  std::vector<State> states;
+  states.reserve((size_t)npos * perPos);
+  for (uint32_t i = 0; i < npos; i++) {
+    State st = toState(pos[i]);
+    for (uint32_t j = 0; j < perPos; j++) states.push_back(st);
+  }
+  const int saved = omp_get_max_threads();
+  omp_set_num_threads(1);
+  HostHeuristicPlayoutDriver driver;
+  std::vector<PlayerId> w = driver.runPlayouts(states);
+  omp_set_num_threads(saved);
+  for (size_t i = 0; i < w.size(); i++) winners[i] = (int8_t)w[i];
+  return 0;
+}
+
+// Choice frequencies of one noisy heuristic ply from the position (src/heuristicPlayout.cpp:
+// 15-16,22-36 for one ply, drawn `trials` times): the reference's weights and its own noise
+// source, std::normal_distribution<float>(0, HEURISTIC_SIGMA) over a std::default_random_engine
+// seeded with `seed`.  counts[i] = times the move of reference index i was chosen; returns n.
+int ref_noisy_choice_counts(const or_pos *p, uint32_t trials, uint32_t seed, uint32_t *counts,
+                            int cap) {
+  State state = toState(*p);
+  std::vector<Move> moves = state.getMoves();
+  if ((int)moves.size() > cap) return -1;
+  unsigned stateScore[NUM_PLAYERS];
+  scoreState(state, stateScore);
+  std::vector<float> weights(moves.size());
+  for (size_t i = 0; i < moves.size(); i++) {
+    int moveScore[NUM_PLAYERS];
+    scoreMove(state, moves[i], moveScore);
+    weights[i] = getWeight(state, stateScore, moveScore);
+  }
+  std::default_random_engine generator(seed);
+  std::normal_distribution<float> distribution(0, HEURISTIC_SIGMA);
+  for (size_t i = 0; i < moves.size(); i++) counts[i] = 0;
+  for (uint32_t t = 0; t < trials; t++) {
+    unsigned optIndex = (unsigned)-1;
+    float optWeight = -1 / 0.0;
+    for (size_t i = 0; i < moves.size(); i++) {
+      float weight = weights[i] + distribution(generator);
+      if (weight > optWeight) {
+        optIndex = (unsigned)i;
+        optWeight = weight;
+      }
+    }
+    counts[optIndex]++;
+  }
+  return (int)moves.size();
 }
 
 // Start positions in the manner of genRandomStates (src/driver.cpp:76-104), with the same

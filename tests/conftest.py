@@ -45,6 +45,7 @@ def golden():
         "movelists": load("movelists.npz"),
         "playouts": load("playouts.npz"),
         "scores": load("scores.npz")["scores"],
+        "noisy": load("noisy.npz"),
     }
     with open(os.path.join(GOLDEN, "perft.json")) as f:
         g["perft"] = json.load(f)

@@ -16,9 +16,15 @@ Every vector below is produced by the reference build (oracle/_ref/libref_assert
                   playouts (random on the k in [0, 30] openings, heuristic on the k in [20, 60]
                   midgame positions; seed 12345, epoch 0), two bits per winner (winner + 1),
                   and each position's summed ply count
+  noisy.npz       the noisy heuristic (src/heuristicPlayout.cpp:12-48 with its N(0, 0.11)
+                  noise): playout traces with the oracle's noise draws over the reference's own
+                  move generation and weights; per-ply choice counts of the reference's weights
+                  with its own std::normal_distribution noise; winner counts of the reference's
+                  own HostHeuristicPlayoutDriver::runPlayouts
   MANIFEST.json   SHA-256 of each file
 
-    python tests/golden/make_golden.py --full-only   # rewrites full_configs.npz only
+    python tests/golden/make_golden.py --full-only    # rewrites full_configs.npz only
+    python tests/golden/make_golden.py --noisy-only   # rewrites noisy.npz only
 """
 from __future__ import annotations
 
@@ -58,10 +64,66 @@ def full_configs(ref, open1024: np.ndarray, mid1024: np.ndarray) -> None:
     np.savez_compressed(os.path.join(HERE, "full_configs.npz"), **out)
 
 
+CHOICE_TRIALS = 20000
+DRIVER_PLAYOUTS = 16384
+
+
+def noisy_fixtures(ref, mid1024: np.ndarray, placed: np.ndarray) -> None:
+    """noisy.npz: see the module docstring."""
+    out = {}
+    # playout traces: the reference's move lists and weights, the oracle's draws
+    out["trace_pos"] = mid1024[:256]
+    _, out["traces"] = ref.run_playouts(mid1024[:256], per_pos=16, seed=SEED_PLAY, epoch=0,
+                                        policy=pyoracle.POLICY_HEURISTIC_NOISY)
+    # per-ply choice counts: midgame positions whose moves do not all weigh the same (finite
+    # weights), placements where two or more moves take the last opponent piece (weight +inf:
+    # the first of them in reference order is played), and midgame positions whose moves all
+    # weigh the same (the draws alone decide: a uniform choice)
+    finite, infinite, single = [], [], []
+    for p in ref.start_positions(SEED_POS + 1, 4000, 10, 70, True):
+        ms = ref.gen_moves(p)
+        ws = [ref.score_move_index(p, k)[2] for k in range(len(ms))]
+        if len(ms) >= 2 and len(set(ws)) >= 2 and np.isfinite(max(ws)) and len(finite) < 64:
+            finite.append(p)
+        if len(ms) >= 3 and len(set(ws)) == 1 and np.isfinite(ws[0]) and len(single) < 16:
+            single.append(p)
+        if len(finite) == 64 and len(single) == 16:
+            break
+    for p in placed:
+        if ref.is_game_over(p):
+            continue
+        ms = ref.gen_moves(p)
+        ws = [ref.score_move_index(p, k)[2] for k in range(len(ms))]
+        if sum(1 for w in ws if np.isinf(w)) >= 2:
+            infinite.append(p)
+        if len(infinite) == 16:
+            break
+    cpos = np.array(finite + infinite + single, dtype=pyoracle.POS_DTYPE)
+    counts = np.zeros((len(cpos), 64), dtype=np.uint32)
+    nmoves = np.zeros(len(cpos), dtype=np.uint32)
+    for i, p in enumerate(cpos):
+        c = ref.noisy_choice_counts(p, CHOICE_TRIALS, 1000 + i)
+        counts[i, :len(c)] = c
+        nmoves[i] = len(c)
+    out.update(choice_pos=cpos, choice_counts=counts, choice_n=nmoves,
+               choice_trials=np.uint32(CHOICE_TRIALS), choice_infinite_from=np.uint32(len(finite)),
+               choice_single_from=np.uint32(len(finite) + len(infinite)))
+    # the reference driver itself: winner counts (player 1, player 2, draw) per position
+    dpos = mid1024[256:272]
+    w = ref.host_heuristic_driver(dpos, DRIVER_PLAYOUTS).reshape(len(dpos), DRIVER_PLAYOUTS)
+    out["driver_pos"] = dpos
+    out["driver_counts"] = np.stack([(w == 0).sum(1), (w == 1).sum(1), (w == -1).sum(1)],
+                                    axis=1).astype(np.uint32)
+    np.savez_compressed(os.path.join(HERE, "noisy.npz"), **out)
+    print(f"noisy: {len(out['traces'])} traces, {len(finite)} + {len(infinite)} + {len(single)} "
+          f"choice positions, "
+          f"driver {len(dpos)} x {DRIVER_PLAYOUTS}")
+
+
 def write_manifest() -> None:
     manifest = {}
     for name in ("positions.npz", "movelists.npz", "perft.json", "playouts.npz", "scores.npz",
-                 "full_configs.npz"):
+                 "full_configs.npz", "noisy.npz"):
         with open(os.path.join(HERE, name), "rb") as f:
             manifest[name] = hashlib.sha256(f.read()).hexdigest()
     manifest["source"] = "oracle/_ref/libref_assert.so (reference host code, asserts enabled)"
@@ -74,6 +136,14 @@ def main() -> None:
     if "--full-only" in sys.argv:
         with np.load(os.path.join(HERE, "positions.npz")) as z:
             full_configs(ref, z["open1024"], z["mid1024"])
+        write_manifest()
+        return
+    if "--noisy-only" in sys.argv:
+        with np.load(os.path.join(HERE, "positions.npz")) as z:
+            mid1024 = z["mid1024"]
+        with np.load(os.path.join(HERE, "movelists.npz")) as z:
+            positions = z["positions"]
+        noisy_fixtures(ref, mid1024, positions)
         write_manifest()
         return
 
@@ -184,6 +254,7 @@ def main() -> None:
     np.savez_compressed(os.path.join(HERE, "scores.npz"), scores=sc)
 
     full_configs(ref, open1024, mid1024)
+    noisy_fixtures(ref, mid1024, positions)
     write_manifest()
     print(f"positions {len(positions)}, moves {len(moves)}, max n {counts.max()}, "
           f"max jumps {moves['jumps'].max()}, king loops {len(loops)} ({tries} tries), "

@@ -2,8 +2,9 @@
 // compiled for the host and checked against the oracle (test infrastructure).
 //
 // Each playout runs the sequence k_playouts runs for one lane: the ply's Philox word, the fast
-// ply (ply_random_fast / ply_heuristic_fast), and, when that defers a multi-jump capture, the
-// full ply (ply_random_given / ply_heuristic_given) with the same word and the fast ply's
+// ply (ply_random_fast / ply_heuristic_fast / ply_noisy_fast), and, when that defers a
+// multi-jump capture, the full ply (ply_random_given / ply_heuristic_given / ply_noisy_given)
+// with the same word and the fast ply's
 // analysis of the position.  Winners, ply counts, the largest
 // list length and the per-ply hash chain over every move list and chosen index must equal the
 // oracle's traces (oracle/oracle.c, which follows src/state.cu and src/playout.cpp:17-32).
@@ -32,8 +33,9 @@ int play(const or_pos &p, uint32_t seed, uint32_t pi, uint32_t j, uint32_t epoch
     uint32_t ld = 0;
     bool d = false;
     Gen g;
-    PlyResult res = policy == OR_POLICY_RANDOM ? ply_random_fast<true>(L, r, &ld, d, g)
-                                               : ply_heuristic_fast<true>(L, r, &ld, d, g);
+    PlyResult res = policy == OR_POLICY_RANDOM      ? ply_random_fast<true>(L, r, &ld, d, g)
+                    : policy == OR_POLICY_HEURISTIC ? ply_heuristic_fast<true>(L, r, &ld, d, g)
+                                                    : ply_noisy_fast<true>(L, r, &ld, d, g);
     if (d) {
       deferred++;
       // the kernel's walk phase: the full ply with the same word and the fast ply's analysis
@@ -42,8 +44,10 @@ int play(const or_pos &p, uint32_t seed, uint32_t pi, uint32_t j, uint32_t epoch
         bool deep = false;
         res = ply_random_trees<true>(L, g, r, &ld, deep);
         if (deep) res = ply_random_given<true>(L, g, r, &ld);
-      } else {
+      } else if (policy == OR_POLICY_HEURISTIC) {
         res = ply_heuristic_given<true>(L, g, r, &ld);
+      } else {
+        res = ply_noisy_given<true>(L, g, r, &ld);
       }
     }
     h = hash_step(h, ply_record(ld, res.n, res.chosen));
@@ -229,7 +233,7 @@ int main(int argc, char **argv) {
     if (set == 0) or_gen_start_positions(2016, N, 0, 30, 0, pos.data());
     else if (set == 1) or_gen_start_positions(777, N, 20, 60, 1, pos.data());
     else or_gen_start_positions(99, N, 40, 120, 1, pos.data());
-    for (int policy = 0; policy < 2; policy++) {
+    for (int policy = 0; policy < 3; policy++) {
       std::vector<or_trace> ot((size_t)N * P);
       or_run_playouts(pos.data(), N, nullptr, P, 12345, 3, policy, nullptr, ot.data(), 0);
       long pbad = 0;

@@ -11,7 +11,8 @@ import sys
 import numpy as np
 import pytest
 
-from gpu_ai_amd.engine import POLICY_HEURISTIC, POLICY_RANDOM, REFILL_QUEUE, REFILL_STATIC
+from gpu_ai_amd.engine import (POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY, POLICY_RANDOM,
+                               REFILL_QUEUE, REFILL_STATIC)
 
 pytestmark = pytest.mark.gpu
 
@@ -82,7 +83,7 @@ def test_traced_playouts_match_reference_golden(engine, golden):
         assert res.stats["plies"] == int(expect["plies"].sum(dtype=np.uint64))
 
 
-@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
+@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY])
 def test_traced_full_positions_vs_oracle(engine, oracle, golden, policy):
     pos = golden["positions"]["open1024" if policy == POLICY_RANDOM else "mid1024"]
     announce(f"traced 1024 positions x 64, policy {policy}")
@@ -117,6 +118,56 @@ def test_heuristic_1024x1024_full_bitexact(engine, oracle, golden):
     assert res.stats["plies"] == int(expect["plies"].sum(dtype=np.uint64))
 
 
+def test_noisy_traces_match_reference_golden(engine, golden):
+    """The noisy heuristic's traces written by the reference's own move generation and weights
+    with the same draws (256 midgame positions x 16)."""
+    g = golden["noisy"]
+    announce("noisy heuristic traced, 256 x 16, against the reference's traces")
+    res = engine.run_traced(g["trace_pos"], per_pos=16, seed=12345, epoch=0,
+                            policy=POLICY_HEURISTIC_NOISY)
+    assert trace_mismatches(res, g["traces"]) == 0
+    assert res.stats["max_moves"] == int(g["traces"]["max_n"].max())
+
+
+def test_noisy_1024x1024_full_bitexact(engine, oracle, golden):
+    pos = golden["positions"]["mid1024"]
+    announce("noisy-1024x1024: 1024 midgame positions x 1024 noisy heuristic playouts")
+    res = engine.run_playouts(pos, per_pos=1024, seed=12345, epoch=0,
+                              policy=POLICY_HEURISTIC_NOISY)
+    w, tr = oracle.run_playouts(pos, per_pos=1024, seed=12345, epoch=0,
+                               policy=POLICY_HEURISTIC_NOISY)
+    assert np.array_equal(res.winners, w)
+    assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
+
+
+def test_noisy_outcomes_match_reference_driver(engine, golden):
+    """Winner counts of the noisy policy on the device against those of the reference's own
+    HostHeuristicPlayoutDriver (16 midgame positions x 16384 playouts, its own noise), by a
+    pooled two-sample chi-square test; random playouts from the same positions fail it."""
+    from scipy import stats
+    g = golden["noisy"]
+    per = 16384
+
+    def pooled_p(policy):
+        w = engine.run_playouts(g["driver_pos"], per_pos=per, seed=321, epoch=0,
+                                policy=policy).winners.reshape(-1, per)
+        chi, dof = 0.0, 0
+        for ref, row in zip(g["driver_counts"], w):
+            mine = np.array([(row == 0).sum(), (row == 1).sum(), (row == -1).sum()])
+            t = np.stack([ref, mine]).astype(np.float64)
+            t = t[:, t.sum(axis=0) > 0]
+            if t.shape[1] < 2:
+                continue
+            r = stats.chi2_contingency(t, correction=False)
+            chi += r.statistic
+            dof += r.dof
+        return float(stats.chi2.sf(chi, dof))
+
+    announce("noisy heuristic 16 x 16384 against the reference driver's winner counts")
+    assert pooled_p(POLICY_HEURISTIC_NOISY) > 1e-3
+    assert pooled_p(POLICY_RANDOM) < 1e-12
+
+
 def test_random_64x4096_bitexact(engine, oracle, golden):
     pos = golden["positions"]["open1024"][:64]
     announce("random-64x4096: 64 positions x 4096 random playouts")
@@ -129,7 +180,7 @@ def test_random_64x4096_bitexact(engine, oracle, golden):
 @pytest.mark.parametrize("block,bpc,refill,defer", [(64, 1, REFILL_STATIC, 1), (64, 16, REFILL_QUEUE, 16),
                                                     (256, 8, REFILL_STATIC, 8), (1024, 2, REFILL_QUEUE, 1),
                                                     (512, 4, REFILL_QUEUE, 64)])
-@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
+@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY])
 def test_launch_shape_does_not_change_results(engine, golden, block, bpc, refill, defer, policy):
     pos = golden["positions"]["mid1024"][:256]
     announce(f"tuning block {block} blocks/CU {bpc} refill {refill} defer {defer} policy {policy}")
@@ -166,7 +217,7 @@ def test_uneven_counts_empty_and_single(engine, oracle, golden):
     assert trace_mismatches(one, tr1) == 0
 
 
-@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
+@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY])
 def test_longest_first_order_uneven_counts(engine, oracle, golden, policy):
     """Batches large enough for the longest-first hand-out (at least 256 playouts per CU):
     uneven counts, zeros and repeated positions; every result slot and trace bit-exact."""
@@ -182,7 +233,7 @@ def test_longest_first_order_uneven_counts(engine, oracle, golden, policy):
     assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
 
 
-def test_batch_beyond_pinned_staging(engine, golden):
+def test_batch_beyond_pinned_staging(engine, oracle, golden):
     """A batch whose winners exceed the 64 MiB of pinned staging (copied straight to the
     caller's memory) returns what the same positions give in a small, staged call."""
     pos = golden["positions"]["open1024"]
@@ -191,9 +242,40 @@ def test_batch_beyond_pinned_staging(engine, golden):
     big = engine.run_playouts(pos, per_pos=per, seed=8, epoch=1)
     assert big.winners.size == len(pos) * per
     assert set(np.unique(big.winners)) <= {-1, 0, 1}
-    # a position's stream is keyed by its index in the batch: compare the batch's first three
-    first = engine.run_playouts(pos[:3], per_pos=per, seed=8, epoch=1)
-    assert np.array_equal(big.winners[:3 * per], first.winners)
+    # a position's stream is keyed by its index in the batch, so counts that are zero elsewhere
+    # pick any slice of it: the first three, a middle block and the last three positions
+    for lo, hi in ((0, 3), (500, 508), (1021, 1024)):
+        counts = np.zeros(len(pos), dtype=np.uint32)
+        counts[lo:hi] = per
+        part = engine.run_playouts(pos, counts=counts, seed=8, epoch=1)
+        assert np.array_equal(big.winners[lo * per:hi * per], part.winners), (lo, hi)
+    # and one position of the batch against the oracle
+    counts = np.zeros(len(pos), dtype=np.uint32)
+    counts[777] = per
+    w, _ = oracle.run_playouts(pos, counts=counts, seed=8, epoch=1, traces=False)
+    assert np.array_equal(big.winners[777 * per:778 * per], w)
+
+
+def test_direct_upload_beyond_pinned_staging(engine, oracle, golden):
+    """A batch whose positions and offsets exceed the 64 MiB of pinned staging (about 3.4M
+    positions, uploaded straight from the caller's memory): slices against small calls with zero
+    counts elsewhere, and against the oracle."""
+    base = golden["positions"]["open1024"]
+    pos = np.tile(base, 4096)  # 4,194,304 positions x 20 bytes of position and offset
+    counts = np.ones(len(pos), dtype=np.uint32)
+    counts[1::2] = 2
+    announce(f"mi_run_playouts with {len(pos)} positions (direct upload), "
+             f"{int(counts.sum())} playouts")
+    big = engine.run_playouts(pos, counts=counts, seed=3, epoch=5)
+    offs = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
+    assert big.winners.size == offs[-1]
+    for lo, hi in ((0, 4), (2_000_000, 2_000_006), (len(pos) - 4, len(pos))):
+        part_counts = np.zeros(len(pos), dtype=np.uint32)
+        part_counts[lo:hi] = counts[lo:hi]
+        part = engine.run_playouts(pos, counts=part_counts, seed=3, epoch=5)
+        assert np.array_equal(big.winners[offs[lo]:offs[hi]], part.winners), (lo, hi)
+        w, _ = oracle.run_playouts(pos, counts=part_counts, seed=3, epoch=5, traces=False)
+        assert np.array_equal(part.winners, w), (lo, hi)
 
 
 def test_terminal_and_odd_positions(engine, oracle, golden):
@@ -240,7 +322,7 @@ def test_device_start_positions_match_reference_golden(engine, golden):
 
 
 @pytest.mark.parametrize("refill", [REFILL_QUEUE, REFILL_STATIC])
-@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
+@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY])
 def test_graph_replays_match_eager(engine, golden, policy, refill):
     """Replays one graph on batches of different sizes (each replay sizes the playout grid to its
     batch), with eager launches and synchronisations in between, and checks every replay against

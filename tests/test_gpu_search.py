@@ -188,7 +188,7 @@ def test_keyed_playouts_match_oracle_and_ignore_batch_layout(engine, oracle, gol
             assert np.array_equal(plain.winners, eager.winners)
 
 
-@pytest.mark.parametrize("policy", [0, 1])
+@pytest.mark.parametrize("policy", [0, 1, 2])
 def test_driver_run_playouts_matches_oracle(golden, oracle, policy):
     """DeviceMI355XPlayoutDriver::runPlayouts on a leaf vector with repeated consecutive states
     (including finished games, as terminal tree nodes hand out copies): each call's winners equal

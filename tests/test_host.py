@@ -66,3 +66,22 @@ def test_run_ai_usage():
     from gpu_ai_amd import host_api
     out = subprocess.run([host_api.run_ai_path(), "-h"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0 and "gen_moves_test" in out.stdout
+
+
+def test_driver_names_select_the_reference_policies():
+    """getPlayoutDriver's names (src/playout.cpp:189-223): the reference's heuristic drivers
+    play the noisy heuristic (src/heuristicPlayout.cpp:33), hybrid_heuristic is a plain hybrid
+    driver there (src/playout.cpp:211-213), names it does not know are rejected."""
+    from gpu_ai_amd import host_api
+    from gpu_ai_amd.engine import POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY, POLICY_RANDOM
+    expect = {"host": POLICY_RANDOM, "device_single": POLICY_RANDOM,
+              "device_multiple": POLICY_RANDOM, "device_coarse": POLICY_RANDOM,
+              "hybrid": POLICY_RANDOM, "hybrid_heuristic": POLICY_RANDOM,
+              "optimal": POLICY_RANDOM, "device_mi355x": POLICY_RANDOM,
+              "host_heuristic": POLICY_HEURISTIC_NOISY, "device_heuristic": POLICY_HEURISTIC_NOISY,
+              "optimal_heuristic": POLICY_HEURISTIC_NOISY,
+              "device_mi355x_heuristic": POLICY_HEURISTIC_NOISY,
+              "device_mi355x_heuristic_argmax": POLICY_HEURISTIC,
+              "hybrid_heruistic": -1, "": -1, "Host": -1}
+    for name, policy in expect.items():
+        assert host_api.playout_policy(name) == policy, name

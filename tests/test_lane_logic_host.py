@@ -62,11 +62,12 @@ def test_lane_logic_matches_oracle(checker):
     assert choices > 10000
 
 
-@pytest.mark.parametrize("k", range(1, 7))
+@pytest.mark.parametrize("k", range(1, 10))
 def test_planted_oracle_mistake_is_caught(checker_obj, tmp_path, k):
     """The oracle rebuilt with one planted mistake (OR_MUTATION k in oracle/oracle.c: jump
-    order, king relanding, draw ply, promotion in captures, heuristic ties, Philox rounds) must
-    disagree with the lane logic."""
+    order, king relanding, draw ply, promotion in captures, heuristic ties, Philox rounds, the
+    noisy heuristic's sigma, its draws indexed in reference order) must disagree with the lane
+    logic."""
     so = str(tmp_path / f"liboracle_m{k}.so")
     _run(["gcc", "-O2", "-std=c11", "-fopenmp", "-fPIC", "-shared", f"-DOR_MUTATION={k}",
           os.path.join(ROOT, "oracle", "oracle.c"), "-o", so])

@@ -34,6 +34,16 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_host_library_exports_every_declared_symbol():
+    from gpu_ai_amd.build import HOST_LIB_PATH
+    assert os.path.exists(HOST_LIB_PATH), "build the host library first: python -m gpu_ai_amd.build"
+    lib = ctypes.CDLL(HOST_LIB_PATH)
+    names = declared("gpuai_")
+    assert "gpuai_playout_policy" in names and "gpuai_driver_run_playouts" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
 def test_strerror_without_device():
     from gpu_ai_amd import engine
     lib = engine.library()

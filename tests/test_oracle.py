@@ -275,7 +275,7 @@ def test_oracle_matches_reference_build_64_positions(oracle):
     ref = pyoracle.Reference()
     pos = oracle.start_positions(2016, 64, 0, 30, False)
     assert pos.tobytes() == ref.start_positions(2016, 64, 0, 30, False).tobytes()
-    for policy in (0, 1):
+    for policy in (0, 1, 2):
         _, a = oracle.run_playouts(pos, per_pos=64, seed=4242, epoch=3, policy=policy)
         _, b = ref.run_playouts(pos, per_pos=64, seed=4242, epoch=3, policy=policy)
         assert compare_traces(a, b) == 0

@@ -7,7 +7,8 @@ import sys
 import numpy as np
 import pytest
 
-from gpu_ai_amd.engine import MI_ERR_INVALID_ARGUMENT, POLICY_HEURISTIC, POLICY_RANDOM, MiError
+from gpu_ai_amd.engine import (MI_ERR_INVALID_ARGUMENT, POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY,
+                               POLICY_RANDOM, MiError)
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +40,7 @@ def _noalloc(engine, pos, per_pos, counts, seed, epoch, policy):
                 b.free()
 
 
-@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC])
+@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY])
 def test_noalloc_equals_run_playouts_twice(engine, golden, policy):
     pos = golden["positions"]["open1024" if policy == POLICY_RANDOM else "mid1024"]
     rng = np.random.default_rng(40 + policy)
