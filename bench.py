@@ -10,7 +10,7 @@ A step is one launch of the persistent playout kernel over the staged batch of t
                     heuristic playouts (largest weight, ties broken by the ply's word)
   noisy           : the same positions x 1024 playouts of the reference's heuristic, each move's
                     weight plus an N(0, 0.11) draw
-  small          : 64 positions x 4096 random playouts (the host path's own size)
+  small           : 64 positions x 4096 random playouts (the host path's own size)
   multisearch     : 256 concurrent MCTS searches sharing the device (one line, its own fields)
   value           : value-network training and evaluation (one line, its own fields)
   cpu             : CPU rates only (reference host code and restatement, 1 and all threads)
@@ -141,7 +141,7 @@ def bench_cpu(args) -> int:
     if os.path.exists(pyoracle.REF_SO):
         codes.insert(0, ("reference host code (oracle/_ref/libref.so)",
                          pyoracle.Reference(pyoracle.REF_SO),
-                         [("philox", 0), ("glibc rand()", 2)]))
+                         [("philox", 0), ("glibc rand()", pyoracle.POLICY_RANDOM_LIBC)]))
     rows = []
     for code, lib, choices in codes:
         for choice, policy in choices:
@@ -173,7 +173,11 @@ def bench_multisearch() -> int:
     runs = {m: host_api.multisearch(mode=m) for m in ("graph", "pipelined", "per_search")}
     g, pl, p = runs["graph"], runs["pipelined"], runs["per_search"]
     best = max(("graph", g), ("pipelined", pl), key=lambda kv: kv[1]["playouts_per_second"])
-    fmt = lambda r: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+    # per-search arrays (budgets, root trials) stay out of the line; their totals go in
+    fmt = lambda r: {**{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()
+                        if not hasattr(v, "shape")},
+                     "budget_total": int(r["budgets"].sum()),
+                     "root_trials_total": int(r["root_trials"].sum())}
     line = {
         "metric": "aggregate playouts/s of 256 concurrent MCTS searches on 1 MI355X",
         "value": round(best[1]["playouts_per_second"], 1), "unit": "playouts/s", "n_gpus": 1,
@@ -289,7 +293,7 @@ def bench_value(args) -> int:
                         "after_epochs": epochs + args.warmup},
         "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 3), "peak": 2500.0,
                      "unit": "TFLOP/s", "frac": round(achieved / 2.5e15, 5), "traffic": None,
-                     "kernel": "training step (k_encode, k_gemm x5, k_head_train, k_colsum, k_adam)",
+                     "kernel": "training step (k_train_rows, k_wgrad_part, k_adam)",
                      "flops_per_step": flops, "device_us_per_step": round(device_s / steps * 1e6, 2)},
     }
     print(json.dumps(line), flush=True)

@@ -27,6 +27,7 @@ TRACE_DTYPE = np.dtype([("winner", "i1"), ("max_n", "u1"), ("pad", "<u2"), ("pli
 POLICY_RANDOM = 0
 POLICY_HEURISTIC = 1        # without noise, ties broken by the ply's word
 POLICY_HEURISTIC_NOISY = 2  # the reference's N(0, 0.11) draw on each move's weight
+POLICY_RANDOM_LIBC = 9      # reference harness only: glibc rand() % n (timing, not reproducible)
 
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
