@@ -62,9 +62,6 @@ mi_pos *stage_pos(mi_mlp *n) { return (mi_pos *)n->h_stage; }
 float *stage_values(mi_mlp *n) { return (float *)((char *)n->h_stage + sizeof(mi_pos) * n->max_rows); }
 float *stage_targets(mi_mlp *n) { return stage_values(n) + n->max_rows; }
 
-// Rows per split of the weight-gradient GEMMs (K = rows): 128, or 64 when rows is an odd
-// multiple of 64, so a 4096-row batch gives 32 splits (512 and 256 workgroups for dW2, dW1).
-
 K::AdamArgs adam_args(mi_mlp *n, uint32_t rows, float lr, float c1, float c2, int update,
                       float *loss_out = nullptr) {
   K::AdamArgs a = {};
@@ -82,7 +79,7 @@ K::AdamArgs adam_args(mi_mlp *n, uint32_t rows, float lr, float c1, float c2, in
   a.P = n->P;
   a.splits = (rows + 255) / 256;
   a.pV = n->pV;
-  a.blocks = rows / 64;
+  a.blocks = rows / K::kTrainRows;
   a.W1t = n->W1t;
   a.W2t = n->W2t;
   a.W2r = n->W2r;
@@ -240,7 +237,7 @@ int mi_mlp_create(mi_ctx *ctx, uint32_t max_batch, uint32_t seed, mi_mlp **out) 
   if (e == hipSuccess) e = dev_alloc(n, &n->dH2T, 256 * R);
   if (e == hipSuccess) e = dev_alloc(n, &n->dH1T, 256 * R);
   if (e == hipSuccess) e = dev_alloc(n, &n->P, ((R + 255) / 256) * (size_t)(32768 + 65536));
-  if (e == hipSuccess) e = dev_alloc(n, &n->pV, (R / 32) * K::kVecSlots);
+  if (e == hipSuccess) e = dev_alloc(n, &n->pV, (R / K::kTrainRows) * K::kVecSlots);
   if (e == hipSuccess) e = dev_alloc(n, &n->pos, R);
   if (e == hipSuccess)
     e = hipHostMalloc(&n->h_stage, R * (sizeof(mi_pos) + 2 * sizeof(float)), hipHostMallocDefault);

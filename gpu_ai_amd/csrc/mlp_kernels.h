@@ -10,13 +10,17 @@
 namespace gpuai {
 namespace mlp {
 
-// Partial slots per 64-row block of k_train_rows: b1 at 0-255, b2 256-511, w3 512-767, b3 768,
+// Rows per workgroup of k_train_rows (a multiple of 16, dividing 64): the training batch is a
+// multiple of 64 rows.
+constexpr uint32_t kTrainRows = 32;
+
+// Partial slots per row block of k_train_rows: b1 at 0-255, b2 256-511, w3 512-767, b3 768,
 // loss 769.
 constexpr uint32_t kVecSlots = 1024;
 
-// One training step's row pass (k_train_rows), 64 rows per workgroup: row r < n is
+// One training step's row pass (k_train_rows), kTrainRows rows per workgroup: row r < n is
 // pos[idx ? idx[r] : r] with target targets[idx ? idx[r] : r]; rows n and up (to the multiple
-// of 64) are padding.  Writes the transposed activations X^T, H1^T, dH2^T, dH1^T ([feature][ld],
+// of kTrainRows) are padding.  Writes the transposed activations X^T, H1^T, dH2^T, dH1^T ([feature][ld],
 // the step's rows at 0..rows-1) and block b's vector partials at pV + b * kVecSlots.
 struct TrainArgs {
   const mi_pos *pos;
@@ -39,7 +43,7 @@ struct AdamArgs {
   uint32_t rows, ld;
   float *P;          // weight-gradient partials: [splits][W1 32768 | W2 65536]
   uint32_t splits;   // ceil(rows / 256)
-  const float *pV;   // the row pass's vector partials, `blocks` of them (rows / 64)
+  const float *pV;   // the row pass's vector partials, `blocks` of them (rows / kTrainRows)
   uint32_t blocks;
   uint16_t *W1t, *W2t, *W2r;
   float lr, beta1, beta2, eps, c1, c2;  // c1 = 1/(1-beta1^t), c2 = 1/(1-beta2^t)
