@@ -549,7 +549,7 @@ int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin
 namespace {
 
 // Measures every candidate launch shape on the uploaded batch -- workgroup sizes 64..1024, 1..8
-// resident waves per SIMD, both refill policies, deferral thresholds 1, 8 and 16 -- `reps`
+// resident waves per SIMD, both refill policies, deferral thresholds 1, 8, 16 and 32 -- `reps`
 // launches each (median kernel time), records the fastest of each refill policy in the tuning
 // table's entry for the batch's policy and size bucket, and returns the fastest overall.
 int tune_sweep(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, bool with_offs,
@@ -559,7 +559,9 @@ int tune_sweep(mi_ctx *ctx, uint32_t npos, uint32_t per_pos, uint64_t total, boo
   // 3 and 4 waves per SIMD are the fast ones on random-1024x1024 (more lanes in flight lengthen
   // the end of a batch, fewer leave the SIMDs short of waves to issue from)
   const uint32_t waves_per_simd[] = {1u, 2u, 3u, 4u, 8u};
-  const uint32_t defers[] = {1u, 8u, 16u};
+  // 32 for the noisy heuristic, whose deferred plies make the draws: 1.07 against 1.16-1.19 ms
+  // at 16 on noisy-1024x1024 (the other policies gain nothing past 16)
+  const uint32_t defers[] = {1u, 8u, 16u, 32u};
   mi_tuning refill_best[2] = {};
   float refill_ms[2] = {3.0e38f, 3.0e38f};
   uint32_t n = 0;

@@ -10,9 +10,10 @@
 // LDS, and writes what the weight gradients need transposed (X^T, H1^T, dH2^T, dH1^T:
 // [feature][row], rows as K) plus each block's partial sums of the vector gradients.  Every
 // workgroup reads all of the BF16 weights (320 KB) from L2; 32 rows per workgroup measured
-// 36.2 us per 4096-row step against 38.6 with 64 rows and 39.3 with 16.  k_wgrad_part forms 64x64 tiles of dW1 and dW2
-// over 256-row splits of the batch (split-K: 24 tiles x 16 splits for 4096 rows), and k_adam sums
-// every gradient's partials in a fixed order and applies Adam, refreshing the BF16 copies.
+// 36.2 us per 4096-row step against 38.6 with 64 rows and 39.3 with 16.  k_wgrad_part forms
+// 64x64 tiles of dW1 and dW2 over 256-row splits of the batch (split-K: 24 tiles x 16 splits for
+// 4096 rows), and k_adam sums every gradient's partials in a fixed order and applies Adam,
+// refreshing the BF16 copies.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -143,8 +144,8 @@ __global__ void __launch_bounds__(256) k_forward(const mi_pos *__restrict__ pos,
 
 // ---------------------------------------------------------------------------------------------
 // Training.  Row r of the batch is pos[idx ? idx[r] : r] with target targets[idx ? idx[r] : r];
-// rows n and up (to the multiple of 32) are padding: zero inputs and no loss.  The arithmetic:
-// H1 = bf(relu(X W1 + b1)), H2 = bf(relu(H1 W2 + b2)), y = tanh(H2 . w3 + b3),
+// rows n and up (to the multiple of kTrainRows) are padding: zero inputs and no loss.  The
+// arithmetic: H1 = bf(relu(X W1 + b1)), H2 = bf(relu(H1 W2 + b2)), y = tanh(H2 . w3 + b3),
 // dz = 2 (y - t)(1 - y^2) / n, dH2 = bf(dz w3) where H2 > 0, dH1 = bf(dH2 W2^T) where H1 > 0;
 // dW1 = X^T dH1, dW2 = H1^T dH2, b1 and b2 the column sums of dH1 and dH2, w3 = sum of dz H2,
 // b3 = sum of dz, loss = sum of (y - t)^2 / n.
@@ -502,13 +503,15 @@ __global__ void __launch_bounds__(256) k_wgrad_part(AdamArgs a) {
         P[(m0 + qr + r * 16 + 4 * (lane >> 4) + v) * 256u + n0 + qc + c * 16 + lr] = acc[r][c][v];
 }
 
-// Adam over every parameter.  Each gradient is the sum of its partials in split order: W1 and W2
-// over the weight-gradient splits (P), b1, b2, w3 and b3 over the row pass's row blocks (pV;
-// slot 769 holds the loss).  Workgroups take 16x16 tiles of W1 ([128][256]) and W2 ([256][256]),
-// then the 769 vector parameters; the BF16 copies are refreshed, the transposed ones (W1t, W2t:
-// [out][in]) through an LDS tile so that their stores are contiguous.
+// Adam over every parameter.  Each gradient is the sum of its partials in a fixed order: W1 and
+// W2 over the weight-gradient splits (P), in split order; b1, b2, w3 and b3 over the row pass's
+// row blocks (pV; slot 769 holds the loss), as eight sums over consecutive eighths of the blocks,
+// added in order (so each thread's loads are one round: 128 blocks at 4096 rows).  Workgroups
+// take 16x16 tiles of W1 ([128][256]) and W2 ([256][256]), then the vector parameters, 32 slots
+// per workgroup; the BF16 copies are refreshed, the transposed ones (W1t, W2t: [out][in])
+// through an LDS tile so that their stores are contiguous.
 constexpr uint32_t AT = 16, kAdamW1 = (128 / AT) * (256 / AT), kAdamW2 = (256 / AT) * (256 / AT);
-constexpr uint32_t kAdamVec = (769 + 255) / 256;
+constexpr uint32_t kVecPerWg = 32, kAdamVec = (770 + kVecPerWg - 1) / kVecPerWg;
 
 __global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
   __shared__ uint16_t tile[AT][AT + 2];
@@ -530,21 +533,24 @@ __global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
     (w1 ? a.W1t : a.W2t)[(o0 + ro) * (w1 ? 128u : 256u) + i0 + ri] = tile[ro][ri];
     return;
   }
-  const uint32_t slot = (b - kAdamW1 - kAdamW2) * 256 + t;  // b1 0-255, b2 256-511, w3 512-767,
-  if (slot > 769u) return;                                   // b3 768, loss 769
-  const uint32_t p = slot < 256u ? MI_MLP_OFF_B1 + slot : MI_MLP_OFF_B2 + (slot - 256u);
-  float w = 0.f, m = 0.f, v = 0.f;
-  if (slot < 769u) {
-    w = a.w[p];
-    m = a.m[p];
-    v = a.v[p];
-  }
-  const float g = sum_partials(a.pV + slot, kVecSlots, a.blocks);
+  __shared__ float eighth[8][kVecPerWg];
+  const uint32_t sl = t % kVecPerWg, e = t / kVecPerWg;  // slot in the workgroup, eighth
+  const uint32_t slot = (b - kAdamW1 - kAdamW2) * kVecPerWg + sl;  // b1 0-255, b2 256-511,
+  const uint32_t per = (a.blocks + 7u) / 8u;                      // w3 512-767, b3 768, loss 769
+  const uint32_t lo = min(e * per, a.blocks), hi = min(lo + per, a.blocks);
+  eighth[e][sl] = slot <= 769u ? sum_partials(a.pV + (size_t)lo * kVecSlots + slot, kVecSlots, hi - lo)
+                               : 0.f;
+  lds_barrier();
+  if (e != 0u || slot > 769u) return;
+  float g = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; k++) g += eighth[k][sl];
   if (slot == 769u) {
     if (a.loss_out) *a.loss_out = g;
     return;
   }
-  adam_step(a, p, g, w, m, v);
+  const uint32_t p = slot < 256u ? MI_MLP_OFF_B1 + slot : MI_MLP_OFF_B2 + (slot - 256u);
+  adam_step(a, p, g, a.w[p], a.m[p], a.v[p]);
 }
 
 __global__ void k_zero(float *p, uint32_t n) {

@@ -170,8 +170,8 @@ int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin
 /* Autotuner (no counterpart in the reference, whose OptimalPlayoutDriver picks between whole
  * drivers by a runtime model of the batch size, src/playout.cpp:80-166).  Runs the given batch
  * under every candidate launch shape -- workgroup sizes 64..1024, 1..8 resident waves per SIMD
- * (which sets the playouts each lane runs), both refill policies and deferral thresholds 1, 8 and
- * 16 -- `reps` times each and takes the one with the smallest median kernel time.
+ * (which sets the playouts each lane runs), both refill policies and deferral thresholds 1, 8, 16
+ * and 32 -- `reps` times each and takes the one with the smallest median kernel time.
  *
  * The context keeps a launch-shape table: per policy, batch-size bucket (batches of 2^b to
  * 2^(b+1) - 1 playouts) and refill policy, the fastest shape measured.  Every tuning run records
