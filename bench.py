@@ -21,16 +21,16 @@ roofline: the path is integer- and branch-bound.  Its work unit is one ply (gene
 apply, terminal test) priced at a frozen W = 192 int32 lane-ops; achieved = plies x W / average
 kernel time (HIP events on the library's stream), against the chip's int32 VALU issue rate
 CUs x 64 lanes x clock; peak_measured is that rate as a probe kernel of plain v_add_u32 /
-v_xor_b32 chains reaches it on this device (mi_valu_rate).  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
-recorded in profiles/ (null when absent); roofline.hbm sets those bytes over the kernel time against
-the 8 TB/s HBM peak (the path is not memory bound).
+v_xor_b32 chains reaches it on this device (mi_valu_rate).  traffic: HBM bytes per launch from
+rocprofv3 FETCH_SIZE/WRITE_SIZE runs recorded in profiles/ (null when absent); roofline.hbm sets
+those bytes over the kernel time against the 8 TB/s HBM peak (the path is not memory bound).
 cpu_baseline: the reference's own host code (oracle/_ref/libref.so: State::genMoves twice per ply
 as src/playout.cpp:17-32 does, OpenMP over playouts, Philox draws) on a bounded sample of the same
 workload, on the job's whole CPU share (OMP_NUM_THREADS, else every CPU it may use) and on one
 thread, with the host's nproc, affinity and CPU model.
 net_leaf_win_rate: the win-rate half of the metric, measured in the run (random workload, unless
---no-match or --tuning): the value network is trained on the device (2M positions labelled by 256
-random playouts, 24 epochs, seconds), then 400 games of network leaves against playout leaves at
+--no-match or --tuning): the value network is trained on the device (8M positions labelled by 256
+random playouts, 48 epochs, seconds), then 400 games of network leaves against playout leaves at
 250 ms per move each (about 2.5 minutes; progress on stderr).  The network's score is
 (wins + draws / 2) / games with its standard error.
 """
@@ -201,11 +201,14 @@ def bench_multisearch() -> int:
 # and the network player's best k against it.
 MATCH_NET_K = 1024.0
 MATCH_PLAYOUT_K = 256.0
+# The network of the match: value MSE 0.0055 against 1024-playout estimates (2M positions x 24
+# epochs: 0.0088; the match score did not change measurably between them)
+MATCH_POSITIONS, MATCH_EPOCHS = 8_000_000, 48
 
 
 def measured_match(eng, args) -> dict:
-    """Trains the value network in the run (2M positions labelled by 256 random playouts each,
-    k in [0, 150] plies, 24 epochs of batch 4096, Adam lr 1e-3) and plays the 400-game match of
+    """Trains the value network in the run (8M positions labelled by 256 random playouts each,
+    k in [0, 150] plies, 48 epochs of batch 4096, Adam lr 1e-3) and plays the 400-game match of
     network leaves against playout leaves at equal wall time per move (netmatch.hpp): the
     measured half of the metric.  Progress lines go to stderr."""
     from gpu_ai_amd import host_api, value
@@ -215,17 +218,17 @@ def measured_match(eng, args) -> dict:
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
-    pos, t = value.make_dataset(eng, 2_000_000, POS_SEED, kmax=150, playouts=256)
+    pos, t = value.make_dataset(eng, MATCH_POSITIONS, POS_SEED, kmax=150, playouts=256)
     label_s = time.perf_counter() - t0
     log(f"labelled {len(pos)} positions with 256 playouts each in {label_s:.1f} s")
     with ValueNet(eng, 4096, seed=POS_SEED) as net:
         t1 = time.perf_counter()
-        hist = value.train(net, pos, t, 24, 4096, 1e-3)
+        hist = value.train(net, pos, t, MATCH_EPOCHS, 4096, 1e-3)
         train_s = time.perf_counter() - t1
         del pos, t
         ev = value.evaluate(eng, net, 16384, ks=(1, 16, 256))
         params = net.get_params()
-    log(f"trained 24 epochs in {train_s:.1f} s (loss {hist[-1]['loss']:.4f}, value MSE "
+    log(f"trained {MATCH_EPOCHS} epochs in {train_s:.1f} s (loss {hist[-1]['loss']:.4f}, value MSE "
         f"{ev['mse_net']:.4f}); playing {args.match_games} games at {args.match_ms:g} ms per move")
     t2 = time.perf_counter()
     r = host_api.net_match(params, games=args.match_games, seed=POS_SEED, move_ms=args.match_ms,
@@ -239,8 +242,8 @@ def measured_match(eng, args) -> dict:
             "draws": r["draws"], "move_ms": args.match_ms, "openings_seed": POS_SEED,
             "net_exploration": MATCH_NET_K, "playout_exploration": args.match_playout_k,
             "leaves": {"network": r["net_leaves"], "playouts": r["playout_leaves"]},
-            "training": {"positions": 2_000_000, "labels": "256 random playouts",
-                         "epochs": 24, "label_s": round(label_s, 2),
+            "training": {"positions": MATCH_POSITIONS, "labels": "256 random playouts",
+                         "epochs": MATCH_EPOCHS, "label_s": round(label_s, 2),
                          "train_s": round(train_s, 2), "final_loss": round(hist[-1]["loss"], 5),
                          "value_mse": round(ev["mse_net"], 5),
                          "mse_playouts": {k: round(v, 5) for k, v in ev["mse_playouts"].items()}},

@@ -169,8 +169,9 @@ __device__ __forceinline__ void store_transposed(const uint16_t (*T)[TP], int fe
 
 // The forward and backward pass of KR rows: wave w (of eight) owns columns 32w..32w+31 of each
 // 256-wide layer (two 16x16 column tiles, RT row tiles).  Each workgroup reads the whole of the
-// BF16 weights (320 KB), so the rows per workgroup set the weight traffic through L2.  A operands come from LDS ([row][k]),
-// B operands from the BF16 weight copies in global memory (W1t, W2t: [out][in]; W2r: [in][out]).
+// BF16 weights (320 KB), so the rows per workgroup set the weight traffic through L2.  A operands
+// come from LDS ([row][k]), B operands from the BF16 weight copies in global memory (W1t, W2t:
+// [out][in]; W2r: [in][out]).
 // Every global load is issued at the start (a dependent load waits out the memory latency, and
 // a layer's panel loaded slice by slice would wait it out once per 32-deep slice); the pass then
 // only computes and stores.
