@@ -77,7 +77,7 @@ K::AdamArgs adam_args(mi_mlp *n, uint32_t rows, float lr, float c1, float c2, in
   a.rows = rows;
   a.ld = n->max_rows;
   a.P = n->P;
-  a.splits = (rows + 255) / 256;
+  a.splits = (rows + K::kWSplitRows - 1) / K::kWSplitRows;
   a.pV = n->pV;
   a.blocks = rows / K::kTrainRows;
   a.W1t = n->W1t;
@@ -236,7 +236,8 @@ int mi_mlp_create(mi_ctx *ctx, uint32_t max_batch, uint32_t seed, mi_mlp **out) 
   if (e == hipSuccess) e = dev_alloc(n, &n->H1T, 256 * R);
   if (e == hipSuccess) e = dev_alloc(n, &n->dH2T, 256 * R);
   if (e == hipSuccess) e = dev_alloc(n, &n->dH1T, 256 * R);
-  if (e == hipSuccess) e = dev_alloc(n, &n->P, ((R + 255) / 256) * (size_t)(32768 + 65536));
+  if (e == hipSuccess)
+    e = dev_alloc(n, &n->P, ((R + K::kWSplitRows - 1) / K::kWSplitRows) * (size_t)(32768 + 65536));
   if (e == hipSuccess) e = dev_alloc(n, &n->pV, (R / K::kTrainRows) * K::kVecSlots);
   if (e == hipSuccess) e = dev_alloc(n, &n->pos, R);
   if (e == hipSuccess)

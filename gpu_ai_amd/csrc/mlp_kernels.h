@@ -14,6 +14,9 @@ namespace mlp {
 // multiple of 64 rows.
 constexpr uint32_t kTrainRows = 32;
 
+// Rows per split of the weight-gradient products (split-K over the batch; a multiple of 32).
+constexpr uint32_t kWSplitRows = 128;
+
 // Partial slots per row block of k_train_rows: b1 at 0-255, b2 256-511, w3 512-767, b3 768,
 // loss 769.
 constexpr uint32_t kVecSlots = 1024;

@@ -11,9 +11,10 @@
 // [feature][row], rows as K) plus each block's partial sums of the vector gradients.  Every
 // workgroup reads all of the BF16 weights (320 KB) from L2; 32 rows per workgroup measured
 // 36.2 us per 4096-row step against 38.6 with 64 rows and 39.3 with 16.  k_wgrad_part forms
-// 64x64 tiles of dW1 and dW2 over 256-row splits of the batch (split-K: 24 tiles x 16 splits for
-// 4096 rows), and k_adam sums every gradient's partials in a fixed order and applies Adam,
-// refreshing the BF16 copies.
+// 64x64 tiles of dW1 and dW2 over kWSplitRows-row splits of the batch (split-K: 24 tiles x 32
+// splits of 128 rows for 4096 rows; 128 measured 31.6 us per step, 256 32.1, 512 32.3), and
+// k_adam sums every gradient's partials in a fixed order and applies Adam, refreshing the BF16
+// copies.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -439,14 +440,16 @@ __device__ __forceinline__ float sum_partials(const float *p, size_t stride, uin
 }
 
 // Weight-gradient partials: workgroup (tile, split) forms one 64x64 tile of dW2 (tiles 0..15,
-// [256 in][256 out]) or dW1 (16..23, [128][256]) over the split's rows [256 split, +256) of the
+// [256 in][256 out]) or dW1 (16..23, [128][256]) over the split's rows [S split, +S) of the
 // batch, dW1[j][i] = sum_r X^T[j][r] dH1^T[i][r], dW2[i][o] = sum_r H1^T[i][r] dH2^T[o][r], and
-// writes it to P + split * kWParams (W1 at 0, W2 at 32768, each [in][out]).  The split's operand
-// rows (64 features x 256 rows each, 64 KB) come in with one round of 16-byte loads (16 per
-// thread in flight) into LDS; rows past the batch read as zero.  Wave w computes the 32x32
-// quadrant (w / 2, w % 2) as 2 x 2 MFMA tiles over eight 32-deep slices.
-constexpr uint32_t kWSplit = 256, kWTiles = 24, kWParams = 32768 + 65536;
-constexpr int WP = 256 + 8;  // LDS pitch of the staged operand rows
+// writes it to P + split * kWParams (W1 at 0, W2 at 32768, each [in][out]); S = kWSplitRows.  The
+// split's operand rows (64 features x S rows each) come in with one round of 16-byte loads (S/16
+// per thread in flight) into LDS; rows past the batch read as zero.  Wave w computes the 32x32
+// quadrant (w / 2, w % 2) as 2 x 2 MFMA tiles over S/32 slices of 32.
+constexpr uint32_t kWSplit = kWSplitRows, kWTiles = 24, kWParams = 32768 + 65536;
+constexpr int WP = (int)kWSplit + 8;  // LDS pitch of the staged operand rows
+constexpr uint32_t kWLoads = kWSplit / 32;  // 16-byte loads per thread and operand
+static_assert(kWSplit % 32 == 0, "split rows: a multiple of 32");
 
 __global__ void __launch_bounds__(256) k_wgrad_part(AdamArgs a) {
   __shared__ __attribute__((aligned(16))) uint16_t As[64][WP];
@@ -459,17 +462,18 @@ __global__ void __launch_bounds__(256) k_wgrad_part(AdamArgs a) {
   const uint16_t *A = w1 ? a.XT : a.H1T;
   const uint16_t *B = w1 ? a.dH1T : a.dH2T;
   const uint32_t k0 = split * kWSplit;
-  uint4 va[8], vb[8];
+  constexpr uint32_t UR = kWSplit / 8;  // 16-byte units per operand row
+  uint4 va[kWLoads], vb[kWLoads];
 #pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const uint32_t u = t + 256u * j, row = u >> 5, k = k0 + (u & 31u) * 8u;
+  for (uint32_t j = 0; j < kWLoads; j++) {
+    const uint32_t u = t + 256u * j, row = u / UR, k = k0 + (u % UR) * 8u;
     const bool in = k < a.rows;
     va[j] = in ? *(const uint4 *)(A + (size_t)(m0 + row) * a.ld + k) : uint4{0u, 0u, 0u, 0u};
     vb[j] = in ? *(const uint4 *)(B + (size_t)(n0 + row) * a.ld + k) : uint4{0u, 0u, 0u, 0u};
   }
 #pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const uint32_t u = t + 256u * j, row = u >> 5, c = (u & 31u) * 8u;
+  for (uint32_t j = 0; j < kWLoads; j++) {
+    const uint32_t u = t + 256u * j, row = u / UR, c = (u % UR) * 8u;
     *(uint4 *)&As[row][c] = va[j];
     *(uint4 *)&Bs[row][c] = vb[j];
   }
@@ -482,7 +486,7 @@ __global__ void __launch_bounds__(256) k_wgrad_part(AdamArgs a) {
 #pragma unroll
     for (int c = 0; c < 2; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int s = 0; s < 8; s++) {
+  for (uint32_t s = 0; s < kWSplit / 32; s++) {
     bf16x8 x[2], y[2];
 #pragma unroll
     for (int r = 0; r < 2; r++) x[r] = *(const bf16x8 *)&As[qr + r * 16 + lr][s * 32 + kg];
