@@ -228,11 +228,16 @@ int main(int argc, char **argv) {
   const int N = argc > 1 ? std::atoi(argv[1]) : 1024;
   const int P = argc > 2 ? std::atoi(argv[2]) : 8;
   long bad = 0, playouts = 0, deferred = 0, lists = 0, mixed = 0;
-  for (int set = 0; set < 3; set++) {
+  for (int set = 0; set < 4; set++) {
     std::vector<or_pos> pos(N);
     if (set == 0) or_gen_start_positions(2016, N, 0, 30, 0, pos.data());
     else if (set == 1) or_gen_start_positions(777, N, 20, 60, 1, pos.data());
-    else or_gen_start_positions(99, N, 40, 120, 1, pos.data());
+    else if (set == 2) or_gen_start_positions(99, N, 40, 120, 1, pos.data());
+    else {  // random placements, draw counters near the limit: finished games, captures of the
+            // last piece (heuristic weight +inf), long king sequences
+      pos = placements(N);
+      for (int i = 0; i < N; i++) pos[i].counter = (uint8_t)(i % 4 == 0 ? 48 : i % 4 == 1 ? 49 : 0);
+    }
     for (int policy = 0; policy < 3; policy++) {
       std::vector<or_trace> ot((size_t)N * P);
       or_run_playouts(pos.data(), N, nullptr, P, 12345, 3, policy, nullptr, ot.data(), 0);

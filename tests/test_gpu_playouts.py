@@ -283,7 +283,7 @@ def test_terminal_and_odd_positions(engine, oracle, golden):
     # random placements and king-loop positions sit at the end of the golden set; they include
     # finished games (counter >= 50, no moves) and men on rows a game never leaves them on
     odd = ml[-3300:]
-    for policy in (POLICY_RANDOM, POLICY_HEURISTIC):
+    for policy in (POLICY_RANDOM, POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY):
         announce(f"odd positions x 8, policy {policy}")
         res = engine.run_traced(odd, per_pos=8, seed=31, epoch=0, policy=policy)
         _, tr = oracle.run_playouts(odd, per_pos=8, seed=31, epoch=0, policy=policy)
