@@ -45,7 +45,12 @@ void usage(const char *argv0) {
                "   $GPUAI_VALUE_NET, default ./value.ckpt)\n"
                "  playout drivers: host host_heuristic device_single device_heuristic"
                " device_multiple device_coarse hybrid hybrid_heuristic optimal"
-               " optimal_heuristic device_mi355x device_mi355x_heuristic\n";
+               " optimal_heuristic device_mi355x device_mi355x_heuristic"
+               " device_mi355x_heuristic_argmax\n"
+               "  (the *_heuristic drivers add N(0, 0.11) noise to each move's weight, as the"
+               " reference's do; hybrid_heuristic plays random playouts, as the reference's"
+               " factory makes it; device_mi355x_heuristic_argmax is the heuristic without"
+               " noise)\n";
 }
 
 Mode parseMode(const std::string &s) {
