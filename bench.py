@@ -349,8 +349,9 @@ def main() -> int:
         b, bpc, refill, defer = (int(x) for x in args.tuning.split(","))
         eng.set_tuning(b, bpc, refill, defer)
     elif not args.no_autotune:
-        # the autotuner times every candidate launch shape on this batch (its own Philox epochs)
-        tuned, _ = eng.autotune(pos, per_pos=cfg["per_pos"], policy=cfg["policy"], reps=3)
+        # the autotuner times every candidate launch shape on this batch, five launches each (its
+        # own Philox epochs), and keeps the one of smallest median time
+        tuned, _ = eng.autotune(pos, per_pos=cfg["per_pos"], policy=cfg["policy"], reps=5)
     total = eng.stage(pos, per_pos=cfg["per_pos"])
     for i in range(args.warmup):
         eng.launch_staged(seed=PLAY_SEED, epoch=1000 + i, policy=cfg["policy"])
