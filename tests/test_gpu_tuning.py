@@ -108,3 +108,30 @@ def test_autotune_sizes_and_set_tuning(tuned):
         eng.set_tuning(128, 2, REFILL_QUEUE, 16)
         assert eng.tuning_table() == []
         assert eng.run_playouts(pos[:1024], per_pos=2, seed=1).stats["block_size"] == 128
+
+
+def test_table_keeps_policies_apart(tuned):
+    """Each policy has its own rows of the table: tuning the noisy heuristic at a size leaves the
+    random policy's entry of that bucket as it was, and each launch takes its own policy's shape
+    with results equal to those of a fixed shape."""
+    from gpu_ai_amd.engine import POLICY_HEURISTIC_NOISY
+    _, pos, _ = tuned
+    with Engine(0) as eng:
+        announce("mi_autotune_sizes 16384, random then noisy heuristic")
+        rnd = eng.autotune_sizes(pos, [16384], policy=POLICY_RANDOM, reps=1)[0]
+        before = {(e["policy"], e["bucket"], e["refill"]): e for e in eng.tuning_table()}
+        noisy = eng.autotune_sizes(pos, [16384], policy=POLICY_HEURISTIC_NOISY, reps=1)[0]
+        table = {(e["policy"], e["bucket"], e["refill"]): e for e in eng.tuning_table()}
+        assert {p for p, _, _ in table} == {POLICY_RANDOM, POLICY_HEURISTIC_NOISY}
+        for key, e in before.items():
+            assert table[key] == e  # the random rows are untouched
+        assert rnd["policy"] == POLICY_RANDOM and noisy["policy"] == POLICY_HEURISTIC_NOISY
+        sel = pos[:1024]  # 1024 x 16 playouts: the tuned bucket
+        for policy, best in ((POLICY_RANDOM, rnd), (POLICY_HEURISTIC_NOISY, noisy)):
+            r = eng.run_playouts(sel, per_pos=16, seed=3, epoch=1, policy=policy)
+            assert (r.stats["block_size"], r.stats["refill"]) == \
+                (best["block_size"], best["refill"]), policy
+            with Engine(0) as fixed:
+                fixed.set_tuning(256, 4, REFILL_QUEUE, 8)
+                ref = fixed.run_playouts(sel, per_pos=16, seed=3, epoch=1, policy=policy)
+            assert np.array_equal(r.winners, ref.winners), policy
