@@ -55,6 +55,23 @@ def test_gen_moves_golden(engine, golden):
     assert flat.tobytes() == moves.tobytes()
 
 
+def test_unknown_policy_is_rejected(engine, golden):
+    """Policy numbers past MI_POLICY_HEURISTIC_NOISY are refused by every entry point that takes
+    one, before anything runs, and the context stays usable."""
+    from gpu_ai_amd.engine import MI_ERR_INVALID_ARGUMENT, MiError
+    pos = golden["positions"]["open1024"][:8]
+    announce("unknown policy 3 at run, traced, autotune and graph creation")
+    for call in (lambda: engine.run_playouts(pos, per_pos=4, policy=3),
+                 lambda: engine.run_traced(pos, per_pos=4, policy=3),
+                 lambda: engine.autotune(pos, per_pos=4, policy=3, reps=1),
+                 lambda: engine.graph(8, 64, policy=3)):
+        with pytest.raises(MiError) as e:
+            call()
+        assert e.value.code == MI_ERR_INVALID_ARGUMENT
+    ok = engine.run_playouts(pos, per_pos=4, policy=POLICY_HEURISTIC_NOISY)
+    assert ok.winners.size == 32
+
+
 def test_gen_moves_capacity_error(engine, golden):
     from gpu_ai_amd.engine import MI_ERR_CAPACITY, MiError
     pos = golden["positions"]["open1024"][:64]
