@@ -14,8 +14,12 @@
  * with each random draw taken from a counter-based Philox4x32-10 stream: the word for ply t of
  * playout j of position i is word (t & 3) of Philox(key = (seed, i),
  * counter = (t >> 2, j, 0, epoch)).  A random ply picks move (r mod n) of the reference-ordered
- * list; a heuristic ply picks T[r mod |T|] of the reference-ordered set T of moves of maximal
- * weight (S_mover + d_mover) / (S_opp + d_opp), compared exactly.
+ * list.  A noisy heuristic ply (MI_POLICY_HEURISTIC_NOISY, the reference's) adds to each move's
+ * weight (S_mover + d_mover) / (S_opp + d_opp) an N(0, 0.11) draw made from r and the move's
+ * index and plays the first largest sum; where all weights are equal it plays move (r mod n),
+ * the argmax of iid draws being uniform there.  A plain heuristic ply (MI_POLICY_HEURISTIC)
+ * picks T[r mod |T|] of the reference-ordered set T of moves of maximal weight, compared
+ * exactly.
  *
  * Squares: s = 4*row + (col >> 1), col = 2*(s & 3) + (row even ? 1 : 0) (src/state.cu:173-176).
  */
@@ -74,8 +78,9 @@ typedef struct mi_tuning {
   uint32_t block_size;      /* threads per workgroup: 64..1024, a multiple of 64 */
   uint32_t blocks_per_cu;   /* resident workgroups per compute unit for the persistent grid */
   uint32_t refill;          /* MI_REFILL_* */
-  uint32_t defer_min;       /* a wave plays its parked multi-jump plies (both policies) once this
-                               many lanes wait (or all active lanes wait, or 8 iterations pass) */
+  uint32_t defer_min;       /* a wave plays its parked plies (multi-jump captures; for the noisy
+                               heuristic also plies whose moves weigh differently) once this many
+                               lanes wait (or all active lanes wait, or 8 iterations pass) */
 } mi_tuning;
 
 /* Per-call statistics. */
