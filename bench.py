@@ -433,7 +433,11 @@ def main() -> int:
         except Exception as e:  # the baseline is reported, never required for the GPU figure
             line["cpu_baseline"] = {"value": None, "error": str(e)}
     if args.workload == "random" and not args.no_match and not args.tuning:
-        line["net_leaf_win_rate"] = measured_match(eng, args)
+        try:
+            line["net_leaf_win_rate"] = measured_match(eng, args)
+        except Exception as e:  # the throughput line is printed whatever happens to the match
+            print(f"[bench] match failed: {e}", file=sys.stderr, flush=True)
+            line["net_leaf_win_rate"] = {"score": None, "error": str(e)}
     else:
         line["net_leaf_win_rate"] = None  # skipped: --no-match, a profiled run, or another workload
     eng.close()
