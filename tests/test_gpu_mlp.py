@@ -91,7 +91,7 @@ def test_mlp_forward_colour_symmetry(engine, positions):
         np.testing.assert_array_equal(net.forward(pos), net.forward(M.mirror(pos)))
 
 
-@pytest.mark.parametrize("n", [200, 512, 2048, 4096])
+@pytest.mark.parametrize("n", [200, 300, 512, 2048, 4096])  # 300: a half-filled last split
 def test_mlp_gradients_match_oracle(engine, positions, n):
     w = _params(6)
     pos = positions[:n]
