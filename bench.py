@@ -251,10 +251,11 @@ def measured_match(eng, args) -> dict:
 
 
 # FLOPs of one training step of the 128-256-256-1 network on a batch of B positions: forward
-# 2B(128*256 + 256*256 + 256), backward 2B(128*256 + 2*256*256) (dW1, dW2, dH1) + 2B*256 (dw3)
+# 2B(128*256 + 256*256 + 256), backward 2B(128*256 + 2*256*256) (dW1, dW2, dH1) + 2B*2*256
+# (dw3, dH2): 262,912 multiply-adds per position
 def mlp_step_flops(batch: int) -> int:
     fwd = 2 * batch * (128 * 256 + 256 * 256 + 256)
-    bwd = 2 * batch * (128 * 256 + 2 * 256 * 256 + 256)
+    bwd = 2 * batch * (128 * 256 + 2 * 256 * 256 + 2 * 256)
     return fwd + bwd
 
 
