@@ -40,6 +40,7 @@ import argparse
 import json
 import os
 import re
+import statistics
 import sys
 import time
 
@@ -67,6 +68,30 @@ WORKLOADS = {
                   desc="random-64x4096: 64 seeded positions x 4096 random playouts"),
 }
 POS_SEED, PLAY_SEED = 2016, 12345
+
+
+def host_call_overheads(eng) -> list:
+    """Host-side time above the kernel of mi_run_playouts calls of MCTS batch sizes (host buffers
+    in and out): the median over ten calls after two, per batch."""
+    import numpy as np
+    pos = eng.start_positions(POS_SEED + 1, 2048, 0, 30, False)
+    batches = [("512 positions x 1", pos[:512], 1, None),
+               ("2048 positions x 1-3 (4095 playouts, uneven counts as an MCTS batch)", pos,
+                None, (np.arange(2048) % 3 + 1).astype(np.uint32)),
+               ("1024 positions x 64", pos[:1024], 64, None)]
+    out = []
+    for name, p, per_pos, counts in batches:
+        rows = []
+        for e in range(12):
+            st = eng.run_playouts(p, per_pos=per_pos, counts=counts, seed=PLAY_SEED,
+                                  epoch=7000 + e).stats
+            if e >= 2:
+                rows.append((st["total_ms"], st["kernel_ms"]))
+        out.append({"batch": name, "playouts": int(st["playouts"]),
+                    "total_ms": round(statistics.median(r[0] for r in rows), 4),
+                    "kernel_ms": round(statistics.median(r[1] for r in rows), 4),
+                    "overhead_us": round(1e3 * statistics.median(r[0] - r[1] for r in rows), 1)})
+    return out
 
 
 def read_traffic(workload: str):
@@ -407,13 +432,22 @@ def main() -> int:
                                 "clock_mhz": info["clock_khz"] / 1e3}},
     }
     # the same batch through mi_run_playouts with host buffers (positions in, winners out, over
-    # PCIe): reported beside the figure, never as it
-    for e in range(3):  # the first call with host buffers also sets up HIP's staging copies
-        hb = eng.run_playouts(pos, per_pos=cfg["per_pos"], seed=PLAY_SEED, epoch=5000 + e,
-                              policy=cfg["policy"])
-    line["host_buffers"] = {"total_ms": round(hb.stats["total_ms"], 4),
-                            "kernel_ms": round(hb.stats["kernel_ms"], 4),
-                            "playouts_per_s": round(total / (hb.stats["total_ms"] / 1e3), 1)}
+    # PCIe): reported beside the figure, never as it.  Two calls set up the context's packed
+    # batch buffers and HIP's copy paths; the medians of the next ten calls are reported.
+    calls = []
+    for e in range(12):
+        st = eng.run_playouts(pos, per_pos=cfg["per_pos"], seed=PLAY_SEED, epoch=5000 + e,
+                              policy=cfg["policy"]).stats
+        if e >= 2:
+            calls.append((st["total_ms"], st["kernel_ms"]))
+    hb_total = statistics.median(c[0] for c in calls)
+    line["host_buffers"] = {"total_ms": round(hb_total, 4),
+                            "kernel_ms": round(statistics.median(c[1] for c in calls), 4),
+                            "overhead_ms": round(statistics.median(c[0] - c[1] for c in calls), 4),
+                            "calls": len(calls),
+                            "playouts_per_s": round(total / (hb_total / 1e3), 1)}
+    if args.workload == "random":
+        line["host_buffers"]["mcts_sized"] = host_call_overheads(eng)
     # the HBM roofline beside the VALU one: measured bytes per launch over the kernel time
     if traffic:
         line["roofline"]["hbm"] = {"achieved": round(traffic / avg_kernel_s / 1e9, 3),

@@ -50,9 +50,15 @@ struct mi_ctx {
   bool ordered = false;
   std::vector<uint32_t> host_order, host_ord_offs;
   std::vector<mi_pos> host_ord_pos;
-  // pinned staging of mi_run_playouts: positions | offsets on the way in, work words | winners
-  // on the way back.  Rewritten only while the stream is idle.
+  // pinned staging of mi_stage and of batches too large for the packed route: positions |
+  // offsets on the way in, work words | winners on the way back.  Rewritten only while the
+  // stream is idle.
   gpuai::HostBuf h_in, h_back;
+  // the packed batch of mi_run_playouts (inputs | work words | winners in one device block and
+  // its pinned mirror, one copy each way); used and released only by that route, which waits
+  // for its stream before it returns
+  gpuai::DevBuf batch;
+  gpuai::HostBuf h_batch;
   // staged batch
   uint32_t staged_npos = 0, staged_per_pos = 0, staged_total = 0;
   bool staged = false, staged_offs = false;
