@@ -250,6 +250,36 @@ def test_longest_first_order_uneven_counts(engine, oracle, golden, policy):
     assert res.stats["plies"] == int(tr["plies"].sum(dtype=np.uint64))
 
 
+@pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC_NOISY])
+def test_ordered_batch_with_keys_and_tallies(engine, oracle, golden, policy):
+    """A batch large enough for the longest-first hand-out, with stream keys, uneven counts and
+    zeros: keyed winners against the oracle, and per-position tallies of the unkeyed batch
+    against its winners (the tally kernel reads the offsets in position order while the
+    playouts were handed out in rank order)."""
+    pos = np.concatenate([golden["positions"]["open1024"], golden["positions"]["mid1024"]])
+    rng = np.random.default_rng(29 + policy)
+    counts = rng.integers(0, 96, len(pos)).astype(np.uint32)
+    counts[::5] = 0
+    keys = rng.integers(0, 2**32, len(pos), dtype=np.uint64).astype(np.uint32)
+    total = int(counts.sum())
+    assert total >= 256 * engine.device_info()["compute_units"]
+    announce(f"ordered batch: keys and tallies, {total} playouts, policy {policy}")
+    got = engine.run_playouts(pos, counts=counts, seed=13, epoch=6, policy=policy, keys=keys)
+    w, _ = oracle.run_playouts(pos, counts=counts, seed=13, epoch=6, policy=policy, threads=16,
+                               traces=False, keys=keys)
+    assert np.array_equal(got.winners, w)
+    plain = engine.run_playouts(pos, counts=counts, seed=13, epoch=6, policy=policy)
+    assert not np.array_equal(plain.winners, got.winners)
+    tallies, st = engine.run_tally(pos, counts=counts, seed=13, epoch=6, policy=policy)
+    assert st["playouts"] == total
+    offs = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
+    expect = np.zeros((len(pos), 3), dtype=np.uint32)
+    for i in range(len(pos)):
+        seg = plain.winners[offs[i]:offs[i + 1]]
+        expect[i] = [(seg == -1).sum(), (seg == 0).sum(), (seg == 1).sum()]
+    assert np.array_equal(tallies, expect)
+
+
 def test_batch_beyond_pinned_staging(engine, oracle, golden):
     """A batch whose winners exceed the 64 MiB of pinned staging (copied straight to the
     caller's memory) returns what the same positions give in a small, staged call."""
