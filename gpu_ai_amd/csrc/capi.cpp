@@ -670,6 +670,8 @@ int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin
   ctx->last_error.clear();
   if (n == 0) return MI_OK;
   if (!out || kmax < kmin) return fail(ctx, MI_ERR_INVALID_ARGUMENT, "null output or kmax < kmin");
+  if (kmax - kmin == 0xFFFFFFFFu)  // the count of ply numbers, kmax - kmin + 1, must fit 32 bits
+    return fail(ctx, MI_ERR_INVALID_ARGUMENT, "kmax - kmin + 1 exceeds 2^32 - 1");
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MI_ERR_DEVICE, "hipSetDevice failed");
   int rc;
   if ((rc = ensure(ctx, ctx->pos, sizeof(mi_pos) * (size_t)n))) return rc;

@@ -115,35 +115,52 @@ CK_HD double recip(double dn) {
 
 // Exact r mod n for 1 <= n.  On the device: the refined reciprocal, a floor, an exact fused
 // remainder, and one correction each way (r / n is estimated to far better than one unit, so the
-// floor is off by at most one).
+// floor is off by at most one).  The remainder r - q*n is an integer in (-n, 2n), exact in double
+// precision, and is corrected there: as a 32-bit integer it would overflow for n >= 2^31.
 CK_HD uint32_t mod_u32(uint32_t r, uint32_t n) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  double dn = (double)n, dr = (double)r;
-  double q = __builtin_floor(dr * recip(dn));
-  int32_t rem = (int32_t)__builtin_fma(-q, dn, dr);
-  rem += (rem < 0) ? (int32_t)n : 0;
-  rem -= (rem >= (int32_t)n) ? (int32_t)n : 0;
-  return (uint32_t)rem;
+  const double dn = (double)n, dr = (double)r;
+  const double q = __builtin_floor(dr * recip(dn));
+  double m = __builtin_fma(-q, dn, dr);
+  m = m < 0.0 ? m + dn : (m >= dn ? m - dn : m);
+  return (uint32_t)m;
 #else
   return r % n;
 #endif
 }
 
-// Exact quotient and remainder of r / n for 1 <= n (same estimate-and-correct scheme).
+// Exact r mod n for 1 <= n < 2^18 (move and tie counts, the per-ply draw), with no correction
+// step.  After two Newton steps from a seed of at least 13 bits the reciprocal x has a relative
+// error below 2^-52, so |r*x - r/n| plus the rounding of the fused sum stays under
+// (2^32 / n)(2^-52 + 2^-53) = 1.5 * 2^-20 / n.  Adding 2^-19 lifts fma(r, x, 2^-19) to at least
+// r/n; it stays below floor(r/n) + 1 because frac(r/n) <= 1 - 1/n and
+// 1.5 * 2^-20 / n + 2^-19 < 1/n while n < 2^18.  The floor is then the exact quotient, and the
+// fused remainder r - q*n is exact.
+CK_HD uint32_t mod_small(uint32_t r, uint32_t n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double dn = (double)n, dr = (double)r;
+  const double q = __builtin_floor(__builtin_fma(dr, recip(dn), 0x1p-19));
+  return (uint32_t)__builtin_fma(-q, dn, dr);
+#else
+  return r % n;
+#endif
+}
+
+// Exact quotient and remainder of r / n for 1 <= n (same estimate-and-correct scheme, the
+// correction in double precision).
 CK_HD void divmod_u32(uint32_t r, uint32_t n, uint32_t &q, uint32_t &rem) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  double dn = (double)n, dr = (double)r;
+  const double dn = (double)n, dr = (double)r;
   double qd = __builtin_floor(dr * recip(dn));
-  int32_t m = (int32_t)__builtin_fma(-qd, dn, dr);
-  uint32_t qq = (uint32_t)(int64_t)qd;
-  if (m < 0) {
-    m += (int32_t)n;
-    qq -= 1u;
-  } else if (m >= (int32_t)n) {
-    m -= (int32_t)n;
-    qq += 1u;
+  double m = __builtin_fma(-qd, dn, dr);
+  if (m < 0.0) {
+    m += dn;
+    qd -= 1.0;
+  } else if (m >= dn) {
+    m -= dn;
+    qd += 1.0;
   }
-  q = qq;
+  q = (uint32_t)qd;
   rem = (uint32_t)m;
 #else
   q = r / n;
@@ -881,7 +898,7 @@ CK_HD PlyResult ply_random_given(Lane &L, const Gen &g, uint32_t r, uint32_t *li
     res.winner = L.counter < 50u ? (int32_t)(L.turn ^ 1u) : -1;
     return res;
   }
-  uint32_t k = mod_u32(r, n);
+  uint32_t k = mod_small(r, n);
   res.chosen = k;
   res.over = 0u;
   res.winner = 0;
@@ -920,7 +937,7 @@ CK_HD PlyResult ply_random_trees(Lane &L, const Gen &g, uint32_t r, uint32_t *li
     res.winner = -1;
     return res;
   }
-  const uint32_t k = mod_u32(r, n);
+  const uint32_t k = mod_small(r, n);
   res.chosen = k;
   const Choice c = tree_select(L, g, L.turn ? n - 1u - k : k, m01, m23);
   apply(L, c.from, c.to, c.removed);
@@ -959,7 +976,7 @@ CK_HD PlyResult ply_random_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool &d
     res.winner = L.counter < 50u ? (int32_t)(L.turn ^ 1u) : -1;
     return res;
   }
-  const uint32_t k = mod_u32(r, n);
+  const uint32_t k = mod_small(r, n);
   res.chosen = k;
   res.over = 0u;
   res.winner = 0;
@@ -1073,7 +1090,7 @@ CK_HD PlyResult heuristic_body(Lane &L, const Gen &g, uint32_t r, uint32_t *list
     res.winner = L.counter < 50u ? (int32_t)(L.turn ^ 1u) : -1;
     return res;
   }
-  uint32_t k = mod_u32(r, ties);
+  uint32_t k = mod_small(r, ties);
   uint32_t km = L.turn ? ties - 1u - k : k;
   res.over = 0u;
   res.winner = 0;
@@ -1353,7 +1370,7 @@ CK_HD PlyResult noisy_body(Lane &L, const Gen &g, uint32_t r, uint32_t *list_dig
       c = slot_choice(L, g, bsq, bsl);
       km = nb.pick;
     } else {
-      const uint32_t k = inf ? 0u : mod_u32(r, n);  // reference index
+      const uint32_t k = inf ? 0u : mod_small(r, n);  // reference index
       km = L.turn ? n - 1u - k : k;
       uint32_t sq, sl;
       select_slot(g.slot, km, sq, sl);
@@ -1398,7 +1415,7 @@ CK_HD PlyResult noisy_body(Lane &L, const Gen &g, uint32_t r, uint32_t *list_dig
     n = nb.m;
     km = nb.pick;
     if (single) {
-      const uint32_t k = fd == 0u ? 0u : mod_u32(r, n);
+      const uint32_t k = fd == 0u ? 0u : mod_small(r, n);
       km = L.turn ? n - 1u - k : k;
       c = select_walk(L, g, km);
     }
@@ -1453,7 +1470,7 @@ CK_HD PlyResult ply_noisy_fast(Lane &L, uint32_t r, uint32_t *list_dig, bool &de
     res.winner = L.counter < 50u ? (int32_t)(L.turn ^ 1u) : -1;
     return res;
   }
-  const uint32_t k = inf ? 0u : mod_u32(r, n);  // reference index
+  const uint32_t k = inf ? 0u : mod_small(r, n);  // reference index
   uint32_t sq, sl;
   select_slot(g.slot, L.turn ? n - 1u - k : k, sq, sl);
   const Choice c = slot_choice(L, g, sq, sl);

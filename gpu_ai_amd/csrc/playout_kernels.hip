@@ -400,7 +400,9 @@ __global__ void __launch_bounds__(256) k_divmod(const uint32_t *__restrict__ r,
   uint32_t qq, rr;
   ck::divmod_u32(r[i], n[i], qq, rr);
   q[i] = qq;
-  rem[i] = ck::mod_u32(r[i], n[i]) == rr ? rr : 0xFFFFFFFFu;
+  // the remainder, or all ones when the general or (below 2^18) the per-ply remainder differs
+  const bool small_ok = n[i] >= (1u << 18) || ck::mod_small(r[i], n[i]) == rr;
+  rem[i] = ck::mod_u32(r[i], n[i]) == rr && small_ok ? rr : 0xFFFFFFFFu;
 }
 
 // Plain grid-stride copy of 16-byte words (traffic-counter calibration).

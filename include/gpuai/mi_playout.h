@@ -167,8 +167,9 @@ int mi_gen_moves(mi_ctx *ctx, const mi_pos *pos, uint32_t npos, mi_move *moves, 
  * from the initial position (r = word 0 of Philox(key (seed, i), counter (0, a, 1, 1))), each ply
  * choosing move (r_t mod n_t) with r_t = word (t & 3) of Philox(key (seed, i),
  * counter (t >> 2, a, 1, 0)), stopping early when the game is over.  With reject_terminal, a
- * finished position is redrawn with attempt a + 1.  Follows genRandomStates
- * (src/driver.cpp:75-104), whose shared engine is not reproducible. */
+ * finished position is redrawn with attempt a + 1.  kmin <= kmax, and kmax - kmin + 1 must fit
+ * 32 bits.  Follows genRandomStates (src/driver.cpp:75-104), whose shared engine is not
+ * reproducible. */
 int mi_gen_start_positions(mi_ctx *ctx, uint32_t seed, uint32_t n, uint32_t kmin, uint32_t kmax,
                            uint32_t reject_terminal, mi_pos *out);
 
@@ -269,9 +270,11 @@ int mi_run_playouts_noalloc(mi_ctx *ctx, const mi_pos *pos, uint32_t npos,
                             uint32_t epoch, uint32_t policy, int8_t *winners, uint32_t *status,
                             void *stream);
 
-/* Diagnostic: runs the device's exact quotient/remainder routine (used to pick r mod n for
- * every ply) on caller data, q[i] = r[i] / n[i] and rem[i] = r[i] % n[i], n[i] >= 1.  No
- * counterpart in the reference, whose draws use the % operator on the host (src/player.cpp:15). */
+/* Diagnostic: runs the device's exact quotient/remainder routines on caller data, q[i] =
+ * r[i] / n[i] and rem[i] = r[i] % n[i], n[i] >= 1; rem[i] is all ones where the general
+ * remainder, or for n[i] < 2^18 the one that picks r mod n on every ply, disagrees with the
+ * quotient routine.  No counterpart in the reference, whose draws use the % operator on the host
+ * (src/player.cpp:15). */
 int mi_check_divmod(mi_ctx *ctx, const uint32_t *r, const uint32_t *n, uint32_t count,
                     uint32_t *q, uint32_t *rem);
 

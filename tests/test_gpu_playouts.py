@@ -27,13 +27,17 @@ def trace_mismatches(res, tr) -> int:
 
 
 def test_divmod_exact(engine):
-    announce("mi_check_divmod: n in [1, 1100], boundary and random r")
+    announce("mi_check_divmod: n in [1, 1100] and large n, boundary and random r")
     rng = np.random.default_rng(1)
     ns, rs = [], []
-    for n in range(1, 1101):
+    large = [2**17 - 1, 2**17, 2**18 - 1, 2**18, 2**20 + 7, 2**31, 2**32 - 1]
+    for n in list(range(1, 1101)) + large:
         base = [0, 1, n - 1, n, n + 1, 2**31 - 1, 2**31, 2**32 - 1, 2**32 - n, 2**32 - n - 1]
         q = (2**32 - 1) // n
         base += [q * n - 1, q * n, (q // 2) * n, (q // 2) * n - 1]
+        # exact multiples and the values just below them (r/n an integer, or 1/n short of one)
+        k = rng.integers(1, q + 1, 32, dtype=np.uint64)
+        base += [int(x) * n for x in k] + [int(x) * n - 1 for x in k]
         r = np.array([x % 2**32 for x in base] + list(rng.integers(0, 2**32, 64, dtype=np.uint64)),
                      dtype=np.uint64)
         rs.append(r)
