@@ -446,7 +446,9 @@ def main() -> int:
                             "overhead_ms": round(statistics.median(c[0] - c[1] for c in calls), 4),
                             "calls": len(calls),
                             "playouts_per_s": round(total / (hb_total / 1e3), 1)}
-    if args.workload == "random":
+    # MCTS-sized calls in autotuned runs only: a run with a fixed shape (as profiled runs are)
+    # launches the 1M-playout batch alone
+    if args.workload == "random" and not args.tuning:
         line["host_buffers"]["mcts_sized"] = host_call_overheads(eng)
     # the HBM roofline beside the VALU one: measured bytes per launch over the kernel time
     if traffic:
