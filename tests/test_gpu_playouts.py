@@ -372,6 +372,19 @@ def test_device_start_positions_match_reference_golden(engine, golden):
     assert mid1024.tobytes() == golden["positions"]["mid1024"].tobytes()
 
 
+def test_start_positions_reject_bad_ply_ranges(engine):
+    """kmax < kmin, and a range of 2^32 ply numbers (kmax - kmin + 1 wraps to zero), are refused
+    before anything runs; the widest range that fits still works."""
+    from gpu_ai_amd.engine import MI_ERR_INVALID_ARGUMENT, MiError
+    announce("mi_gen_start_positions with bad ply ranges")
+    for kmin, kmax in ((5, 4), (0, 2**32 - 1)):
+        with pytest.raises(MiError) as e:
+            engine.start_positions(3, 8, kmin, kmax)
+        assert e.value.code == MI_ERR_INVALID_ARGUMENT
+    wide = engine.start_positions(3, 8, 1, 2**32 - 1)  # 2^32 - 1 ply numbers; games end first
+    assert wide.shape == (8,)
+
+
 @pytest.mark.parametrize("refill", [REFILL_QUEUE, REFILL_STATIC])
 @pytest.mark.parametrize("policy", [POLICY_RANDOM, POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY])
 def test_graph_replays_match_eager(engine, golden, policy, refill):
