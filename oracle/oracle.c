@@ -698,6 +698,7 @@ int or_run_playouts_keyed(const or_pos *pos, uint32_t npos, const uint32_t *coun
  * engine: k uniform in [kmin, kmax] random plies, stopping early when the game is over. */
 int or_gen_start_positions(uint32_t seed, uint32_t n, uint32_t kmin, uint32_t kmax,
                            int reject_terminal, or_pos *out) {
+  if (kmax < kmin || kmax - kmin == 0xFFFFFFFFu) return -2;  /* kmax - kmin + 1 must fit 32 bits */
   int failed = 0;
 #pragma omp parallel for schedule(dynamic, 16)
   for (int64_t i = 0; i < (int64_t)n; i++) {

@@ -122,7 +122,8 @@ int or_run_playouts_keyed(const or_pos *pos, uint32_t npos, const uint32_t *coun
 /* ---- start positions ----
  * k = kmin + (r mod (kmax-kmin+1)) random plies from the start position, stopping early at
  * game over (src/driver.cpp:90-98).  With reject_terminal, a finished position is redrawn with
- * the next attempt counter. */
+ * the next attempt counter.  Returns -2 for kmax < kmin or a range of 2^32 ply numbers, -1 when
+ * reject_terminal found no unfinished position in 4096 attempts. */
 int or_gen_start_positions(uint32_t seed, uint32_t n, uint32_t kmin, uint32_t kmax,
                            int reject_terminal, or_pos *out);
 

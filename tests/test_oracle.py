@@ -116,6 +116,10 @@ def test_start_positions_golden(oracle, golden):
     assert mid1024.tobytes() == golden["positions"]["mid1024"].tobytes()
     # the midgame set rejects finished positions
     assert not any(oracle.is_game_over(p) for p in mid1024)
+    # kmax < kmin and a 2^32-wide ply range are refused (the range would wrap to zero)
+    for kmin, kmax in ((5, 4), (0, 2**32 - 1)):
+        with pytest.raises(RuntimeError):
+            oracle.start_positions(3, 4, kmin, kmax)
 
 
 def playout_sets(golden):
