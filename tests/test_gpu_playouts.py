@@ -284,6 +284,44 @@ def test_ordered_batch_with_keys_and_tallies(engine, oracle, golden, policy):
     assert np.array_equal(tallies, expect)
 
 
+def test_packed_batches_alternate_layouts(engine, oracle, golden):
+    """Calls on one context whose packed batches differ in layout, one after another: ordered
+    and in index order, with and without stream keys and uneven counts, larger and smaller than
+    the last, with tallies and traces between them.  A section left over from an earlier call's
+    layout would show up in some call's winners, traces or tallies against the oracle."""
+    opens, mids = golden["positions"]["open1024"], golden["positions"]["mid1024"]
+    rng = np.random.default_rng(41)
+    keys = rng.integers(0, 2**32, 1024, dtype=np.uint64).astype(np.uint32)
+    uneven = rng.integers(0, 200, 1024).astype(np.uint32)
+    calls = [
+        ("ordered, keyed, 1024 x 96", mids, dict(per_pos=96, keys=keys)),
+        ("index order, 40 positions, uneven", opens[:40],
+         dict(counts=uneven[:40] % 17)),
+        ("ordered, uneven counts", opens, dict(counts=uneven)),
+        ("index order, keyed, 8 x 3", mids[:8], dict(per_pos=3, keys=keys[:8])),
+        ("ordered, 512 x 160", mids[:512], dict(per_pos=160)),
+    ]
+    for step, (name, pos, kw) in enumerate(calls):
+        policy = (POLICY_RANDOM, POLICY_HEURISTIC, POLICY_HEURISTIC_NOISY)[step % 3]
+        announce(f"packed batch {step}: {name}, policy {policy}")
+        got = engine.run_playouts(pos, seed=19, epoch=step, policy=policy, **kw)
+        w, _ = oracle.run_playouts(pos, seed=19, epoch=step, policy=policy, threads=16,
+                                   traces=False, **kw)
+        assert np.array_equal(got.winners, w), name
+        if "keys" not in kw:
+            tallies, _ = engine.run_tally(pos, seed=19, epoch=step, policy=policy, **kw)
+            counts = kw["counts"] if "counts" in kw else np.full(len(pos), kw["per_pos"])
+            offs = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
+            for i in range(0, len(pos), 37):
+                seg = w[offs[i]:offs[i + 1]]
+                assert list(tallies[i]) == [(seg == -1).sum(), (seg == 0).sum(),
+                                            (seg == 1).sum()], (name, i)
+        if step == 1:
+            res = engine.run_traced(pos, seed=19, epoch=step, policy=policy, **kw)
+            _, tr = oracle.run_playouts(pos, seed=19, epoch=step, policy=policy, **kw)
+            assert trace_mismatches(res, tr) == 0
+
+
 def test_batch_beyond_pinned_staging(engine, oracle, golden):
     """A batch whose winners exceed the 64 MiB of pinned staging (copied straight to the
     caller's memory) returns what the same positions give in a small, staged call."""
