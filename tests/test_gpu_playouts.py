@@ -410,17 +410,19 @@ def test_device_start_positions_match_reference_golden(engine, golden):
     assert mid1024.tobytes() == golden["positions"]["mid1024"].tobytes()
 
 
-def test_start_positions_reject_bad_ply_ranges(engine):
+def test_start_positions_reject_bad_ply_ranges(engine, oracle):
     """kmax < kmin, and a range of 2^32 ply numbers (kmax - kmin + 1 wraps to zero), are refused
-    before anything runs; the widest range that fits still works."""
+    before anything runs.  Single ply counts and ranges of 2^31 ply numbers or more (the draw
+    r mod (kmax - kmin + 1) with a divisor past 2^31) give the oracle's positions."""
     from gpu_ai_amd.engine import MI_ERR_INVALID_ARGUMENT, MiError
-    announce("mi_gen_start_positions with bad ply ranges")
+    announce("mi_gen_start_positions with bad and extreme ply ranges")
     for kmin, kmax in ((5, 4), (0, 2**32 - 1)):
         with pytest.raises(MiError) as e:
             engine.start_positions(3, 8, kmin, kmax)
         assert e.value.code == MI_ERR_INVALID_ARGUMENT
-    wide = engine.start_positions(3, 8, 1, 2**32 - 1)  # 2^32 - 1 ply numbers; games end first
-    assert wide.shape == (8,)
+    for kmin, kmax in ((0, 0), (7, 7), (3, 2**31 + 8), (1, 2**32 - 1)):
+        got = engine.start_positions(3, 256, kmin, kmax)  # games end long before k plies
+        assert got.tobytes() == oracle.start_positions(3, 256, kmin, kmax).tobytes(), (kmin, kmax)
 
 
 @pytest.mark.parametrize("refill", [REFILL_QUEUE, REFILL_STATIC])
