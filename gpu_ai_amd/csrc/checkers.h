@@ -99,9 +99,11 @@ constexpr uint32_t TAG_START = 1u;
 constexpr uint32_t TAG_SEARCH = 2u;
 constexpr uint32_t TAG_MLP = 3u;
 
-// Reciprocal of n in double precision: the hardware estimate (v_rcp_f64, a seed of roughly
-// single precision) refined by two Newton-Raphson steps, each of which squares the relative
-// error, so the result is within a few units of 2^-52 of 1/n.
+// Reciprocal of n in double precision: a seed (on the device the hardware estimate v_rcp_f64, of
+// roughly single precision; on the host 1/n) refined by two Newton-Raphson steps, each of which
+// squares the relative error, so the result is within a few units of 2^-52 of 1/n.  The
+// remainder routines below run the same code on the host and the device, so the host's checks
+// of the lane logic exercise them on every ply.
 CK_HD double recip(double dn) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double x = __builtin_amdgcn_rcp(dn);
@@ -113,20 +115,16 @@ CK_HD double recip(double dn) {
   return x;
 }
 
-// Exact r mod n for 1 <= n.  On the device: the refined reciprocal, a floor, an exact fused
-// remainder, and one correction each way (r / n is estimated to far better than one unit, so the
-// floor is off by at most one).  The remainder r - q*n is an integer in (-n, 2n), exact in double
-// precision, and is corrected there: as a 32-bit integer it would overflow for n >= 2^31.
+// Exact r mod n for 1 <= n: the refined reciprocal, a floor, an exact fused remainder, and one
+// correction each way (r / n is estimated to far better than one unit, so the floor is off by at
+// most one).  The remainder r - q*n is an integer in (-n, 2n), exact in double precision, and is
+// corrected there: as a 32-bit integer it would overflow for n >= 2^31.
 CK_HD uint32_t mod_u32(uint32_t r, uint32_t n) {
-#if defined(__HIP_DEVICE_COMPILE__)
   const double dn = (double)n, dr = (double)r;
   const double q = __builtin_floor(dr * recip(dn));
   double m = __builtin_fma(-q, dn, dr);
   m = m < 0.0 ? m + dn : (m >= dn ? m - dn : m);
   return (uint32_t)m;
-#else
-  return r % n;
-#endif
 }
 
 // Exact r mod n for 1 <= n < 2^18 (move and tie counts, the per-ply draw), with no correction
@@ -137,19 +135,14 @@ CK_HD uint32_t mod_u32(uint32_t r, uint32_t n) {
 // 1.5 * 2^-20 / n + 2^-19 < 1/n while n < 2^18.  The floor is then the exact quotient, and the
 // fused remainder r - q*n is exact.
 CK_HD uint32_t mod_small(uint32_t r, uint32_t n) {
-#if defined(__HIP_DEVICE_COMPILE__)
   const double dn = (double)n, dr = (double)r;
   const double q = __builtin_floor(__builtin_fma(dr, recip(dn), 0x1p-19));
   return (uint32_t)__builtin_fma(-q, dn, dr);
-#else
-  return r % n;
-#endif
 }
 
 // Exact quotient and remainder of r / n for 1 <= n (same estimate-and-correct scheme, the
 // correction in double precision).
 CK_HD void divmod_u32(uint32_t r, uint32_t n, uint32_t &q, uint32_t &rem) {
-#if defined(__HIP_DEVICE_COMPILE__)
   const double dn = (double)n, dr = (double)r;
   double qd = __builtin_floor(dr * recip(dn));
   double m = __builtin_fma(-qd, dn, dr);
@@ -162,10 +155,6 @@ CK_HD void divmod_u32(uint32_t r, uint32_t n, uint32_t &q, uint32_t &rem) {
   }
   q = (uint32_t)qd;
   rem = (uint32_t)m;
-#else
-  q = r / n;
-  rem = r % n;
-#endif
 }
 
 // ------------------------------------------------------------------------------------------
