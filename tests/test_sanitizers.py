@@ -6,7 +6,9 @@ Built here from the sources:
     while the game thread calls getMove and move under the tree mutex, and concurrent GameTree
     searches with OpenMP selection at the root, leaves scored by a host-only driver;
   * tests/lane_logic_check.cpp (the kernels' per-lane game logic, compiled for the host, against
-    the oracle) and tests/tree_policy_check.cpp under ASan + UBSan.
+    the oracle) and tests/tree_policy_check.cpp under ASan + UBSan;
+  * the oracle's C restatement driven by tests/oracle_sanitize_check.c, built by gcc under
+    ASan + UBSan and plainly, with the same digest of results required from both.
 UBSan reports are fatal (-fno-sanitize-recover).  The OpenMP runtime is not instrumented: its
 internal races are excluded (ignore_noninstrumented_modules) and the one allocation libomp keeps
 to the end of the process is a suppressed leak.  Sanitizer flags go to the host compilation only.
@@ -115,3 +117,29 @@ def test_lane_logic_under_asan_ubsan(workdir, oracle):
                  ASAN, [oracle_so, "-Wl,-rpath," + os.path.dirname(oracle_so)] + ASAN_LINK)
     out = _run([exe, "96", "4"], _asan_env(workdir))
     assert "mismatches 0" in out
+
+
+def test_oracle_under_asan_ubsan(tmp_path):
+    """The oracle's C restatement (every entry point: start positions, move lists and moves,
+    scores, playouts of each policy with uniform and uneven counts and stream keys, choice
+    frequencies, noise draws) under AddressSanitizer + UndefinedBehaviorSanitizer, built with gcc,
+    and the same digest of its results as a plain -O2 build."""
+    gcc = shutil.which("gcc")
+    if not gcc:
+        pytest.skip("gcc not available")
+    src = [os.path.join(ROOT, "oracle", "oracle.c"),
+           os.path.join(ROOT, "tests", "oracle_sanitize_check.c")]
+    common = ["-std=c11", "-fopenmp", "-I" + os.path.join(ROOT, "oracle")]
+    digests = []
+    for name, flags in (("plain", ["-O2"]),
+                        ("sanitized", ["-O1", "-g", "-fno-omit-frame-pointer",
+                                       "-fsanitize=address,undefined",
+                                       "-fno-sanitize-recover=all"])):
+        exe = str(tmp_path / name)
+        proc = subprocess.run([gcc, *common, *flags, *src, "-lm", "-o", exe],
+                              stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        assert proc.returncode == 0, proc.stdout
+        out = _run([exe], {"ASAN_OPTIONS": "detect_leaks=1",
+                           "UBSAN_OPTIONS": "print_stacktrace=1:halt_on_error=1"}, timeout=300)
+        digests.append(out.strip().splitlines()[-1])
+    assert digests[0].startswith("digest ") and digests[0] == digests[1], digests
